@@ -1,0 +1,114 @@
+"""ctypes binding of librbmd_force.so (the C ABI declared in include/rbmd_force.h).
+
+torch is imported before the library is loaded: torch ships a HIP runtime with
+the same SONAME (libamdhip64.so.7), and loading it first makes the dynamic
+linker bind this library to that one runtime, so torch device pointers and
+streams are valid here.  A missing library is an ImportError -- there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch  # noqa: F401  (must precede the CDLL load, see module docstring)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librbmd_force.so")
+
+RBMD_OK = 0
+RBMD_ERR_INVALID = -1
+RBMD_ERR_HIP = -2
+RBMD_ERR_BOX = -3
+RBMD_ERR_PRECISION = -4
+RBMD_ERR_STATE = -5
+RBMD_ERR_CAPACITY = -6
+
+KSPACE_NONE = 0
+KSPACE_RBE = 1
+KSPACE_EWALD = 2
+RBMD_EV = 1
+
+
+class ForceParams(C.Structure):
+    _fields_ = [
+        ("box_lo", C.c_double * 3),
+        ("box_len", C.c_double * 3),
+        ("rc_lj", C.c_double),
+        ("rc_coul", C.c_double),
+        ("skin", C.c_double),
+        ("alpha", C.c_double),
+        ("qqr2e", C.c_double),
+        ("ntypes", C.c_int32),
+        ("eps", C.POINTER(C.c_double)),
+        ("sigma", C.POINTER(C.c_double)),
+        ("precision", C.c_int32),
+        ("kspace", C.c_int32),
+        ("p_max", C.c_int32),
+        ("ewald_tol", C.c_double),
+        ("list_capacity", C.c_int32),
+    ]
+
+
+class EnergyVirial(C.Structure):
+    _fields_ = [
+        ("e_lj", C.c_double),
+        ("e_real", C.c_double),
+        ("e_self", C.c_double),
+        ("e_kspace", C.c_double),
+        ("virial", C.c_double * 6),
+    ]
+
+
+# name -> (restype, argtypes); the symbol test checks this table against the header
+_P = C.c_void_p
+_SIGS = {
+    "rbmd_force_create": (C.c_int, [C.POINTER(ForceParams), C.c_int64, C.POINTER(C.c_void_p)]),
+    "rbmd_force_destroy": (None, [_P]),
+    "rbmd_force_last_error": (C.c_char_p, [_P]),
+    "rbmd_force_set_atoms": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
+    "rbmd_force_set_kvectors": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "rbmd_force_sample_kvectors": (C.c_int, [_P, C.c_int32, C.c_uint64, C.c_uint64, _P]),
+    "rbmd_force_compute": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, C.c_int32, _P]),
+    "rbmd_force_get_energy_virial": (C.c_int, [_P, C.POINTER(EnergyVirial), _P]),
+    "rbmd_md_run": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                              C.c_double, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32, _P]),
+    "rbmd_force_get_neighbors": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "rbmd_force_get_kspace": (C.c_int, [_P, C.POINTER(C.c_int32), _P, _P, _P, _P]),
+    "rbmd_force_get_stats": (C.c_int, [_P, _P]),
+    "rbmd_force_set_timing": (C.c_int, [_P, C.c_int32]),
+    "rbmd_force_get_timing": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int64)]),
+    "rbmd_util_copy": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "rbmd_philox4x32": (None, [_P, _P, _P]),
+    "rbmd_sys_fcc": (C.c_int, [C.c_int32, C.c_double, _P, _P, _P, C.POINTER(C.c_double)]),
+    "rbmd_sys_random": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_uint64, _P, _P, _P]),
+    "rbmd_sys_charges": (C.c_int, [C.c_int64, C.c_uint64, _P]),
+    "rbmd_sys_velocities": (C.c_int, [C.c_int64, C.c_double, _P, C.c_uint64, _P, _P, _P]),
+    "rbmd_sample_kvectors_host": (C.c_int, [C.c_double, _P, C.c_int32, C.c_uint64, C.c_uint64,
+                                            _P, C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load (once) and return the HIP library; raise ImportError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m copy_gpu_amd.build` "
+            "(the HIP library is required; there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)

@@ -1,0 +1,654 @@
+// api.hip -- C ABI of the MI355X charged-LJ force engine (declared in include/rbmd_force.h).
+//
+// Host orchestration of one force evaluation (the work of Force::Execute,
+// rbmd-gpu src/force/src/lj_cut_coul_kspace_force.cpp:73-78, and of
+// ExecutionNVT::ComputeAllForce, rbmd/framework/run/ExecutionNVT.cpp:185-222):
+//   check -> [bin, scan, scatter, cellsort, list]* -> gather -> [rho, rho_final]
+//   -> pair -> [kforce] -> [ev_final]
+// (* the build kernels run every call but exit at once unless the displacement
+// check or the host asked for a rebuild, so a step needs no host round trip and
+// can be captured in a HIP graph).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/rbmd_force.h"
+#include "state.h"
+
+namespace rbmd {
+
+void build_kcdf(double alpha, const double* L, std::vector<double>& cdf, int M[3]);
+double rbe_S(double alpha, const double* L);
+
+static const char* kPassNames[P_NUM] = {"check", "bin",  "scan",   "scatter",   "cellsort",
+                                        "list",  "gather", "rho",  "rho_final", "pair",
+                                        "kforce", "kick", "ev_final", "sample"};
+
+void timer_begin(Engine& e, int pass, hipStream_t s, hipEvent_t* ev)
+{
+    *ev = nullptr;
+    if (!e.timing || e.capturing) return;
+    hipEvent_t a;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    hipEventRecord(a, s);
+    *ev = a;
+}
+
+void timer_end(Engine& e, int pass, hipStream_t s, hipEvent_t a)
+{
+    if (!a) return;
+    hipEvent_t b;
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return;
+    }
+    hipEventRecord(b, s);
+    e.pending[pass].push_back({a, b});
+}
+
+static void timer_drain(Engine& e)
+{
+    for (int p = 0; p < P_NUM; ++p) {
+        for (auto& r : e.pending[p]) {
+            float ms = 0.f;
+            hipEventSynchronize(r.b);
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                e.t_sum[p] += ms;
+                e.t_cnt[p] += 1;
+            }
+            hipEventDestroy(r.a);
+            hipEventDestroy(r.b);
+        }
+        e.pending[p].clear();
+    }
+}
+
+}  // namespace rbmd
+
+using namespace rbmd;
+
+struct rbmd_force_s {
+    Engine e;
+};
+
+static thread_local std::string g_create_err;
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _st = (expr);                                                         \
+        if (_st != hipSuccess) {                                                         \
+            char _b[256];                                                                \
+            snprintf(_b, sizeof _b, "%s failed: %s", #expr, hipGetErrorString(_st));     \
+            E.err = _b;                                                                  \
+            return RBMD_ERR_HIP;                                                         \
+        }                                                                                \
+    } while (0)
+
+static int fail(Engine& E, int code, const char* msg)
+{
+    E.err = msg;
+    return code;
+}
+
+template <typename P>
+static hipError_t dalloc(P** p, size_t bytes)
+{
+    return hipMalloc((void**)p, bytes ? bytes : 16);
+}
+
+static void free_all(Engine& E)
+{
+    void* ptrs[] = {E.xr,       E.yr,    E.zr,     E.xs_ref, E.xs,        E.perm,     E.scell,
+                    E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
+                    E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
+                    E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
+                    E.ev_part,  E.ev};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (E.h_stats) hipHostFree(E.h_stats);
+    if (E.graph_exec) hipGraphExecDestroy(E.graph_exec);
+    if (E.graph) hipGraphDestroy(E.graph);
+    if (E.own_stream) hipStreamDestroy(E.own_stream);
+}
+
+static int alloc_list(Engine& E, int cap)
+{
+    if (E.nlist) hipFree(E.nlist);
+    E.nlist = nullptr;
+    E.cap = cap;
+    HIP_TRY(dalloc(&E.nlist, (size_t)E.n_max * (size_t)cap * sizeof(int32_t)));
+    return RBMD_OK;
+}
+
+static int alloc_kspace(Engine& E, int P)
+{
+    // k-vector arrays, coefficients and per-k terms for P slots; the partial
+    // structure factors are bounded by the rho geometry: blocks_x * P <=
+    // max(2048 * 64, P) for any n and P (at most 2048 / ceil(P/64) atom tiles).
+    if (P <= E.kcap && E.kpart) return RBMD_OK;
+    void* olds[] = {E.km, E.kw, E.rho, E.kcoef, E.kev, E.kpart};
+    for (void* p : olds)
+        if (p) hipFree(p);
+    E.km = nullptr;
+    E.kw = nullptr;
+    E.rho = nullptr;
+    E.kcoef = nullptr;
+    E.kev = nullptr;
+    E.kpart = nullptr;
+    const int cap = std::max(P, E.kcap);
+    E.kcap = cap;
+    HIP_TRY(dalloc(&E.km, (size_t)cap * 3 * sizeof(int32_t)));
+    HIP_TRY(dalloc(&E.kw, (size_t)cap * sizeof(double)));
+    HIP_TRY(dalloc(&E.rho, (size_t)cap * 2 * sizeof(double)));
+    HIP_TRY(dalloc(&E.kcoef, (size_t)cap * 64));
+    HIP_TRY(dalloc(&E.kev, (size_t)cap * 7 * sizeof(double)));
+    E.kpart_cap = 2 * (int64_t)std::max(2048 * 64, cap);
+    HIP_TRY(dalloc(&E.kpart, (size_t)E.kpart_cap * sizeof(double)));
+    return RBMD_OK;
+}
+
+// Half-space Ewald lattice: m with exp(-k^2/4alpha) >= tol, weight 2 exp(-k^2/4alpha).
+static void ewald_lattice(const Engine& E, std::vector<int32_t>& m, std::vector<double>& w)
+{
+    const double kmax2 = -4.0 * E.alpha * log(E.ewald_tol);
+    int M[3];
+    for (int d = 0; d < 3; ++d) M[d] = (int)floor(sqrt(kmax2) * E.box.L[d] / (2.0 * M_PI)) + 1;
+    for (int a = 0; a <= M[0]; ++a)
+        for (int b = -M[1]; b <= M[1]; ++b)
+            for (int g = -M[2]; g <= M[2]; ++g) {
+                if (a == 0 && (b < 0 || (b == 0 && g <= 0))) continue;
+                const double kx = 2 * M_PI * a / E.box.L[0], ky = 2 * M_PI * b / E.box.L[1],
+                             kz = 2 * M_PI * g / E.box.L[2];
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 > kmax2) continue;
+                m.push_back(a);
+                m.push_back(b);
+                m.push_back(g);
+                w.push_back(2.0 * exp(-k2 / (4.0 * E.alpha)));
+            }
+}
+
+extern "C" {
+
+const char* rbmd_force_last_error(rbmd_force_t h)
+{
+    if (!h) return g_create_err.c_str();
+    return h->e.err.c_str();
+}
+
+int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* out)
+{
+    g_create_err.clear();
+    if (!p || !out || n_max < 0 || n_max > (int64_t)INT32_MAX / 2) {
+        g_create_err = "invalid arguments to rbmd_force_create";
+        return RBMD_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (p->precision != 32 && p->precision != 64) {
+        g_create_err = "precision must be 32 or 64";
+        return RBMD_ERR_INVALID;
+    }
+    if (p->ntypes < 1 || p->ntypes > kMaxTypes || !p->eps || !p->sigma) {
+        g_create_err = "ntypes must be in [1, 8] with host eps/sigma arrays";
+        return RBMD_ERR_INVALID;
+    }
+    if (!(p->rc_lj > 0) || !(p->rc_coul >= 0) || !(p->skin >= 0) ||
+        (p->kspace != RBMD_KSPACE_NONE && !(p->alpha > 0)) || p->kspace < 0 || p->kspace > 2 ||
+        (p->kspace == RBMD_KSPACE_RBE && p->p_max < 1)) {
+        g_create_err = "invalid cutoff, skin, alpha, kspace or p_max";
+        return RBMD_ERR_INVALID;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (!(p->box_len[d] > 0)) {
+            g_create_err = "box lengths must be positive";
+            return RBMD_ERR_INVALID;
+        }
+    rbmd_force_t h = new rbmd_force_s();
+    Engine& E = h->e;
+    for (int d = 0; d < 3; ++d) {
+        E.box.lo[d] = p->box_lo[d];
+        E.box.L[d] = p->box_len[d];
+        E.box.invL[d] = 1.0 / p->box_len[d];
+    }
+    E.rc_lj = p->rc_lj;
+    E.rc_coul = p->rc_coul;
+    E.skin = p->skin;
+    E.alpha = p->alpha;
+    E.qqr2e = p->qqr2e;
+    E.precision = p->precision;
+    E.kspace = p->kspace;
+    E.p_max = p->p_max;
+    E.ewald_tol = p->ewald_tol > 0 ? p->ewald_tol : 1e-16;
+    E.r_list = std::max(p->rc_lj, p->rc_coul) + p->skin;
+    E.r_in = p->rc_lj + p->skin;
+    E.n_max = n_max;
+    for (int d = 0; d < 3; ++d) {
+        E.dims[d] = (int)(E.box.L[d] / E.r_list);
+        E.cell[d] = E.box.L[d] / (E.dims[d] > 0 ? E.dims[d] : 1);
+    }
+    if (E.dims[0] < 3 || E.dims[1] < 3 || E.dims[2] < 3) {
+        g_create_err = "box must be at least 3 list radii (max cutoff + skin) on every axis";
+        delete h;
+        return RBMD_ERR_BOX;
+    }
+    E.ncell = E.dims[0] * E.dims[1] * E.dims[2];
+    // pair parameters
+    PairParams& pp = E.pp;
+    pp.rcl2 = p->rc_lj * p->rc_lj;
+    pp.rcc2 = p->rc_coul * p->rc_coul;
+    pp.alpha = p->alpha;
+    pp.sqrt_alpha = sqrt(p->alpha);
+    pp.two_sqrt_alpha_over_pi = 2.0 * sqrt(p->alpha / M_PI);
+    pp.qqr2e = p->qqr2e;
+    pp.ntypes = p->ntypes;
+    for (int a = 0; a < p->ntypes; ++a)
+        for (int b = 0; b < p->ntypes; ++b) {
+            const double s = 0.5 * (p->sigma[a] + p->sigma[b]);
+            pp.eps24[a * kMaxTypes + b] = 24.0 * sqrt(p->eps[a] * p->eps[b]);
+            pp.sig2[a * kMaxTypes + b] = s * s;
+        }
+    if (p->ntypes == 1) {
+        pp.eps24[0] = 24.0 * p->eps[0];
+        pp.sig2[0] = p->sigma[0] * p->sigma[0];
+    }
+    // capacity: 1.35x the mean row at the density of a full handle, >= 128
+    const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
+    const double mean = (double)n_max / V * 4.0 / 3.0 * M_PI * pow(E.r_list, 3);
+    int cap = (int)ceil(1.35 * mean + 32.0);
+    cap = std::max(128, ((cap + 63) / 64) * 64);
+    if (p->list_capacity > 0) cap = ((p->list_capacity + 63) / 64) * 64;
+
+    auto bail = [&](int code) {
+        g_create_err = E.err;
+        free_all(E);
+        delete h;
+        return code;
+    };
+    int rc;
+#define CR(expr)                                                  \
+    if (hipError_t _s = (expr); _s != hipSuccess) {               \
+        E.err = std::string(#expr) + ": " + hipGetErrorString(_s); \
+        return bail(RBMD_ERR_HIP);                                \
+    }
+    const size_t N = (size_t)std::max<int64_t>(n_max, 1);
+    CR(dalloc(&E.xr, N * 8));
+    CR(dalloc(&E.yr, N * 8));
+    CR(dalloc(&E.zr, N * 8));
+    CR(dalloc(&E.xs_ref, N * sizeof(double4)));
+    CR(dalloc(&E.xs, N * sizeof(double4)));
+    CR(dalloc(&E.perm, N * 4));
+    CR(dalloc(&E.scell, N * 4));
+    CR(dalloc(&E.ts, N * 4));
+    CR(dalloc(&E.atom_cell, N * 4));
+    CR(dalloc(&E.atom_rank, N * 4));
+    CR(dalloc(&E.cell_count, (size_t)E.ncell * 4));
+    CR(dalloc(&E.cell_start, ((size_t)E.ncell + 1) * 4));
+    CR(dalloc(&E.ncount, N * sizeof(int2)));
+    CR(dalloc(&E.flags, 64));
+    CR(dalloc(&E.stats, 64));
+    CR(dalloc(&E.q, N * 8));
+    CR(dalloc(&E.type, N * 4));
+    CR(dalloc(&E.d_pp, sizeof(PairParams)));
+    CR(dalloc(&E.d_step, 64));
+    CR(dalloc(&E.ev, 16 * 8));
+    E.ev_part_blocks = (int)((N + 3) / 4);
+    CR(dalloc(&E.ev_part, (size_t)E.ev_part_blocks * 8 * 8));
+    CR(hipHostMalloc((void**)&E.h_stats, 64, hipHostMallocDefault));
+    CR(hipStreamCreateWithFlags(&E.own_stream, hipStreamNonBlocking));
+    CR(hipMemset(E.flags, 0, 64));
+    CR(hipMemset(E.stats, 0, 64));
+    CR(hipMemset(E.d_step, 0, 64));
+    CR(hipMemset(E.ev, 0, 16 * 8));
+    CR(hipMemset(E.cell_start, 0, ((size_t)E.ncell + 1) * 4));
+    CR(hipMemset(E.xr, 0, N * 8));
+    CR(hipMemset(E.yr, 0, N * 8));
+    CR(hipMemset(E.zr, 0, N * 8));
+    CR(hipMemcpy(E.d_pp, &E.pp, sizeof(PairParams), hipMemcpyHostToDevice));
+    memset(E.h_stats, 0, 64);
+    if ((rc = alloc_list(E, cap)) != RBMD_OK) return bail(rc);
+    if (E.kspace == RBMD_KSPACE_RBE) {
+        build_kcdf(E.alpha, E.box.L, E.h_kcdf, E.kM);
+        E.S = rbe_S(E.alpha, E.box.L);
+        CR(dalloc(&E.kcdf, E.h_kcdf.size() * sizeof(double)));
+        CR(hipMemcpy(E.kcdf, E.h_kcdf.data(), E.h_kcdf.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+        if ((rc = alloc_kspace(E, E.p_max)) != RBMD_OK) return bail(rc);
+    } else if (E.kspace == RBMD_KSPACE_EWALD) {
+        std::vector<int32_t> m;
+        std::vector<double> w;
+        ewald_lattice(E, m, w);
+        const int P = (int)w.size();
+        if ((rc = alloc_kspace(E, P)) != RBMD_OK) return bail(rc);
+        CR(hipMemcpy(E.km, m.data(), m.size() * 4, hipMemcpyHostToDevice));
+        CR(hipMemcpy(E.kw, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+        E.P = P;
+    }
+#undef CR
+    *out = h;
+    return RBMD_OK;
+}
+
+void rbmd_force_destroy(rbmd_force_t h)
+{
+    if (!h) return;
+    hipDeviceSynchronize();
+    free_all(h->e);
+    delete h;
+}
+
+int rbmd_force_set_atoms(rbmd_force_t h, int64_t n, const double* q, const int32_t* type,
+                         void* stream)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (n < 0 || n > E.n_max) return fail(E, RBMD_ERR_CAPACITY, "n exceeds n_max");
+    if (n > 0 && !q) return fail(E, RBMD_ERR_INVALID, "charges required");
+    hipStream_t s = (hipStream_t)stream;
+    E.n = n;
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(E.q, q, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+        if (type) {
+            HIP_TRY(hipMemcpyAsync(E.type, type, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        } else {
+            HIP_TRY(hipMemsetAsync(E.type, 0, (size_t)n * 4, s));
+        }
+        // self energy -sqrt(alpha/pi) sum q^2 (output/ThermoOutput.cpp:196-203), on the host
+        std::vector<double> hq((size_t)n);
+        HIP_TRY(hipMemcpyAsync(hq.data(), q, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        double sq = 0.0;
+        for (double v : hq) sq += v * v;
+        E.e_self = E.kspace ? -sqrt(E.alpha / M_PI) * sq * E.qqr2e : 0.0;
+    }
+    launch_force_rebuild(E, s);
+    if (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) {
+        const int rc = alloc_kspace(E, E.kspace == RBMD_KSPACE_RBE ? std::max(E.p_max, E.P) : E.P);
+        if (rc != RBMD_OK) return rc;
+    }
+    E.atoms_set = true;
+    return RBMD_OK;
+}
+
+int rbmd_force_set_kvectors(rbmd_force_t h, int32_t P, const int32_t* m, const double* w,
+                            void* stream)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (E.kspace != RBMD_KSPACE_RBE) return fail(E, RBMD_ERR_STATE, "handle has no RBE k-space");
+    if (P < 1 || P > E.p_max || !m) return fail(E, RBMD_ERR_CAPACITY, "P out of [1, p_max]");
+    hipStream_t s = (hipStream_t)stream;
+    E.P = P;
+    int rc = alloc_kspace(E, P);
+    if (rc != RBMD_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(E.km, m, (size_t)P * 12, hipMemcpyDeviceToDevice, s));
+    if (w) {
+        HIP_TRY(hipMemcpyAsync(E.kw, w, (size_t)P * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+        launch_kweights_default(E, s);
+    }
+    return RBMD_OK;
+}
+
+int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_t step,
+                               void* stream)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (E.kspace != RBMD_KSPACE_RBE) return fail(E, RBMD_ERR_STATE, "handle has no RBE k-space");
+    if (P < 1 || P > E.p_max) return fail(E, RBMD_ERR_CAPACITY, "P out of [1, p_max]");
+    E.P = P;
+    int rc = alloc_kspace(E, P);
+    if (rc != RBMD_OK) return rc;
+    launch_sample(E, P, seed, &step, (hipStream_t)stream);
+    return RBMD_OK;
+}
+
+static int force_pass(Engine& E, const double* x, const double* y, const double* z, void* fx,
+                      void* fy, void* fz, bool ev, hipStream_t s, bool checked)
+{
+    if (!checked) launch_check(E, x, y, z, s);
+    launch_build(E, x, y, z, s);
+    launch_gather(E, x, y, z, s);
+    const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
+    if (ks) launch_rho(E, x, y, z, s);
+    launch_pair(E, fx, fy, fz, ev, s);
+    if (ks) launch_kforce(E, x, y, z, fx, fy, fz, s);
+    if (ev) launch_ev_final(E, true, s);
+    return RBMD_OK;
+}
+
+// Grow the list rows if the last build overflowed (rare: the pair pass is
+// still exact through its stencil fallback, only slower).
+static int maintain(Engine& E, hipStream_t s)
+{
+    if (E.h_stats[2] == 0ull) return RBMD_OK;
+    HIP_TRY(hipStreamSynchronize(s));
+    unsigned long long st[8];
+    HIP_TRY(hipMemcpy(st, E.stats, 64, hipMemcpyDeviceToHost));
+    if (st[2] == 0ull) {
+        E.h_stats[2] = 0;
+        return RBMD_OK;
+    }
+    const int newcap = ((int)(E.cap * 1.5) + 63) / 64 * 64;
+    int rc = alloc_list(E, newcap);
+    if (rc != RBMD_OK) return rc;
+    E.h_stats[2] = 0;
+    launch_force_rebuild(E, s);
+    return RBMD_OK;
+}
+
+int rbmd_force_compute(rbmd_force_t h, const double* x, const double* y, const double* z,
+                       void* fx, void* fy, void* fz, int32_t flags, void* stream)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (!E.atoms_set) return fail(E, RBMD_ERR_STATE, "rbmd_force_set_atoms not called");
+    if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
+        return fail(E, RBMD_ERR_STATE, "no k-vectors: call set_kvectors or sample_kvectors");
+    if (E.n == 0) return RBMD_OK;
+    if (!x || !y || !z || !fx || !fy || !fz) return fail(E, RBMD_ERR_INVALID, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = maintain(E, s);
+    if (rc != RBMD_OK) return rc;
+    force_pass(E, x, y, z, fx, fy, fz, (flags & RBMD_EV) != 0, s, false);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
+    return RBMD_OK;
+}
+
+int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* stream)
+{
+    if (!h || !out) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    double v[10];
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    HIP_TRY(hipMemcpy(v, E.ev, sizeof v, hipMemcpyDeviceToHost));
+    out->e_lj = v[0];
+    out->e_real = v[1];
+    out->e_self = v[2];
+    out->e_kspace = v[3];
+    for (int k = 0; k < 6; ++k) out->virial[k] = v[4 + k];
+    return RBMD_OK;
+}
+
+static void md_step(Engine& E, double* x, double* y, double* z, double* vx, double* vy,
+                    double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
+                    double dt, uint64_t seed, const uint64_t* step_host, hipStream_t s)
+{
+    launch_kick_drift_check(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, s);
+    if (E.kspace == RBMD_KSPACE_RBE) launch_sample(E, E.P, seed, step_host, s);
+    force_pass(E, x, y, z, fx, fy, fz, false, s, true);
+    launch_kick(E, vx, vy, vz, fx, fy, fz, mass, dt, s);
+}
+
+int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, double* vy,
+                double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
+                double dt, int64_t nsteps, uint64_t seed, uint64_t step0, int32_t use_graph,
+                void* stream)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (!E.atoms_set) return fail(E, RBMD_ERR_STATE, "rbmd_force_set_atoms not called");
+    if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
+        return fail(E, RBMD_ERR_STATE, "no k-vectors: sample once to fix P");
+    if (E.n == 0 || nsteps <= 0) return RBMD_OK;
+    hipStream_t s = stream ? (hipStream_t)stream : E.own_stream;
+    int rc = maintain(E, s);
+    if (rc != RBMD_OK) return rc;
+    if (!use_graph) {
+        for (int64_t k = 0; k < nsteps; ++k) {
+            const uint64_t st = step0 + (uint64_t)k;
+            md_step(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, seed, &st, s);
+        }
+        HIP_TRY(hipGetLastError());
+    } else {
+        std::vector<const void*> key = {x, y, z, vx, vy, vz, fx, fy, fz, img, mass,
+                                        (const void*)(uintptr_t)E.P, (const void*)(uintptr_t)E.n,
+                                        (const void*)(uintptr_t)E.cap};
+        // dt and seed are baked into the captured kernels: key them too
+        uint64_t dtbits;
+        memcpy(&dtbits, &dt, 8);
+        key.push_back((const void*)(uintptr_t)dtbits);
+        key.push_back((const void*)(uintptr_t)seed);
+        if (!E.graph_exec || key != E.graph_key) {
+            if (E.graph_exec) hipGraphExecDestroy(E.graph_exec);
+            if (E.graph) hipGraphDestroy(E.graph);
+            E.graph_exec = nullptr;
+            E.graph = nullptr;
+            E.capturing = true;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            md_step(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, seed, nullptr, s);
+            hipError_t st = hipStreamEndCapture(s, &E.graph);
+            E.capturing = false;
+            if (st != hipSuccess) {
+                E.err = std::string("graph capture failed: ") + hipGetErrorString(st);
+                return RBMD_ERR_HIP;
+            }
+            HIP_TRY(hipGraphInstantiate(&E.graph_exec, E.graph, nullptr, nullptr, 0));
+            E.graph_key = key;
+        }
+        // device step counter = step0; the graph's rho_final kernel advances it
+        unsigned long long s0 = step0;
+        HIP_TRY(hipMemcpyAsync(E.d_step, &s0, 8, hipMemcpyHostToDevice, s));
+        for (int64_t k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(E.graph_exec, s));
+    }
+    HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint32_t fl[2];
+    HIP_TRY(hipMemcpy(fl, E.flags, 8, hipMemcpyDeviceToHost));
+    if (fl[1]) return fail(E, RBMD_ERR_INVALID, "non-finite positions during the run");
+    return RBMD_OK;
+}
+
+int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int64_t cap,
+                             int64_t* count)
+{
+    if (!h || !offsets || !count) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    HIP_TRY(hipDeviceSynchronize());
+    const int64_t n = E.n;
+    std::vector<int32_t> perm((size_t)n), rows((size_t)n * E.cap);
+    std::vector<int2> nc((size_t)n);
+    HIP_TRY(hipMemcpy(perm.data(), E.perm, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nc.data(), E.ncount, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rows.data(), E.nlist, (size_t)n * E.cap * 4, hipMemcpyDeviceToHost));
+    // rows are indexed by sorted slot; emit them in original atom order
+    std::vector<int64_t> slot_of((size_t)n);
+    for (int64_t s = 0; s < n; ++s) slot_of[(size_t)perm[(size_t)s]] = s;
+    int64_t e = 0;
+    offsets[0] = 0;
+    bool overflow = false;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t s = slot_of[(size_t)i];
+        const int ni = nc[(size_t)s].x, no = nc[(size_t)s].y;
+        if (ni + no > E.cap) overflow = true;
+        const int32_t* row = rows.data() + (size_t)s * E.cap;
+        for (int k = 0; k < ni + no && !overflow; ++k) {
+            const int idxk = k < ni ? k : E.cap - 1 - (k - ni);
+            if (idx && e < cap) idx[e] = perm[(size_t)row[idxk]];
+            ++e;
+        }
+        offsets[i + 1] = e;
+    }
+    *count = e;
+    if (overflow) return fail(E, RBMD_ERR_CAPACITY, "list rows overflowed at the last build");
+    return RBMD_OK;
+}
+
+int rbmd_force_get_kspace(rbmd_force_t h, int32_t* P, int32_t* m, double* w, double* re,
+                          double* im)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    HIP_TRY(hipDeviceSynchronize());
+    if (P) *P = E.P;
+    if (E.P == 0) return RBMD_OK;
+    if (m) HIP_TRY(hipMemcpy(m, E.km, (size_t)E.P * 12, hipMemcpyDeviceToHost));
+    if (w) HIP_TRY(hipMemcpy(w, E.kw, (size_t)E.P * 8, hipMemcpyDeviceToHost));
+    if (re || im) {
+        std::vector<double> r((size_t)E.P * 2);
+        HIP_TRY(hipMemcpy(r.data(), E.rho, r.size() * 8, hipMemcpyDeviceToHost));
+        for (int k = 0; k < E.P; ++k) {
+            if (re) re[k] = r[2 * (size_t)k];
+            if (im) im[k] = r[2 * (size_t)k + 1];
+        }
+    }
+    return RBMD_OK;
+}
+
+int rbmd_force_get_stats(rbmd_force_t h, int64_t* out)
+{
+    if (!h || !out) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned long long st[8];
+    HIP_TRY(hipMemcpy(st, E.stats, 64, hipMemcpyDeviceToHost));
+    out[0] = (int64_t)st[0];
+    out[1] = (int64_t)st[1];
+    out[2] = E.cap;
+    out[3] = (int64_t)st[2];
+    out[4] = E.ncell;
+    out[5] = (int64_t)st[3];
+    out[6] = E.kspace == RBMD_KSPACE_EWALD ? E.P : 0;
+    out[7] = E.n;
+    return RBMD_OK;
+}
+
+int rbmd_force_set_timing(rbmd_force_t h, int32_t enable)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    timer_drain(E);
+    for (int p = 0; p < P_NUM; ++p) {
+        E.t_sum[p] = 0;
+        E.t_cnt[p] = 0;
+    }
+    E.timing = enable != 0;
+    return RBMD_OK;
+}
+
+int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int64_t* launches)
+{
+    if (!h || !pass) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    timer_drain(E);
+    for (int p = 0; p < P_NUM; ++p)
+        if (strcmp(pass, kPassNames[p]) == 0) {
+            if (mean_ms) *mean_ms = E.t_cnt[p] ? E.t_sum[p] / (double)E.t_cnt[p] : 0.0;
+            if (launches) *launches = E.t_cnt[p];
+            return RBMD_OK;
+        }
+    return fail(E, RBMD_ERR_INVALID, "unknown pass name");
+}
+
+int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream)
+{
+    if (!dst || !src || nbytes < 0 || (nbytes % 16) != 0) return RBMD_ERR_INVALID;
+    launch_copy(dst, src, nbytes, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? RBMD_OK : RBMD_ERR_HIP;
+}
+
+}  // extern "C"

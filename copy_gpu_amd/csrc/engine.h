@@ -1,0 +1,162 @@
+// engine.h -- internal state and device helpers of the MI355X charged-LJ force engine.
+//
+// Layout in HBM (n atoms, C = list row capacity, P = k-vectors):
+//   x_ref[3][n]  FP64  wrapped positions at the last list build (original atom order)
+//   xs_ref[n]    double4 (x,y,z,q) of x_ref in cell-sorted order (list build input)
+//   xs[n]        double4 (x,y,z,q) current positions, unwrapped against x_ref, cell-sorted
+//   perm[n]      int32  sorted slot -> original atom id
+//   scell[n]     int32  cell of each sorted slot
+//   nlist[n*C]   int32  row-major Verlet rows in sorted ids: LJ-range ("inner") entries
+//                       grow from the row start, the rest ("outer") from the row end
+//   ncount[n]    int2   (inner, outer) per row
+//   cell_count/cell_start  int32 counting-sort histogram and CSR bounds
+//   k-space: kvec (m, w), partial structure factors [blocks][P] (FP64), rho[P], per-k
+//   force coefficients, per-k energy/virial terms
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#define RBMD_HD __host__ __device__
+
+namespace rbmd {
+
+constexpr int kWave = 64;
+constexpr int kMaxTypes = 8;
+
+// Orthorhombic periodic box passed by value to kernels.
+struct Box {
+    double lo[3];
+    double L[3];
+    double invL[3];
+};
+
+// Parameters of the pair pass, by value.
+struct PairParams {
+    double rcl2, rcc2;  // squared cutoffs (FP64 decisions)
+    double alpha, sqrt_alpha, two_sqrt_alpha_over_pi, qqr2e;
+    int ntypes;
+    double eps24[kMaxTypes * kMaxTypes];  // 24 * sqrt(eps_i eps_j)
+    double sig2[kMaxTypes * kMaxTypes];   // ((sig_i + sig_j)/2)^2
+};
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11.  Round: (hi0,lo0) = M0*c0, (hi1,lo1) = M1*c2,
+// c = (hi1^c1^k0, lo1, hi0^c3^k1, lo0), key bumped by the Weyl constants.
+RBMD_HD inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
+                                  uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// 53-bit uniform in [0,1) from two 32-bit words (exact in FP64).
+RBMD_HD inline double u53(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Discrete-Gaussian inverse-CDF lookup: first index i with u < cdf[i]
+// (cdf has len entries, cdf[len-1] == 1 up to rounding).
+RBMD_HD inline int cdf_lookup(const double* cdf, int len, double u)
+{
+    int lo = 0, hi = len - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (u < cdf[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// The k-sampler draw shared by host and device: lattice vector of sample l at
+// a given step.  Key = seed, counter = (l, attempt*2 + half, step_lo, step_hi ^ tag).
+// cdf: three concatenated per-axis tables, each of length 2*M_d+1 for m in [-M_d, M_d].
+RBMD_HD inline void sample_kvector(uint64_t seed, uint64_t step, uint32_t l, const double* cdf,
+                                   const int* M, int m_out[3])
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (uint32_t attempt = 0;; ++attempt) {
+        uint32_t r0[4], r1[4];
+        uint32_t c0[4] = {l, attempt * 2u, (uint32_t)step, (uint32_t)(step >> 32) ^ 0x6B537631u};
+        uint32_t c1[4] = {l, attempt * 2u + 1u, (uint32_t)step,
+                          (uint32_t)(step >> 32) ^ 0x6B537631u};
+        philox4x32_10(c0, key, r0);
+        philox4x32_10(c1, key, r1);
+        const double u[3] = {u53(r0[0], r0[1]), u53(r0[2], r0[3]), u53(r1[0], r1[1])};
+        int off = 0;
+        for (int d = 0; d < 3; ++d) {
+            int len = 2 * M[d] + 1;
+            m_out[d] = cdf_lookup(cdf + off, len, u[d]) - M[d];
+            off += len;
+        }
+        if (m_out[0] != 0 || m_out[1] != 0 || m_out[2] != 0) return;
+        if (attempt > 1000000u) {  // unreachable for any sane box; keeps the loop finite
+            m_out[0] = 1;
+            return;
+        }
+    }
+}
+
+#ifdef __HIPCC__
+// ---------------------------------------------------------------- wave64 helpers
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Number of set bits of mask below this lane (ballot compaction offset).
+__device__ inline uint32_t mbcnt64(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+#endif
+
+// ---------------------------------------------------------------- timing
+enum Pass {
+    P_CHECK = 0,
+    P_BIN,
+    P_SCAN,
+    P_SCATTER,
+    P_CELLSORT,
+    P_LIST,
+    P_GATHER,
+    P_RHO,
+    P_RHO_FINAL,
+    P_PAIR,
+    P_KFORCE,
+    P_KICK,
+    P_EV_FINAL,
+    P_SAMPLE,
+    P_NUM
+};
+
+}  // namespace rbmd

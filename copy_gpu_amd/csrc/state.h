@@ -1,0 +1,135 @@
+// state.h -- host-side engine state (device buffers, parameters) and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace rbmd {
+
+// Per-k coefficients of the k-space force: phase from the integer lattice
+// vector m (FP64) and k = 2 pi m / L, A = c Im(rho), B = c Re(rho) with
+// c = w * (-4 pi) / (V k^2) * qqr2e.
+template <typename T>
+struct KCoef {
+    double m[3];
+    T k[3];
+    T A, B;
+};
+
+struct Engine {
+    // ---- parameters
+    Box box{};
+    double rc_lj = 0, rc_coul = 0, skin = 0, alpha = 0, qqr2e = 1, ewald_tol = 1e-16;
+    double r_list = 0, r_in = 0;
+    int precision = 32, kspace = 0, p_max = 0;
+    PairParams pp{};
+    int64_t n_max = 0, n = 0;
+    int dims[3] = {0, 0, 0};
+    double cell[3] = {0, 0, 0};
+    int ncell = 0;
+    int cap = 0;  // list row capacity, multiple of 64
+    double e_self = 0;
+    bool atoms_set = false;
+
+    // ---- device buffers
+    double *xr = nullptr, *yr = nullptr, *zr = nullptr;
+    double4 *xs_ref = nullptr, *xs = nullptr;
+    int32_t *perm = nullptr, *scell = nullptr, *ts = nullptr;
+    int32_t *cell_count = nullptr, *cell_start = nullptr, *atom_cell = nullptr,
+            *atom_rank = nullptr;
+    int32_t* nlist = nullptr;
+    int2* ncount = nullptr;
+    uint32_t* flags = nullptr;              // [0] rebuild request, [1] NaN seen
+    unsigned long long* stats = nullptr;    // [0] builds [1] entries [2] overflow rows [3] inner
+    double* q = nullptr;
+    int32_t* type = nullptr;
+    PairParams* d_pp = nullptr;
+
+    // k-space
+    int P = 0;            // current number of k-vectors
+    int kcap = 0;         // allocated k-vector slots
+    int32_t* km = nullptr;
+    double* kw = nullptr;
+    double* kpart = nullptr;  // [blocks_x][P][2] partial structure factors
+    int64_t kpart_cap = 0;    // doubles
+    double* rho = nullptr;    // [P][2]
+    void* kcoef = nullptr;    // KCoef<T>[P]
+    double* kev = nullptr;    // [P][7] per-k energy / virial terms
+    double* kcdf = nullptr;   // sampler CDF tables
+    int kM[3] = {0, 0, 0};
+    std::vector<double> h_kcdf;
+    double S = 0;             // RBE normaliser H_x H_y H_z - 1
+    unsigned long long* d_step = nullptr;  // device step counter for graph replays
+    bool kvec_weights_default = true;
+
+    // energy / virial
+    double* ev_part = nullptr;  // [blocks][8]
+    int ev_part_blocks = 0;
+    double* ev = nullptr;       // [10]
+
+    // host pinned mirror of stats
+    unsigned long long* h_stats = nullptr;
+
+    hipStream_t own_stream = nullptr;
+
+    // timing
+    bool timing = false;
+    struct Rec {
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending[P_NUM];
+    double t_sum[P_NUM] = {};
+    int64_t t_cnt[P_NUM] = {};
+    std::vector<hipEvent_t> event_pool;
+
+    // MD graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<const void*> graph_key;
+    bool capturing = false;
+
+    std::string err;
+};
+
+// ---- launchers (nlist.hip)
+void launch_check(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
+void launch_force_rebuild(Engine& e, hipStream_t s);
+void launch_build(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
+void launch_gather(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
+void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double* vx, double* vy,
+                             double* vz, const void* fx, const void* fy, const void* fz,
+                             int32_t* img, const double* mass, double dt, hipStream_t s);
+void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,
+                 const void* fz, const double* mass, double dt, hipStream_t s);
+void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s);
+
+// ---- launchers (pair.hip)
+void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s);
+void launch_ev_final(Engine& e, bool with_pair, hipStream_t s);
+
+// ---- launchers (kspace.hip)
+void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, hipStream_t s);
+void launch_kweights_default(Engine& e, hipStream_t s);
+void launch_rho(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
+void launch_kforce(Engine& e, const double* x, const double* y, const double* z, void* fx,
+                   void* fy, void* fz, hipStream_t s);
+
+// timing helpers (api.hip)
+void timer_begin(Engine& e, int pass, hipStream_t s, hipEvent_t* ev);
+void timer_end(Engine& e, int pass, hipStream_t s, hipEvent_t ev);
+
+struct ScopedTimer {
+    Engine& e;
+    int pass;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ScopedTimer(Engine& e_, int p, hipStream_t s_) : e(e_), pass(p), s(s_) { timer_begin(e, pass, s, &a); }
+    ~ScopedTimer() { timer_end(e, pass, s, a); }
+};
+
+}  // namespace rbmd

@@ -1,0 +1,195 @@
+/*
+ * rbmd_force.h -- C ABI of the MI355X charged-LJ force engine.
+ *
+ * One handle = one force evaluator for one periodic orthorhombic system:
+ * cell list + Verlet list, LJ + erfc real-space Coulomb pair pass, Random
+ * Batch Ewald (or full Ewald) k-space, FP64 energy and virial.  All array
+ * arguments are DEVICE pointers unless stated otherwise; streams are
+ * hipStream_t passed as void* (NULL = the null stream).  Every call is
+ * stream-ordered; functions documented as "synchronising" wait for the
+ * stream.  Return value: RBMD_OK (0) or a negative RBMD_ERR_* code; the
+ * message of the last error is available from rbmd_force_last_error().
+ * A handle is not thread-safe; use one per host thread / stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   Force::Init / Force::Execute      rbmd-gpu-clone/rbmd-gpu/src/force/include/force.h:8-25
+ *   LJCutCoulKspaceForce::Init/Execute rbmd-gpu-clone/rbmd-gpu/src/force/src/lj_cut_coul_kspace_force.cpp:37-78
+ *   ExecutionNVT::ComputeAllForce     rbmd/framework/run/ExecutionNVT.cpp:185-222
+ *   ExecutionNVT::Solve (VV step)     rbmd/framework/run/ExecutionNVT.cpp:133-159
+ */
+#ifndef RBMD_FORCE_H
+#define RBMD_FORCE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBMD_OK 0
+#define RBMD_ERR_INVALID (-1)   /* bad argument or parameter                    */
+#define RBMD_ERR_HIP (-2)       /* HIP runtime error                            */
+#define RBMD_ERR_BOX (-3)       /* box shorter than 3 list radii on some axis   */
+#define RBMD_ERR_PRECISION (-4) /* force dtype does not match handle precision */
+#define RBMD_ERR_STATE (-5)     /* call order (e.g. compute before set_atoms)   */
+#define RBMD_ERR_CAPACITY (-6)  /* n or P above the sizes given at create       */
+
+#define RBMD_KSPACE_NONE 0  /* LJ + real-space only                           */
+#define RBMD_KSPACE_RBE 1   /* Random Batch Ewald over P sampled k-vectors    */
+#define RBMD_KSPACE_EWALD 2 /* full Ewald lattice sum (FP64 physics oracle)   */
+
+#define RBMD_EV 1           /* compute_flags: accumulate energy and virial    */
+
+/* Parameters.  Key names follow the reference JSON (res/json_files/LJ_salt/rbmd.json:
+ * cut_off, alpha, coulomb_sample_num, ...). */
+typedef struct {
+    double box_lo[3];   /* box origin                                         */
+    double box_len[3];  /* box edge lengths                                   */
+    double rc_lj;       /* LJ cutoff ("cut_off")                              */
+    double rc_coul;     /* real-space Coulomb cutoff                          */
+    double skin;        /* Verlet skin; list radius = max(rc_lj, rc_coul)+skin */
+    double alpha;       /* Ewald splitting: erfc(sqrt(alpha) r), exp(-k^2/4alpha) */
+    double qqr2e;       /* Coulomb unit factor (1 in LJ units)                */
+    int32_t ntypes;     /* number of atom types (>= 1)                        */
+    const double* eps;  /* HOST array [ntypes], LJ epsilon per type          */
+    const double* sigma;/* HOST array [ntypes], LJ sigma per type            */
+    int32_t precision;  /* 32: FP32 pair/k-space maths, 64: FP64 throughout   */
+    int32_t kspace;     /* RBMD_KSPACE_*                                      */
+    int32_t p_max;      /* RBE: largest P used with this handle               */
+    double ewald_tol;   /* EWALD: keep k with exp(-k^2/4alpha) >= tol (e.g. 1e-16) */
+    int32_t list_capacity; /* Verlet row capacity (multiple of 64); 0 = automatic */
+} rbmd_force_params;
+
+/* FP64 energies (LJ units x qqr2e for Coulomb terms) and virial
+ * W_ab = 1/2 sum_i sum_j (r_i - r_j)_a F_ij,b + k-space part. */
+typedef struct {
+    double e_lj, e_real, e_self, e_kspace;
+    double virial[6]; /* xx, yy, zz, xy, xz, yz */
+} rbmd_energy_virial;
+
+typedef struct rbmd_force_s* rbmd_force_t;
+
+/* Create a handle sized for up to n_max atoms.  Replaces Force::Init()
+ * (force.h:19, lj_cut_coul_kspace_force.cpp:37-71): reads cutoffs, alpha,
+ * P and unit factors, allocates the cell grid, list and k-space scratch. */
+int rbmd_force_create(const rbmd_force_params* params, int64_t n_max, rbmd_force_t* out);
+
+/* Free the handle and all device memory it owns. */
+void rbmd_force_destroy(rbmd_force_t h);
+
+/* Message of the last error on this handle (or of the last failed create
+ * when h is NULL).  Never NULL. */
+const char* rbmd_force_last_error(rbmd_force_t h);
+
+/* Bind per-atom constants: charge (FP64, n) and type (int32, n, nullable =>
+ * all type 0).  Replaces the DeviceData charge/type arrays the Force reads
+ * (rbmd-gpu src/data_manager/include/model/device_data.h:8-156).  Forces a
+ * list rebuild at the next compute.  Copies the data. */
+int rbmd_force_set_atoms(rbmd_force_t h, int64_t n, const double* q, const int32_t* type,
+                         void* stream);
+
+/* Use the given k-vectors for the next computes: m (int32, 3*P, integer
+ * lattice vectors, k = 2 pi m / L) and w (FP64, P, per-k estimator weight;
+ * NULL => the RBE weight S/P of RBEPSAMPLE, model/RBEPSample.h:38-61).
+ * Replaces the host k-list of ExecutionNVT::PreForce (run/ExecutionNVT.cpp:331-353).
+ * m must not be (0,0,0). */
+int rbmd_force_set_kvectors(rbmd_force_t h, int32_t P, const int32_t* m, const double* w,
+                            void* stream);
+
+/* Draw P k-vectors on the device from the exact discrete Gaussian
+ * p(m) ~ exp(-k^2/(4 alpha)), m != 0 (the target of RBEPSAMPLE::Fetch_P_Sample,
+ * model/RBEPSample.h:143-175), counter-based (Philox4x32-10, key = seed,
+ * counter = (l, attempt, step)), weights S/P.  Deterministic in (seed, step). */
+int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_t step,
+                               void* stream);
+
+/* One force evaluation.  x, y, z: FP64 positions (n, any periodic image).
+ * fx, fy, fz: float* when the handle precision is 32, double* when 64.
+ * flags: RBMD_EV to also accumulate energy and virial.
+ * Replaces Force::Execute() (force.h:20; lj_cut_coul_kspace_force.cpp:73-78)
+ * and ExecutionNVT::ComputeAllForce (run/ExecutionNVT.cpp:185-222). */
+int rbmd_force_compute(rbmd_force_t h, const double* x, const double* y, const double* z,
+                       void* fx, void* fy, void* fz, int32_t flags, void* stream);
+
+/* Energies and virial of the last compute with RBMD_EV (synchronising). */
+int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* stream);
+
+/* Velocity-Verlet steps with the force evaluation fused into the step:
+ * kick (v += dt/2 F/m), drift (x += v dt, wrap into the box, image flags),
+ * list check/rebuild, force, kick -- the order of ExecutionNVT::Solve
+ * (run/ExecutionNVT.cpp:133-159) with UpdateVelocityWorklet and
+ * UpdatePositionFlagWorklet (run/worklet/RunWorklet.cpp:1764-1782,1854-1877).
+ * x,y,z,vx,vy,vz FP64 (n); f* as in compute and holding the forces of the
+ * current positions on entry; img int32 (3n, xyz interleaved) or NULL;
+ * mass FP64 (n) or NULL (=1).  With RBE, k-vectors are resampled every step
+ * from (seed, step0 + s).  use_graph != 0 captures one step in a HIP graph
+ * and replays it.  Positions/velocities/forces are updated in place. */
+int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, double* vy,
+                double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
+                double dt, int64_t nsteps, uint64_t seed, uint64_t step0, int32_t use_graph,
+                void* stream);
+
+/* Current full Verlet list in ORIGINAL atom ids as CSR (HOST arrays):
+ * offsets[n+1], idx[cap]; rows unsorted.  Returns the entry count in *count
+ * (if larger than cap only the offsets are written).  Synchronising.
+ * Replaces the list of FullNeighborListBuilder::Build
+ * (rbmd-gpu src/neighbor_list/src/neighbor_list_builder/full_neighbor_list_builder.cpp:31-45). */
+int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int64_t cap,
+                             int64_t* count);
+
+/* k-vectors (HOST m[3P], w[P]) and structure factor (HOST re[P], im[P]) of
+ * the last compute.  Synchronising.  Any output pointer may be NULL. */
+int rbmd_force_get_kspace(rbmd_force_t h, int32_t* P, int32_t* m, double* w, double* re,
+                          double* im);
+
+/* Diagnostic counters (HOST out[8], synchronising): [0] list builds so far,
+ * [1] list entries (full, at the list radius), [2] row capacity, [3] rows
+ * that overflowed at the last build, [4] cells, [5] inner (LJ-range) entries,
+ * [6] Ewald k-vectors (EWALD mode), [7] n. */
+int rbmd_force_get_stats(rbmd_force_t h, int64_t* out);
+
+/* Per-kernel device time (HIP events on the launch stream).  enable != 0
+ * starts recording around every launch of the named passes; get returns the
+ * mean milliseconds per launch and the launch count for one pass name:
+ * "check", "bin", "scan", "scatter", "cellsort", "list", "gather", "rho",
+ * "rho_final", "pair", "kforce", "kick", "ev_final". */
+int rbmd_force_set_timing(rbmd_force_t h, int32_t enable);
+int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int64_t* launches);
+
+/* Plain device copy of nbytes (multiple of 16), used to calibrate HBM
+ * traffic counters against a kernel of known traffic. */
+int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream);
+
+/* ---- host-side system generators (CPU, deterministic, Philox4x32-10) ---- */
+
+/* Philox4x32-10 block (Salmon et al., SC'11): out[4] for counter ctr[4], key[2]. */
+void rbmd_philox4x32(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+
+/* FCC lattice of nc^3 unit cells (4 atoms each) at number density rho,
+ * box [0, L)^3, L = nc (4/rho)^(1/3).  Writes 4 nc^3 FP64 positions (HOST). */
+int rbmd_sys_fcc(int32_t nc, double rho, double* x, double* y, double* z, double* L_out);
+
+/* n points uniform in [0,L)^3 with minimum-image separation >= dmin,
+ * by seeded random sequential addition.  HOST arrays.  Returns RBMD_OK or
+ * RBMD_ERR_INVALID if the packing could not be completed. */
+int rbmd_sys_random(int64_t n, double L, double dmin, uint64_t seed, double* x, double* y,
+                    double* z);
+
+/* Charges: first half +1, second half -1 of a seeded random permutation. */
+int rbmd_sys_charges(int64_t n, uint64_t seed, double* q);
+
+/* Maxwell-Boltzmann velocities at temperature T (k_B = 1), zero total
+ * momentum, rescaled so that sum m v^2 / (3n - 3) = T exactly.  mass NULL => 1. */
+int rbmd_sys_velocities(int64_t n, double T, const double* mass, uint64_t seed, double* vx,
+                        double* vy, double* vz);
+
+/* The device k-sampler, run on the host (same draws, bit-identical m):
+ * m[3P] (HOST), returns S in *S_out (nullable). */
+int rbmd_sample_kvectors_host(double alpha, const double* L, int32_t P, uint64_t seed,
+                              uint64_t step, int32_t* m, double* S_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBMD_FORCE_H */

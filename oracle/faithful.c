@@ -1,0 +1,259 @@
+/*
+ * oracle/faithful.c -- reference-faithful FP32 MD step, the timed CPU baseline.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by bench.py's cpu_baseline leg and by
+ * tests/, never by the product library.  It mirrors the reference RBMD CPU
+ * path's per-step algorithm (FP32 Real, AoS, list rebuilt at rc every step,
+ * N*P structure-factor arrays), so timing it on the host cores stands in for
+ * timing the reference binary, which cannot be built here (it needs VTK-m).
+ * Built separately from refcpu.c so it may use FMA contraction like an
+ * optimised reference build would.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846
+
+double orc_rbe_S(double alpha, const double* L, double* H);
+
+/* ------------------------------------------------------------------ */
+/* reference-faithful FP32 step (timed CPU baseline)                    */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t key;
+    int32_t id;
+} orc_kv;
+static int cmp_kv(const void* a, const void* b)
+{
+    const orc_kv* x = (const orc_kv*)a;
+    const orc_kv* y = (const orc_kv*)b;
+    if (x->key != y->key) return (x->key > y->key) - (x->key < y->key);
+    return (x->id > y->id) - (x->id < y->id);
+}
+
+/* One or more MD steps in the reference's own shape, FP32 `Real`
+ * (Types.h:48), AoS Vec3f arrays, for timing the CPU path:
+ *   kick (RunWorklet.cpp:1764-1782); drift + single-shift wrap
+ *   (:1854-1877); locator rebuild = bin + SortByKey + bounds
+ *   (ContPointLocator.cxx:70-92); full list at rc_coul with NO skin into
+ *   N x (rho ceil(4/3 pi rc^3)+1) slots with Vec3f offsets
+ *   (ExecutionMD.cpp:670-705, RunWorklet.cpp:40-96); near force over the list
+ *   (RunWorklet.cpp:170-237); RBE rho through materialised N*P Re/Im arrays
+ *   and a reduce by key (RunWorklet.cpp:1616-1677, ExecutionMD.cpp:412-448);
+ *   RBE force loop (RunWorklet.cpp:1508-1548) with S/P; F = far + near
+ *   (RunWorklet.cpp:1730-1744); kick.  ExecutionNVT::Solve order
+ *   (run/ExecutionNVT.cpp:133-159).  The k-list of every step is an input
+ *   (m: nsteps x P x 3).  nthreads > 1 parallelises the per-atom loops with
+ *   OpenMP (the VTK-m OpenMP device).  Returns 0, or -1 on list overflow. */
+int orc_faithful_run(int64_t n, float* pos, float* vel, float* force, const float* q,
+                     const double* lo_d, const double* L_d, double rc_lj_d, double rc_coul_d,
+                     double alpha_d, int32_t P, const int32_t* m, int32_t nsteps, double dt_d,
+                     int32_t nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const float lo[3] = {(float)lo_d[0], (float)lo_d[1], (float)lo_d[2]};
+    const float L[3] = {(float)L_d[0], (float)L_d[1], (float)L_d[2]};
+    const float hi[3] = {lo[0] + L[0], lo[1] + L[1], lo[2] + L[2]};
+    const float rc = (float)(rc_coul_d > rc_lj_d ? rc_coul_d : rc_lj_d);
+    const float rcl = (float)rc_lj_d, rcc = (float)rc_coul_d;
+    const float alpha = (float)alpha_d, dt = (float)dt_d;
+    const float V = L[0] * L[1] * L[2];
+    const float rho_sys = (float)((double)n / ((double)L[0] * L[1] * L[2]));
+    const int64_t max_j = (int64_t)(rho_sys * ceilf(4.0f / 3.0f * (float)ORC_PI * rc * rc * rc)) + 1;
+    int dims[3];
+    float dxyz[3];
+    for (int d = 0; d < 3; ++d) {
+        dims[d] = (int)(L[d] / rc);
+        if (dims[d] < 1) dims[d] = 1;
+        dxyz[d] = L[d] / (float)dims[d];
+    }
+    const int64_t ncell = (int64_t)dims[0] * dims[1] * dims[2];
+    orc_kv* kv = (orc_kv*)malloc(sizeof(orc_kv) * (size_t)n);
+    int64_t* lower = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncell);
+    int64_t* upper = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncell);
+    int32_t* pid = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* vid = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n * max_j));
+    float* voff = (float*)malloc(sizeof(float) * 3 * (size_t)(n * max_j));
+    int64_t* vnum = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    float* near = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+    float* far = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+    float* rre = (float*)malloc(sizeof(float) * (size_t)(n * P > 0 ? n * P : 1));
+    float* rim = (float*)malloc(sizeof(float) * (size_t)(n * P > 0 ? n * P : 1));
+    float* rho = (float*)malloc(sizeof(float) * 2 * (size_t)(P > 0 ? P : 1));
+    float* kl = (float*)malloc(sizeof(float) * 3 * (size_t)(P > 0 ? P : 1));
+    int overflow = 0;
+    const float sa = sqrtf(alpha), spi = sqrtf((float)ORC_PI);
+    double H[3];
+    const float S = (float)orc_rbe_S(alpha_d, L_d, H);
+
+    for (int32_t step = 0; step < nsteps; ++step) {
+        /* stage 1: kick; stage 2: drift + wrap */
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            for (int d = 0; d < 3; ++d) {
+                vel[3 * i + d] += 0.5f * force[3 * i + d] / 1.0f * dt * 1.0f;
+                pos[3 * i + d] += vel[3 * i + d] * dt;
+                if (pos[3 * i + d] < lo[d])
+                    pos[3 * i + d] += hi[d] - lo[d];
+                else if (pos[3 * i + d] > hi[d])
+                    pos[3 * i + d] -= hi[d] - lo[d];
+            }
+        }
+        /* locator: bin, sort by key, lower/upper bounds */
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t ijk[3];
+            for (int d = 0; d < 3; ++d) {
+                int64_t v = (int64_t)((pos[3 * i + d] - lo[d]) / dxyz[d]);
+                if (v < 0) v = 0;
+                if (v > dims[d] - 1) v = dims[d] - 1;
+                ijk[d] = v;
+            }
+            kv[i].key = ijk[0] + ijk[1] * dims[0] + ijk[2] * (int64_t)dims[0] * dims[1];
+            kv[i].id = (int32_t)i;
+        }
+        qsort(kv, (size_t)n, sizeof(orc_kv), cmp_kv);
+        for (int64_t i = 0; i < n; ++i) pid[i] = kv[i].id;
+        {
+            int64_t s = 0;
+            for (int64_t c = 0; c < ncell; ++c) {
+                lower[c] = s;
+                while (s < n && kv[s].key == c) ++s;
+                upper[c] = s;
+            }
+        }
+        /* full Verlet list at rc, fixed-stride slots with image offsets */
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : overflow)
+        for (int64_t i = 0; i < n; ++i) {
+            const float* pi = pos + 3 * i;
+            int64_t ijk[3];
+            for (int d = 0; d < 3; ++d) ijk[d] = (int64_t)((pi[d] - lo[d]) / dxyz[d]);
+            int64_t cnt = 0;
+            for (int a = -1; a <= 1; ++a)
+                for (int b = -1; b <= 1; ++b)
+                    for (int g = -1; g <= 1; ++g) {
+                        int64_t nb[3] = {ijk[0] + a, ijk[1] + b, ijk[2] + g};
+                        int64_t w[3];
+                        float off[3];
+                        for (int d = 0; d < 3; ++d) {
+                            w[d] = (nb[d] + dims[d]) % dims[d];
+                            off[d] = dxyz[d] * (float)(w[d] - nb[d]);
+                        }
+                        int64_t cc = w[0] + w[1] * dims[0] + w[2] * (int64_t)dims[0] * dims[1];
+                        for (int64_t p = lower[cc]; p < upper[cc]; ++p) {
+                            int32_t j = pid[p];
+                            float rx = (pos[3 * j] - off[0]) - pi[0];
+                            float ry = (pos[3 * j + 1] - off[1]) - pi[1];
+                            float rz = (pos[3 * j + 2] - off[2]) - pi[2];
+                            float d2 = rx * rx + ry * ry + rz * rz;
+                            if (rc * rc - d2 > 0.0001f && d2 > 0.0001f) {
+                                if (cnt < max_j) {
+                                    vid[i * max_j + cnt] = j;
+                                    voff[3 * (i * max_j + cnt)] = off[0];
+                                    voff[3 * (i * max_j + cnt) + 1] = off[1];
+                                    voff[3 * (i * max_j + cnt) + 2] = off[2];
+                                } else {
+                                    overflow |= 1;
+                                }
+                                ++cnt;
+                            }
+                        }
+                    }
+            vnum[i] = cnt < max_j ? cnt : max_j;
+        }
+        /* near force over the list */
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t i = 0; i < n; ++i) {
+            float f[3] = {0, 0, 0}, fe[3] = {0, 0, 0};
+            const float* pi = pos + 3 * i;
+            for (int64_t p = 0; p < vnum[i]; ++p) {
+                int32_t j = vid[i * max_j + p];
+                float r[3];
+                for (int d = 0; d < 3; ++d) r[d] = (pos[3 * j + d] - voff[3 * (i * max_j + p) + d]) - pi[d];
+                float d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                float dis = sqrtf(d2);
+                if (dis < rcc && dis > 0.01f) {
+                    float g = (1.0f - erff(sa * dis)) / (dis * dis) +
+                              2.0f * sa * expf(-alpha * dis * dis) / (spi * dis);
+                    for (int d = 0; d < 3; ++d) fe[d] += -q[i] * q[j] * g * r[d] / dis;
+                }
+                if (d2 < rcl * rcl && d2 > 0.0001f) {
+                    float s6 = 1.0f / (d2 * d2 * d2);
+                    float c = -24.0f * ((2.0f * s6 - 1.0f) * s6) / d2;
+                    for (int d = 0; d < 3; ++d) f[d] += c * r[d];
+                }
+            }
+            for (int d = 0; d < 3; ++d) near[3 * i + d] = f[d] + fe[d];
+        }
+        /* RBE: N*P materialised structure-factor terms, reduce by key */
+        if (P > 0) {
+            const int32_t* ms = m + (int64_t)step * P * 3;
+            for (int32_t l = 0; l < P; ++l)
+                for (int d = 0; d < 3; ++d) kl[3 * l + d] = 2.0f * (float)ORC_PI * (float)ms[3 * l + d] / L[d];
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                for (int32_t l = 0; l < P; ++l) {
+                    float ph = kl[3 * l] * pos[3 * i] + kl[3 * l + 1] * pos[3 * i + 1] + kl[3 * l + 2] * pos[3 * i + 2];
+                    rre[i + l * n] = q[i] * cosf(ph);
+                    rim[i + l * n] = q[i] * sinf(ph);
+                }
+            }
+#pragma omp parallel for schedule(static)
+            for (int32_t l = 0; l < P; ++l) {
+                float a = 0.0f, b = 0.0f;
+                for (int64_t i = 0; i < n; ++i) {
+                    a += rre[i + l * n];
+                    b += rim[i + l * n];
+                }
+                rho[2 * l] = a;
+                rho[2 * l + 1] = b;
+            }
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                float f[3] = {0, 0, 0};
+                for (int32_t l = 0; l < P; ++l) {
+                    const float* k = kl + 3 * l;
+                    float k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                    float ph = k[0] * pos[3 * i] + k[1] * pos[3 * i + 1] + k[2] * pos[3 * i + 2];
+                    float bc = cosf(ph) * rho[2 * l + 1];
+                    float cs = sinf(ph) * rho[2 * l];
+                    for (int d = 0; d < 3; ++d) f[d] += (-4.0f * (float)ORC_PI * q[i] * k[d] / (V * k2)) * (bc - cs);
+                }
+                for (int d = 0; d < 3; ++d) far[3 * i + d] = f[d] * S / (float)P;
+            }
+        } else {
+            memset(far, 0, sizeof(float) * 3 * (size_t)n);
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) {
+                force[3 * i + d] = far[3 * i + d] + near[3 * i + d];
+                vel[3 * i + d] += 0.5f * force[3 * i + d] * dt;
+            }
+    }
+    free(kv);
+    free(lower);
+    free(upper);
+    free(pid);
+    free(vid);
+    free(voff);
+    free(vnum);
+    free(near);
+    free(far);
+    free(rre);
+    free(rim);
+    free(rho);
+    free(kl);
+    return overflow ? -1 : 0;
+}
+

@@ -1,0 +1,410 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on identical inputs.
+
+Bars: neighbour lists equal as sets, bit for bit; forces within 1e-5 relative
+(L2 over all atoms) in FP32 mode and 1e-12 in FP64 mode; structure factors and
+energies/virial to the same relative bars.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def _make(Lb, n, precision=32, kspace="NONE", P=100, rc_lj=2.5, rc_coul=5.0, alpha=0.5,
+          skin=0.3, lo=(0.0, 0.0, 0.0), eps=(1.0,), sigma=(1.0,), list_capacity=0,
+          ewald_tol=1e-16):
+    from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
+
+    cfg = ForceConfig(box_len=tuple(float(v) for v in Lb), box_lo=lo, cut_off=rc_lj,
+                      coulomb_cut=rc_coul, skin=skin, alpha=alpha, kspace=kspace,
+                      coulomb_sample_num=P, precision=precision, eps=eps, sigma=sigma,
+                      n_max=n, list_capacity=list_capacity, ewald_tol=ewald_tol)
+    return LJCutCoulKspaceForce(cfg)
+
+
+def _compute(f, x, y, z, ev=False):
+    n = x.size
+    X, Y, Z = _dev(x), _dev(y), _dev(z)
+    F = [torch.zeros(n, device="cuda", dtype=f.fdtype) for _ in range(3)]
+    f.Execute(X, Y, Z, *F, energy_virial=ev)
+    torch.cuda.synchronize()
+    return np.stack([t.double().cpu().numpy() for t in F], 1)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _sorted_rows(offs, idx):
+    return [np.sort(idx[offs[i]:offs[i + 1]]) for i in range(len(offs) - 1)]
+
+
+def _ions(n, seed=SEED, rho=0.3):
+    from copy_gpu_amd import systems
+
+    return systems.random_ions(n, rho, 0.8, seed=seed)
+
+
+# ----------------------------------------------------------------------------- lists
+
+def _check_sets(orc, f, x, y, z, lo, Lb, r_list):
+    xw, yw, zw = orc.wrap(x, y, z, lo, Lb)
+    oo, io = orc.neighbors_cells(xw, yw, zw, lo, Lb, r_list)
+    og, ig = f.neighbors()
+    assert og[-1] == oo[-1], (og[-1], oo[-1])
+    ro, rg = _sorted_rows(oo, io), _sorted_rows(og, ig)
+    bad = [i for i in range(len(ro)) if not np.array_equal(ro[i], rg[i])]
+    assert not bad, f"{len(bad)} rows differ, first {bad[:5]}"
+
+
+def test_neighbor_sets_random_ions(orc, gpu):
+    _log("case: neighbor sets, 4000 random ions, density 0.3")
+    x, y, z, q, L = _ions(4000)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size)
+    f.Init(_dev(q))
+    _compute(f, x, y, z)
+    _check_sets(orc, f, x, y, z, np.zeros(3), Lb, 5.3)
+    st = f.stats()
+    assert st["builds"] == 1 and st["overflow_rows"] == 0
+    # unwrapped input (atoms moved by whole periods) gives the same sets
+    rng = np.random.default_rng(1)
+    sh = rng.integers(-2, 3, size=(3, x.size)) * L
+    f2 = _make(Lb, x.size)
+    f2.Init(_dev(q))
+    _compute(f2, x + sh[0], y + sh[1], z + sh[2])
+    _check_sets(orc, f2, x + sh[0], y + sh[1], z + sh[2], np.zeros(3), Lb, 5.3)
+
+
+def test_neighbor_sets_fcc_and_reference_fixture(orc, gpu):
+    from copy_gpu_amd import systems
+
+    _log("case: neighbor sets, FCC 4000 (density 0.8442)")
+    x, y, z, L = systems.fcc(10, 0.8442)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, rc_coul=0.0)
+    f.Init(_dev(np.zeros(x.size)))
+    _compute(f, x, y, z)
+    og, _ = f.neighbors()
+    assert np.all(np.diff(og) == 78)
+    _check_sets(orc, f, x, y, z, np.zeros(3), Lb, 2.8)
+
+    _log("case: neighbor sets, reference LJ_salt 6000 ions, box 10x20x30")
+    d = np.load(os.path.join(GOLDEN, "lj_salt_6000.npz"))
+    Lb = d["hi"] - d["lo"]
+    f = _make(Lb, 6000, rc_coul=2.5, lo=tuple(d["lo"]))
+    f.Init(_dev(d["q"]))
+    _compute(f, d["x"], d["y"], d["z"])
+    _check_sets(orc, f, d["x"], d["y"], d["z"], d["lo"], Lb, 2.8)
+
+
+def test_neighbor_sets_ragged_box_offset_origin(orc, gpu):
+    _log("case: neighbor sets, ragged box with offset origin")
+    rng = np.random.default_rng(7)
+    lo = np.array([-3.0, 2.5, 10.0])
+    Lb = np.array([17.0, 23.5, 19.25])
+    n = 3000
+    x, y, z = (lo[d] + rng.uniform(0, Lb[d], n) for d in range(3))
+    f = _make(Lb, n, rc_coul=4.0, lo=tuple(lo))
+    f.Init(_dev(np.zeros(n)))
+    _compute(f, x, y, z)
+    _check_sets(orc, f, x, y, z, lo, Lb, 4.3)
+
+
+# ----------------------------------------------------------------------------- pair pass
+
+@pytest.mark.parametrize("precision,tol", [(32, 1e-5), (64, 1e-12)])
+def test_pair_forces_and_energy(orc, gpu, precision, tol):
+    _log(f"case: pair forces FP{precision}, 8000 ions")
+    x, y, z, q, L = _ions(8000)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, precision=precision)
+    f.Init(_dev(q))
+    F = _compute(f, x, y, z, ev=True)
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fo, evo = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, 0.5)
+    err = _rel(F, Fo)
+    _log(f"  rel L2 force error {err:.3e}")
+    assert err < tol
+    floor = 1e-3 * np.sqrt(np.mean(np.sum(Fo ** 2, 1)))
+    per_atom = np.linalg.norm(F - Fo, axis=1) / np.maximum(np.linalg.norm(Fo, axis=1), floor)
+    assert per_atom.max() < 50 * tol, per_atom.max()
+    ev = f.energy_virial()
+    assert abs(ev["e_lj"] - evo[0]) < 10 * tol * abs(evo[0])
+    assert abs(ev["e_real"] - evo[1]) < 10 * tol * abs(evo[1])
+    W = ev["virial"]
+    assert np.linalg.norm(W - evo[2:8]) < 10 * tol * np.linalg.norm(evo[2:8])
+
+
+def test_pair_forces_two_types(orc, gpu):
+    _log("case: pair forces FP64, two LJ types")
+    x, y, z, q, L = _ions(3000, seed=3)
+    Lb = np.array([L] * 3)
+    types = (np.arange(x.size) % 3 == 0).astype(np.int32)
+    eps, sig = (1.0, 0.6), (1.0, 1.3)
+    f = _make(Lb, x.size, precision=64, eps=eps, sigma=sig)
+    f.Init(_dev(q), _dev(types, torch.int32))
+    F = _compute(f, x, y, z)
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fo, _ = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, 0.5, types=types,
+                            eps=eps, sigma=sig)
+    assert _rel(F, Fo) < 1e-12
+
+
+def test_overflowed_rows_use_stencil_fallback(orc, gpu):
+    _log("case: list capacity 64 (rows overflow), FP64")
+    x, y, z, q, L = _ions(3000, seed=9)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, precision=64, list_capacity=64)
+    f.Init(_dev(q))
+    F = _compute(f, x, y, z)
+    assert f.stats()["overflow_rows"] > 0
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fo, _ = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, 0.5)
+    assert _rel(F, Fo) < 1e-12
+    # the next call grows the rows and rebuilds
+    F2 = _compute(f, x, y, z)
+    st = f.stats()
+    assert st["overflow_rows"] == 0 and st["cap"] > 64
+    assert _rel(F2, Fo) < 1e-12
+
+
+# ----------------------------------------------------------------------------- k-space
+
+@pytest.mark.parametrize("precision,tol", [(32, 1e-5), (64, 1e-12)])
+def test_rbe_forces_given_klist(orc, gpu, precision, tol):
+    from copy_gpu_amd import systems
+
+    _log(f"case: RBE FP{precision}, 32768 ions, P=100 (c2 size)")
+    x, y, z, q, L = _ions(32768)
+    Lb = np.array([L] * 3)
+    alpha, P = 0.5, 100
+    m, S = systems.sample_kvectors_host(alpha, Lb, P, seed=SEED, step=0)
+    f = _make(Lb, x.size, precision=precision, kspace="RBE", P=P)
+    f.Init(_dev(q))
+    f.set_kvectors(_dev(m.reshape(-1), torch.int32))
+    F = _compute(f, x, y, z, ev=True)
+    mg, wg, re, im = f.kspace()
+    assert np.array_equal(mg, m)
+    assert np.allclose(wg, S / P, rtol=0, atol=0)
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    reo, imo = orc.rho(xw, yw, zw, q, Lb, m)
+    scale = np.max(np.hypot(reo, imo))
+    rerr = max(np.abs(re - reo).max(), np.abs(im - imo).max()) / scale
+    _log(f"  rho max error / max|rho| = {rerr:.3e}")
+    assert rerr < tol
+    Fp, evp = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, alpha)
+    Fk, evk = orc.kspace_forces(xw, yw, zw, q, Lb, alpha, m, np.full(P, S / P), reo, imo)
+    err = _rel(F, Fp + Fk)
+    kerr = _rel(F - Fp, Fk)
+    _log(f"  rel L2 total force error {err:.3e}, k-space part {kerr:.3e}")
+    assert err < tol
+    assert kerr < 10 * tol
+    ev = f.energy_virial()
+    assert abs(ev["e_kspace"] - evk[0]) < 10 * tol * abs(evk[0])
+    assert abs(ev["e_self"] - orc.self_energy(q, alpha)) < 1e-12 * abs(ev["e_self"])
+    W = ev["virial"]
+    Wo = evp[2:8] + evk[1:7]
+    assert np.linalg.norm(W - Wo) < 10 * tol * np.linalg.norm(Wo)
+
+
+def test_device_sampler_matches_host(gpu):
+    from copy_gpu_amd import systems
+
+    _log("case: device k-sampler vs host draws")
+    L = 69.336
+    Lb = np.array([L] * 3)
+    f = _make(Lb, 10, kspace="RBE", P=400, alpha=0.5)
+    for step in (0, 1, 977):
+        f.sample_kvectors(400, SEED, step)
+        m, w, _, _ = f.kspace()
+        mh, S = systems.sample_kvectors_host(0.5, Lb, 400, SEED, step)
+        assert np.array_equal(m, mh)
+        assert np.all(w == S / 400)
+
+
+def _halfspace_lattice(Lb, alpha, tol):
+    kmax2 = -4 * alpha * np.log(tol)
+    M = [int(np.floor(np.sqrt(kmax2) * Lb[d] / (2 * np.pi))) + 1 for d in range(3)]
+    a, b, g = np.meshgrid(np.arange(0, M[0] + 1), np.arange(-M[1], M[1] + 1),
+                          np.arange(-M[2], M[2] + 1), indexing="ij")
+    a, b, g = a.ravel(), b.ravel(), g.ravel()
+    keep = ~((a == 0) & ((b < 0) | ((b == 0) & (g <= 0))))
+    a, b, g = a[keep], b[keep], g[keep]
+    k2 = (2 * np.pi * a / Lb[0]) ** 2 + (2 * np.pi * b / Lb[1]) ** 2 + (2 * np.pi * g / Lb[2]) ** 2
+    keep = k2 <= kmax2
+    m = np.stack([a[keep], b[keep], g[keep]], 1).astype(np.int32)
+    return m, 2 * np.exp(-k2[keep] / (4 * alpha))
+
+
+def test_ewald_mode_fp64(orc, gpu):
+    _log("case: full Ewald FP64, 2000 ions")
+    x, y, z, q, L = _ions(2000, seed=21)
+    Lb = np.array([L] * 3)
+    alpha, tol_k = 0.5, 1e-12
+    f = _make(Lb, x.size, precision=64, kspace="EWALD", alpha=alpha, ewald_tol=tol_k)
+    f.Init(_dev(q))
+    F = _compute(f, x, y, z, ev=True)
+    m, w = _halfspace_lattice(Lb, alpha, tol_k)
+    assert f.stats()["ewald_k"] == m.shape[0]
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fp, evp = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, alpha)
+    Fk, evk = orc.kspace_forces(xw, yw, zw, q, Lb, alpha, m, w)
+    assert _rel(F, Fp + Fk) < 1e-12
+    # physics: the half-space sum equals the full-lattice Ewald sum
+    kc = [int(np.ceil(np.sqrt(-4 * alpha * np.log(tol_k)) * L / (2 * np.pi))) + 1] * 3
+    mf, wf = orc.ewald_kvectors(Lb, alpha, kc)
+    Ff, evf = orc.kspace_forces(xw, yw, zw, q, Lb, alpha, mf, wf)
+    assert _rel(F - Fp, Ff) < 1e-9
+    ev = f.energy_virial()
+    assert abs(ev["e_kspace"] - evf[0]) < 1e-9 * abs(evf[0])
+
+
+# ----------------------------------------------------------------------------- MD loop
+
+def _state(n, seed=SEED):
+    from copy_gpu_amd import systems
+
+    x, y, z, q, L = _ions(n, seed=seed)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=seed)
+    return x, y, z, q, L, vx, vy, vz
+
+
+def test_md_graph_equals_eager(gpu):
+    _log("case: 30 MD steps, graph replay vs eager launches")
+    x, y, z, q, L, vx, vy, vz = _state(4000)
+    Lb = np.array([L] * 3)
+    outs = []
+    for use_graph in (False, True):
+        f = _make(Lb, x.size, kspace="RBE", P=100)
+        f.Init(_dev(q))
+        f.sample_kvectors(100, SEED, 0)
+        X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+        F = [torch.zeros(x.size, device="cuda", dtype=torch.float32) for _ in range(3)]
+        f.Execute(X, Y, Z, *F)
+        img = torch.zeros(3 * x.size, device="cuda", dtype=torch.int32)
+        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=0.002, nsteps=30, seed=SEED, step0=1, img=img,
+                 use_graph=use_graph)
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in (X, Y, Z, VX, VY, VZ, *F, img)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_md_matches_oracle_integrator(orc, gpu):
+    """FP64 velocity Verlet with RBE (device-sampled k per step) against the oracle
+    loop kick / drift / forces / kick on the same k-lists."""
+    from copy_gpu_amd import systems
+
+    _log("case: 10 MD steps FP64 RBE vs oracle loop, 2000 ions")
+    n = 2000
+    x, y, z, q, L, vx, vy, vz = _state(n, seed=31)
+    Lb = np.array([L] * 3)
+    lo = np.zeros(3)
+    alpha, P, dt, steps = 0.5, 50, 0.002, 10
+    f = _make(Lb, n, precision=64, kspace="RBE", P=P)
+    f.Init(_dev(q))
+    f.sample_kvectors(P, SEED, 0)
+    X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+    F = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
+    f.Execute(X, Y, Z, *F)
+    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=steps, seed=SEED, step0=1, use_graph=True)
+    torch.cuda.synchronize()
+
+    def forces(px, py, pz, step):
+        m, S = systems.sample_kvectors_host(alpha, Lb, P, SEED, step)
+        Fp, _ = orc.pair_forces(px, py, pz, q, lo, Lb, 2.5, 5.0, alpha)
+        Fk, _ = orc.kspace_forces(px, py, pz, q, Lb, alpha, m, np.full(P, S / P))
+        return Fp + Fk
+
+    p = list(orc.wrap(x, y, z, lo, Lb))
+    v = [vx, vy, vz]
+    Fo = forces(*p, 0)
+    for s in range(1, steps + 1):
+        v = orc.kick(v, [Fo[:, 0], Fo[:, 1], Fo[:, 2]], dt)
+        p, _ = orc.drift(p, v, dt, lo, Lb)
+        Fo = forces(*p, s)
+        v = orc.kick(v, [Fo[:, 0], Fo[:, 1], Fo[:, 2]], dt)
+    Pg = np.stack([X.cpu().numpy(), Y.cpu().numpy(), Z.cpu().numpy()], 1)
+    Po = np.stack(p, 1)
+    dpos = Pg - Po
+    dpos -= Lb * np.rint(dpos / Lb)
+    assert np.abs(dpos).max() < 1e-9, np.abs(dpos).max()
+    Vg = np.stack([VX.cpu().numpy(), VY.cpu().numpy(), VZ.cpu().numpy()], 1)
+    assert np.abs(Vg - np.stack(v, 1)).max() < 1e-8
+
+
+def test_rebuild_trigger(gpu):
+    _log("case: list rebuild only after half-skin displacement")
+    x, y, z, q, L = _ions(2000, seed=41)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size)
+    f.Init(_dev(q))
+    _compute(f, x, y, z)
+    assert f.stats()["builds"] == 1
+    _compute(f, x + 0.1, y, z)  # 0.1 < skin/2 = 0.15
+    assert f.stats()["builds"] == 1
+    x2 = x.copy()
+    x2[5] += 0.16
+    _compute(f, x2, y, z)
+    assert f.stats()["builds"] == 2
+
+
+def test_errors_and_empty(gpu):
+    from copy_gpu_amd import RBMDError
+
+    _log("case: error paths")
+    with pytest.raises(RBMDError, match="3 list radii"):
+        _make(np.array([10.0, 10.0, 10.0]), 10)
+    f = _make(np.array([20.0] * 3), 100, kspace="RBE", P=10)
+    with pytest.raises(RBMDError, match="set_atoms"):
+        _compute(f, np.zeros(0), np.zeros(0), np.zeros(0))
+    with pytest.raises(RBMDError, match="P out of"):
+        f.sample_kvectors(11, 1, 0)
+    f.Init(_dev(np.zeros(1)))
+    with pytest.raises(RBMDError, match="no k-vectors"):
+        _compute(f, np.ones(1), np.ones(1), np.ones(1))
+    g = _make(np.array([20.0] * 3), 100)
+    g.Init(_dev(np.zeros(0)))
+    _compute(g, np.zeros(0), np.zeros(0), np.zeros(0))  # empty system is a no-op
+    with pytest.raises(RBMDError, match="n exceeds"):
+        g.Init(_dev(np.zeros(101)))
+
+
+def test_million_ions_invariants(gpu):
+    """c3 size (1,048,576 ions, alpha 0.3, rc 6.0, P 500): total force sums to zero
+    (pair part antisymmetric, RBE part sum_l k_l (Re Im - Im Re) = 0) and forces
+    are invariant under a common translation of all atoms."""
+    _log("case: 1,048,576 ions, RBE P=500, invariants")
+    n = 1 << 20
+    x, y, z, q, L = _ions(n)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, n, kspace="RBE", P=500, alpha=0.3, rc_coul=6.0)
+    f.Init(_dev(q))
+    f.sample_kvectors(500, SEED, 0)
+    F = _compute(f, x, y, z)
+    scale = np.linalg.norm(F) / np.sqrt(n)
+    assert np.abs(F.sum(0)).max() < 1e-3 * scale * np.sqrt(n)
+    F2 = _compute(f, x + 1.2345, y - 3.5, z + 0.25)
+    assert _rel(F2, F) < 2e-5
+    st = f.stats()
+    assert st["overflow_rows"] == 0
+    mean_row = st["entries"] / n
+    assert abs(mean_row - 0.3 * 4 / 3 * np.pi * 6.3 ** 3) < 5

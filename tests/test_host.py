@@ -1,0 +1,93 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+declared symbol, Philox known answers, deterministic generators."""
+import os
+import re
+import subprocess
+
+import numpy as np
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "rbmd_force.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rbmd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from copy_gpu_amd import _lib
+
+    lib = _lib.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (rbmd_\w+)", out))
+    assert set(names) <= exported, set(names) - exported
+    # the ctypes table binds exactly the declared entry points
+    assert sorted(_lib.exported_symbols()) == names
+
+
+def test_library_has_gfx950_code_object():
+    from copy_gpu_amd import _lib
+
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors (Random123 kat_vectors)."""
+    from copy_gpu_amd import systems
+
+    cases = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in cases:
+        got = systems.philox4x32(ctr, key)
+        assert tuple(int(v) for v in got) == want, [hex(int(v)) for v in got]
+
+
+def test_random_ions_generator(orc):
+    from copy_gpu_amd import systems
+
+    x, y, z, q, Lb = systems.random_ions(3000, 0.3, 0.8, seed=5)
+    assert abs(Lb - (3000 / 0.3) ** (1 / 3)) < 1e-12
+    for a in (x, y, z):
+        assert a.min() >= 0 and a.max() < Lb
+    assert q.sum() == 0 and set(np.unique(q)) == {-1.0, 1.0}
+    Lv = np.array([Lb] * 3)
+    offs, _ = orc.neighbors_brute(x, y, z, Lv, 0.8 - 1e-9)
+    assert offs[-1] == 0  # minimum separation respected
+    x2, y2, z2, q2, _ = systems.random_ions(3000, 0.3, 0.8, seed=5)
+    assert np.array_equal(x, x2) and np.array_equal(q, q2)
+    x3, *_ = systems.random_ions(3000, 0.3, 0.8, seed=6)
+    assert not np.array_equal(x, x3)
+
+
+def test_velocities_generator():
+    from copy_gpu_amd import systems
+
+    n = 5000
+    vx, vy, vz = systems.velocities(n, 1.0, seed=3)
+    assert abs(vx.sum()) < 1e-9 and abs(vy.sum()) < 1e-9 and abs(vz.sum()) < 1e-9
+    T = (vx @ vx + vy @ vy + vz @ vz) / (3 * n - 3)
+    assert abs(T - 1.0) < 1e-12
+    # Gaussian marginals
+    assert abs(np.mean(vx ** 4) / np.mean(vx ** 2) ** 2 - 3.0) < 0.25
+
+
+def test_fcc_generator():
+    from copy_gpu_amd import systems
+
+    x, y, z, Lb = systems.fcc(10, 0.8442)
+    assert x.size == 4000
+    assert abs(Lb - 10 * (4 / 0.8442) ** (1 / 3)) < 1e-12
+    assert abs(4000 / Lb ** 3 - 0.8442) < 1e-12
