@@ -47,6 +47,7 @@ class ForceParams(C.Structure):
         ("p_max", C.c_int32),
         ("ewald_tol", C.c_double),
         ("list_capacity", C.c_int32),
+        ("tile_stage", C.c_int32),
     ]
 
 

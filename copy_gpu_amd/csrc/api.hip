@@ -105,13 +105,14 @@ static void free_all(Engine& E)
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
-                    E.ev_part,  E.ev};
+                    E.ev_part,  E.ev,    E.big_tiles};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (E.h_stats) hipHostFree(E.h_stats);
     if (E.graph_exec) hipGraphExecDestroy(E.graph_exec);
     if (E.graph) hipGraphDestroy(E.graph);
     if (E.own_stream) hipStreamDestroy(E.own_stream);
+    if (E.order_event) hipEventDestroy(E.order_event);
 }
 
 static int alloc_list(Engine& E, int cap)
@@ -119,7 +120,7 @@ static int alloc_list(Engine& E, int cap)
     if (E.nlist) hipFree(E.nlist);
     E.nlist = nullptr;
     E.cap = cap;
-    HIP_TRY(dalloc(&E.nlist, (size_t)E.n_max * (size_t)cap * sizeof(int32_t)));
+    HIP_TRY(dalloc(&E.nlist, (size_t)E.n_max * (size_t)cap * sizeof(uint16_t)));
     return RBMD_OK;
 }
 
@@ -169,6 +170,29 @@ static void ewald_lattice(const Engine& E, std::vector<int32_t>& m, std::vector<
                 m.push_back(g);
                 w.push_back(2.0 * exp(-k2 / (4.0 * E.alpha)));
             }
+}
+
+// Cell-tile geometry for n atoms: workgroups per tile (enough tiles to load
+// every CU several times) and the LDS stage, 1.25x the mean stencil population
+// (the rare larger tile reads its candidates from global memory).
+static void tile_geometry(Engine& E, int64_t n)
+{
+    const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
+    const double cellv = E.cell[0] * E.cell[1] * E.cell[2];
+    const double mean = 27.0 * (double)n / V * cellv;
+    int cap = ((int)(1.25 * mean + 64.0) + 63) / 64 * 64;
+    const int per = (int)sizeof(double4) + (E.pp.ntypes > 1 ? 4 : 0);
+    const int max_entries = (E.lds_max - 4096) / per;
+    if (E.stage_override > 0) cap = E.stage_override;
+    E.stage_cap = std::max(64, std::min(cap, max_entries));
+    // tiles resident at once (LDS-limited); split a tile over several
+    // workgroups only when there are too few tiles to fill that
+    const int lds_wg = E.stage_cap * per + 2048;
+    const int per_cu = std::max(1, std::min(4, E.lds_cu / lds_wg));
+    const int resident = std::max(E.num_cu, 1) * per_cu;
+    E.split = 1;
+    if (4 * E.ncell < 3 * resident)
+        E.split = std::min(4, (resident + E.ncell - 1) / E.ncell);
 }
 
 extern "C" {
@@ -225,6 +249,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     E.r_list = std::max(p->rc_lj, p->rc_coul) + p->skin;
     E.r_in = p->rc_lj + p->skin;
     E.n_max = n_max;
+    E.stage_override = p->tile_stage;
     for (int d = 0; d < 3; ++d) {
         E.dims[d] = (int)(E.box.L[d] / E.r_list);
         E.cell[d] = E.box.L[d] / (E.dims[d] > 0 ? E.dims[d] : 1);
@@ -274,6 +299,17 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
         return bail(RBMD_ERR_HIP);                                \
     }
     const size_t N = (size_t)std::max<int64_t>(n_max, 1);
+    {
+        int dev = 0, v = 0;
+        CR(hipGetDevice(&dev));
+        CR(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        E.lds_max = v;
+        CR(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        E.num_cu = v;
+        CR(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+        E.lds_cu = v > 0 ? v : E.lds_max;
+    }
+    tile_geometry(E, n_max);
     CR(dalloc(&E.xr, N * 8));
     CR(dalloc(&E.yr, N * 8));
     CR(dalloc(&E.zr, N * 8));
@@ -286,20 +322,22 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.atom_rank, N * 4));
     CR(dalloc(&E.cell_count, (size_t)E.ncell * 4));
     CR(dalloc(&E.cell_start, ((size_t)E.ncell + 1) * 4));
+    CR(dalloc(&E.big_tiles, (size_t)E.ncell * 4));
     CR(dalloc(&E.ncount, N * sizeof(int2)));
     CR(dalloc(&E.flags, 64));
-    CR(dalloc(&E.stats, 64));
+    CR(dalloc(&E.stats, 128));
     CR(dalloc(&E.q, N * 8));
     CR(dalloc(&E.type, N * 4));
     CR(dalloc(&E.d_pp, sizeof(PairParams)));
     CR(dalloc(&E.d_step, 64));
     CR(dalloc(&E.ev, 16 * 8));
-    E.ev_part_blocks = (int)((N + 3) / 4);
+    E.ev_part_blocks = E.ncell * 4 + 64;  // staged pass (split <= 4) + big-tile pass
     CR(dalloc(&E.ev_part, (size_t)E.ev_part_blocks * 8 * 8));
     CR(hipHostMalloc((void**)&E.h_stats, 64, hipHostMallocDefault));
     CR(hipStreamCreateWithFlags(&E.own_stream, hipStreamNonBlocking));
+    CR(hipEventCreateWithFlags(&E.order_event, hipEventDisableTiming));
     CR(hipMemset(E.flags, 0, 64));
-    CR(hipMemset(E.stats, 0, 64));
+    CR(hipMemset(E.stats, 0, 128));
     CR(hipMemset(E.d_step, 0, 64));
     CR(hipMemset(E.ev, 0, 16 * 8));
     CR(hipMemset(E.cell_start, 0, ((size_t)E.ncell + 1) * 4));
@@ -348,6 +386,7 @@ int rbmd_force_set_atoms(rbmd_force_t h, int64_t n, const double* q, const int32
     if (n > 0 && !q) return fail(E, RBMD_ERR_INVALID, "charges required");
     hipStream_t s = (hipStream_t)stream;
     E.n = n;
+    tile_geometry(E, n);
     if (n > 0) {
         HIP_TRY(hipMemcpyAsync(E.q, q, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
         if (type) {
@@ -406,8 +445,12 @@ int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_
     return RBMD_OK;
 }
 
+// One force evaluation; with kick pointers (MD mode) the k-space kernel also
+// applies the closing half kick, otherwise the caller kicks separately.
 static int force_pass(Engine& E, const double* x, const double* y, const double* z, void* fx,
-                      void* fy, void* fz, bool ev, hipStream_t s, bool checked)
+                      void* fy, void* fz, bool ev, hipStream_t s, bool checked,
+                      double* vx = nullptr, double* vy = nullptr, double* vz = nullptr,
+                      const double* mass = nullptr, double dt = 0.0)
 {
     if (!checked) launch_check(E, x, y, z, s);
     launch_build(E, x, y, z, s);
@@ -415,7 +458,7 @@ static int force_pass(Engine& E, const double* x, const double* y, const double*
     const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
     if (ks) launch_rho(E, x, y, z, s);
     launch_pair(E, fx, fy, fz, ev, s);
-    if (ks) launch_kforce(E, x, y, z, fx, fy, fz, s);
+    if (ks) launch_kforce(E, x, y, z, fx, fy, fz, vx, vy, vz, mass, dt, s);
     if (ev) launch_ev_final(E, true, s);
     return RBMD_OK;
 }
@@ -432,7 +475,9 @@ static int maintain(Engine& E, hipStream_t s)
         E.h_stats[2] = 0;
         return RBMD_OK;
     }
-    const int newcap = ((int)(E.cap * 1.5) + 63) / 64 * 64;
+    // longest row of the last build (stats[4]) plus 20% headroom
+    const int want = (int)(1.2 * (double)st[4]) + 16;
+    const int newcap = (std::max((int)(E.cap * 1.5), want) + 63) / 64 * 64;
     int rc = alloc_list(E, newcap);
     if (rc != RBMD_OK) return rc;
     E.h_stats[2] = 0;
@@ -478,10 +523,26 @@ static void md_step(Engine& E, double* x, double* y, double* z, double* vx, doub
                     double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
                     double dt, uint64_t seed, const uint64_t* step_host, hipStream_t s)
 {
-    launch_kick_drift_check(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, s);
-    if (E.kspace == RBMD_KSPACE_RBE) launch_sample(E, E.P, seed, step_host, s);
-    force_pass(E, x, y, z, fx, fy, fz, false, s, true);
-    launch_kick(E, vx, vy, vz, fx, fy, fz, mass, dt, s);
+    SampleArgs sa;
+    if (E.kspace == RBMD_KSPACE_RBE) {
+        sa.P = E.P;
+        sa.seed = seed;
+        sa.step = step_host ? *step_host : 0ull;
+        sa.step_dev = step_host ? nullptr : E.d_step;
+        sa.cdf = E.kcdf;
+        sa.M = make_int3(E.kM[0], E.kM[1], E.kM[2]);
+        sa.w = E.S / (double)E.P;
+        sa.km = E.km;
+        sa.kw = E.kw;
+    }
+    launch_kick_drift_check(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa, s);
+    const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
+    if (ks) {
+        force_pass(E, x, y, z, fx, fy, fz, false, s, true, vx, vy, vz, mass, dt);
+    } else {
+        force_pass(E, x, y, z, fx, fy, fz, false, s, true);
+        launch_kick(E, vx, vy, vz, fx, fy, fz, mass, dt, s);
+    }
 }
 
 int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, double* vy,
@@ -495,7 +556,16 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
         return fail(E, RBMD_ERR_STATE, "no k-vectors: sample once to fix P");
     if (E.n == 0 || nsteps <= 0) return RBMD_OK;
-    hipStream_t s = stream ? (hipStream_t)stream : E.own_stream;
+    // Eager steps run on the caller's stream.  Graph replay needs a capturable
+    // (non-null) stream: with a null caller stream the handle's own stream is
+    // used, ordered after the caller's earlier work through an event.
+    hipStream_t cs = (hipStream_t)stream;
+    hipStream_t s = cs;
+    if (use_graph && !cs) {
+        s = E.own_stream;
+        HIP_TRY(hipEventRecord(E.order_event, cs));
+        HIP_TRY(hipStreamWaitEvent(s, E.order_event, 0));
+    }
     int rc = maintain(E, s);
     if (rc != RBMD_OK) return rc;
     if (!use_graph) {
@@ -550,25 +620,50 @@ int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int
     Engine& E = h->e;
     HIP_TRY(hipDeviceSynchronize());
     const int64_t n = E.n;
-    std::vector<int32_t> perm((size_t)n), rows((size_t)n * E.cap);
+    std::vector<int32_t> perm((size_t)n), scell((size_t)n), cs((size_t)E.ncell + 1);
+    std::vector<uint16_t> rows((size_t)n * E.cap);
     std::vector<int2> nc((size_t)n);
     HIP_TRY(hipMemcpy(perm.data(), E.perm, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(scell.data(), E.scell, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cs.data(), E.cell_start, cs.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(nc.data(), E.ncount, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(rows.data(), E.nlist, (size_t)n * E.cap * 4, hipMemcpyDeviceToHost));
-    // rows are indexed by sorted slot; emit them in original atom order
+    HIP_TRY(hipMemcpy(rows.data(), E.nlist, rows.size() * 2, hipMemcpyDeviceToHost));
+    // rows hold tile-local indices of the atom's cell tile (stencil order of tile.h)
+    auto tile = [&](int c, int off[28], int gstart[27]) {
+        const int dx = E.dims[0], dy = E.dims[1], dz = E.dims[2];
+        const int cz = c / (dx * dy), cy = (c / dx) % dy, cx = c % dx;
+        int o = 0;
+        for (int t = 0; t < 27; ++t) {
+            const int ax = (cx + t / 9 - 1 + dx) % dx, ay = (cy + (t / 3) % 3 - 1 + dy) % dy,
+                      az = (cz + t % 3 - 1 + dz) % dz;
+            const int cc = ax + ay * dx + az * dx * dy;
+            off[t] = o;
+            gstart[t] = cs[(size_t)cc];
+            o += cs[(size_t)cc + 1] - cs[(size_t)cc];
+        }
+        off[27] = o;
+    };
     std::vector<int64_t> slot_of((size_t)n);
     for (int64_t s = 0; s < n; ++s) slot_of[(size_t)perm[(size_t)s]] = s;
     int64_t e = 0;
     offsets[0] = 0;
     bool overflow = false;
+    int off[28], gst[27], last_c = -1;
     for (int64_t i = 0; i < n; ++i) {
         const int64_t s = slot_of[(size_t)i];
         const int ni = nc[(size_t)s].x, no = nc[(size_t)s].y;
         if (ni + no > E.cap) overflow = true;
-        const int32_t* row = rows.data() + (size_t)s * E.cap;
+        const int c = scell[(size_t)s];
+        if (c != last_c) {
+            tile(c, off, gst);
+            last_c = c;
+        }
+        const uint16_t* row = rows.data() + (size_t)s * E.cap;
         for (int k = 0; k < ni + no && !overflow; ++k) {
-            const int idxk = k < ni ? k : E.cap - 1 - (k - ni);
-            if (idx && e < cap) idx[e] = perm[(size_t)row[idxk]];
+            const int u = row[k < ni ? k : E.cap - 1 - (k - ni)];
+            int st = 26;
+            while (off[st] > u) --st;
+            if (idx && e < cap) idx[e] = perm[(size_t)(gst[st] + (u - off[st]))];
             ++e;
         }
         offsets[i + 1] = e;
@@ -606,14 +701,40 @@ int rbmd_force_get_stats(rbmd_force_t h, int64_t* out)
     HIP_TRY(hipDeviceSynchronize());
     unsigned long long st[8];
     HIP_TRY(hipMemcpy(st, E.stats, 64, hipMemcpyDeviceToHost));
+    unsigned long long ls[3] = {0, 0, 0};
+    if (E.n > 0) {
+        launch_list_stats(E, E.stats + 8, nullptr);
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy(ls, E.stats + 8, sizeof ls, hipMemcpyDeviceToHost));
+    }
     out[0] = (int64_t)st[0];
-    out[1] = (int64_t)st[1];
+    out[1] = (int64_t)ls[0];
     out[2] = E.cap;
     out[3] = (int64_t)st[2];
     out[4] = E.ncell;
-    out[5] = (int64_t)st[3];
+    out[5] = (int64_t)ls[1];
     out[6] = E.kspace == RBMD_KSPACE_EWALD ? E.P : 0;
     out[7] = E.n;
+    out[8] = (int64_t)st[6];
+    out[9] = E.stage_cap;
+    // largest tile: stencil populations from the cell bounds
+    std::vector<int32_t> cs((size_t)E.ncell + 1);
+    HIP_TRY(hipMemcpy(cs.data(), E.cell_start, cs.size() * 4, hipMemcpyDeviceToHost));
+    int64_t big = 0;
+    const int dx = E.dims[0], dy = E.dims[1], dz = E.dims[2];
+    for (int c = 0; c < E.ncell; ++c) {
+        const int cz = c / (dx * dy), cy = (c / dx) % dy, cx = c % dx;
+        int64_t tot = 0;
+        for (int t = 0; t < 27; ++t) {
+            const int ax = (cx + t / 9 - 1 + dx) % dx, ay = (cy + (t / 3) % 3 - 1 + dy) % dy,
+                      az = (cz + t % 3 - 1 + dz) % dz;
+            const int cc = ax + ay * dx + az * dx * dy;
+            tot += cs[(size_t)cc + 1] - cs[(size_t)cc];
+        }
+        big = std::max(big, tot);
+    }
+    out[10] = big;
+    out[11] = E.split;
     return RBMD_OK;
 }
 

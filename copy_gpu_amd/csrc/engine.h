@@ -138,6 +138,50 @@ __device__ inline T wave_sum(T v)
 }
 
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Wave64 sum delivered to every lane, with DPP lane moves instead of LDS
+// permutes: quad xor 1, quad xor 2, half-row mirror, row mirror (each lane then
+// holds its 16-lane row sum), row_bcast15 / row_bcast31 (lane 63 holds the
+// total), then a readlane.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+__device__ inline float wave_sum_dpp(float v)
+{
+    v += __int_as_float(dpp_i<0xB1>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x4E>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x141>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x140>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x142, 0xa>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x143, 0xc>(__float_as_int(v)));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL, ROW_MASK>((int)(b & 0xffffffffll));
+    const int hi = dpp_i<CTRL, ROW_MASK>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ inline double wave_sum_dpp(double v)
+{
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 #endif
 
 // ---------------------------------------------------------------- timing

@@ -10,8 +10,8 @@
 //   Ewald force / energy            ExecForceFunction.h:226-245, output/ThermoOutput.cpp:209-255
 //
 // Phases are formed from FP64 fractional coordinates, phi = frac(m . s) with
-// s = (x - lo)/L, so FP32 sines keep full accuracy at any |k.r|; sin/cos are
-// sincospi(2 phi).  FP32 mode accumulates rho with Kahan compensation per lane
+// s = (x - lo)/L, so FP32 sines keep full accuracy at any |k.r|; sin/cos of the
+// turn fraction come from the hardware sine/cosine in FP32 and sincospi in FP64.  FP32 mode accumulates rho with Kahan compensation per lane
 // and reduces the per-block partials in FP64, deterministically.
 #include "state.h"
 
@@ -49,11 +49,15 @@ __global__ void k_fill(double* __restrict__ a, int n, double v)
 template <typename T>
 __device__ inline void sincos2pi(double phi, T* s, T* c);
 
+// FP32: the hardware sine/cosine take their argument in turns (sin(2 pi x));
+// on [-0.5, 0.5] turns their error is 1.3e-7 absolute (measured against FP64),
+// the size of the FP32 rounding of the turn fraction itself.
 template <>
 __device__ inline void sincos2pi<float>(double phi, float* s, float* c)
 {
     const float f = (float)(phi - rint(phi));  // [-0.5, 0.5] turns
-    sincospif(2.0f * f, s, c);
+    *s = __builtin_amdgcn_sinf(f);
+    *c = __builtin_amdgcn_cosf(f);
 }
 
 template <>
@@ -213,36 +217,42 @@ __global__ __launch_bounds__(256) void k_rho_final(
 }
 
 // ---------------------------------------------------------------- k-space force
-// Thread per atom (original order): F_i += q_i sum_l k_l (cos(phi) A_l - sin(phi) B_l).
-// Coefficients are staged through LDS 256 k at a time.
-constexpr int kForceBlock = 64;
-constexpr int kForceStage = 256;
+// Block = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes
+// k = w, w+8, ... of the coefficients staged in LDS (every lane reads the same
+// k: broadcast), then the 8 partial forces of each atom are summed in a fixed
+// order.  F_i += q_i sum_k k (cos(phi) A_k - sin(phi) B_k).  With KICK the
+// final half kick v += dt/2 F/m of the velocity-Verlet step is fused in.
+constexpr int kKfAtoms = 64;
+constexpr int kKfSlices = 8;
+constexpr int kKfThreads = kKfAtoms * kKfSlices;
+constexpr int kKfStage = 512;
 
-template <typename T>
-__global__ __launch_bounds__(kForceBlock) void k_kforce(
+template <typename T, bool KICK>
+__global__ __launch_bounds__(kKfThreads) void k_kforce(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ q, int n, Box box, const KCoef<T>* __restrict__ coef, int P,
-    T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ fz)
+    T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ fz, double* __restrict__ vx,
+    double* __restrict__ vy, double* __restrict__ vz, const double* __restrict__ mass, double dt)
 {
 #pragma clang fp contract(off)
-    __shared__ KCoef<T> st[kForceStage];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ KCoef<T> st[kKfStage];
+    __shared__ T red[kKfSlices][3][kKfAtoms];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * kKfAtoms + lane;
     double sx = 0, sy = 0, sz = 0;
-    T qi = 0;
     if (i < n) {
         sx = (x[i] - box.lo[0]) * box.invL[0];
         sy = (y[i] - box.lo[1]) * box.invL[1];
         sz = (z[i] - box.lo[2]) * box.invL[2];
-        qi = (T)q[i];
     }
     T ax = 0, ay = 0, az = 0;
-    for (int base = 0; base < P; base += kForceStage) {
+    for (int base = 0; base < P; base += kKfStage) {
         __syncthreads();
-        for (int k = threadIdx.x; k < kForceStage && base + k < P; k += kForceBlock)
+        for (int k = threadIdx.x; k < kKfStage && base + k < P; k += kKfThreads)
             st[k] = coef[base + k];
         __syncthreads();
-        const int cnt = (P - base) < kForceStage ? (P - base) : kForceStage;
-        for (int k = 0; k < cnt; ++k) {
+        const int cnt = (P - base) < kKfStage ? (P - base) : kKfStage;
+        for (int k = wv; k < cnt; k += kKfSlices) {
             const KCoef<T>& c = st[k];
             const double phi = fma(c.m[2], sz, fma(c.m[1], sy, c.m[0] * sx));
             T sn, cs;
@@ -253,10 +263,28 @@ __global__ __launch_bounds__(kForceBlock) void k_kforce(
             az += g * c.k[2];
         }
     }
-    if (i < n) {
-        fx[i] += qi * ax;
-        fy[i] += qi * ay;
-        fz[i] += qi * az;
+    red[wv][0][lane] = ax;
+    red[wv][1][lane] = ay;
+    red[wv][2][lane] = az;
+    __syncthreads();
+    if (wv == 0 && i < n) {
+        T s0 = 0, s1 = 0, s2 = 0;
+        for (int w = 0; w < kKfSlices; ++w) {
+            s0 += red[w][0][lane];
+            s1 += red[w][1][lane];
+            s2 += red[w][2][lane];
+        }
+        const T qi = (T)q[i];
+        const T f0 = fx[i] + qi * s0, f1 = fy[i] + qi * s1, f2 = fz[i] + qi * s2;
+        fx[i] = f0;
+        fy[i] = f1;
+        fz[i] = f2;
+        if (KICK) {
+            const double h = 0.5 * dt / (mass ? mass[i] : 1.0);
+            vx[i] += h * (double)f0;
+            vy[i] += h * (double)f1;
+            vz[i] += h * (double)f2;
+        }
     }
 }
 
@@ -320,19 +348,25 @@ void launch_rho(Engine& e, const double* x, const double* y, const double* z, hi
 }
 
 void launch_kforce(Engine& e, const double* x, const double* y, const double* z, void* fx,
-                   void* fy, void* fz, hipStream_t s)
+                   void* fy, void* fz, double* vx, double* vy, double* vz, const double* mass,
+                   double dt, hipStream_t s)
 {
     ScopedTimer t(e, P_KFORCE, s);
     const int n = (int)e.n;
-    const int blocks = (n + kForceBlock - 1) / kForceBlock;
-    if (e.precision == 32)
-        k_kforce<float><<<blocks, kForceBlock, 0, s>>>(x, y, z, e.q, n, e.box,
-                                                        (const KCoef<float>*)e.kcoef, e.P,
-                                                        (float*)fx, (float*)fy, (float*)fz);
-    else
-        k_kforce<double><<<blocks, kForceBlock, 0, s>>>(x, y, z, e.q, n, e.box,
-                                                         (const KCoef<double>*)e.kcoef, e.P,
-                                                         (double*)fx, (double*)fy, (double*)fz);
+    const int blocks = (n + kKfAtoms - 1) / kKfAtoms;
+    const bool kick = vx != nullptr;
+#define RBMD_KF(T, K)                                                                          \
+    k_kforce<T, K><<<blocks, kKfThreads, 0, s>>>(x, y, z, e.q, n, e.box,                       \
+                                                 (const KCoef<T>*)e.kcoef, e.P, (T*)fx, (T*)fy, \
+                                                 (T*)fz, vx, vy, vz, mass, dt)
+    if (e.precision == 32) {
+        if (kick) RBMD_KF(float, true);
+        else RBMD_KF(float, false);
+    } else {
+        if (kick) RBMD_KF(double, true);
+        else RBMD_KF(double, false);
+    }
+#undef RBMD_KF
 }
 
 }  // namespace rbmd

@@ -9,6 +9,7 @@
 // The list radius adds a Verlet skin; the list is rebuilt on the device when an
 // atom has moved more than half the skin since the last build.
 #include "state.h"
+#include "tile.h"
 
 namespace rbmd {
 
@@ -106,6 +107,8 @@ __global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ count
         stats[1] = 0ull;
         stats[2] = 0ull;
         stats[3] = 0ull;
+        stats[4] = 0ull;
+        stats[6] = 0ull;
     }
     __syncthreads();
     for (int base = 0; base < ncell; base += 1024) {
@@ -185,73 +188,149 @@ __global__ __launch_bounds__(64) void k_cellsort(
 }
 
 // ---------------------------------------------------------------- Verlet list
-// One wave per sorted atom i.  The 27 stencil cells are scanned 64 candidates
-// at a time; a candidate j is kept iff (r_list^2 - d2 > 1e-4) && (d2 > 1e-4)
-// with dx = (x_j - x_i) - k L, k the image of the stencil cell, and
-// d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the exact bits the CPU oracle uses.
-// Hits are compacted with ballot + mbcnt: LJ-range hits (d2 < r_in^2) from
-// the row start, the others from the row end.
-__global__ __launch_bounds__(kBlock) void k_list(
-    const double4* __restrict__ xs_ref, const int32_t* __restrict__ scell,
-    const int32_t* __restrict__ cell_start, int n, int3 dims, Box box, double rl2, double rin2,
-    int cap, int32_t* __restrict__ nlist, int2* __restrict__ ncount,
-    const uint32_t* __restrict__ flags, unsigned long long* __restrict__ stats)
+// One workgroup per (home cell, part): the tile's reference positions are
+// staged in LDS already moved to the home cell's image, x_j - s (s the image
+// shift of j's stencil cell), then one wave per home atom i sweeps the whole
+// tile 64 candidates at a time.  A candidate is kept iff
+// (r_list^2 - d2 > 1e-4) && (d2 > 1e-4) with dx = (x_j - s) - x_i and
+// d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the reference's operation order
+// (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  Hits are
+// compacted with ballot + mbcnt into the row as uint16 tile-local indices:
+// LJ-range hits (d2 < r_in^2) from the row start, the others from the row end.
+// Tiles larger than the LDS stage read candidates from global memory; tiles
+// beyond the uint16 range mark their rows overflowed.
+__device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict__ xs_ref, int u)
 {
 #pragma clang fp contract(off)
-    if (flags[0] == 0u) return;
-    const int lane = threadIdx.x & 63;
-    const int s = uniform(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
-    if (s >= n) return;
-    const double4 pi = xs_ref[s];
-    const int c = uniform(scell[s]);
-    const int cz = c / (dims.x * dims.y);
-    const int cy = (c / dims.x) % dims.y;
-    const int cx = c % dims.x;
-    int n_in = 0, n_out = 0;
-    int32_t* __restrict__ row = nlist + (size_t)s * cap;
-    for (int st = 0; st < 27; ++st) {
-        int ax = cx + st / 9 - 1, ay = cy + (st / 3) % 3 - 1, az = cz + st % 3 - 1;
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        if (ax < 0) { ax += dims.x; sx = box.L[0]; }
-        else if (ax >= dims.x) { ax -= dims.x; sx = -box.L[0]; }
-        if (ay < 0) { ay += dims.y; sy = box.L[1]; }
-        else if (ay >= dims.y) { ay -= dims.y; sy = -box.L[1]; }
-        if (az < 0) { az += dims.z; sz = box.L[2]; }
-        else if (az >= dims.z) { az -= dims.z; sz = -box.L[2]; }
-        const int cc = ax + ay * dims.x + az * dims.x * dims.y;
-        const int j0 = cell_start[cc], j1 = cell_start[cc + 1];
-        for (int jb = j0; jb < j1; jb += 64) {
-            const int j = jb + lane;
-            bool hit = false, inner = false;
-            if (j < j1) {
-                const double4 pj = xs_ref[j];
-                const double dx = (pj.x - pi.x) - sx;
-                const double dy = (pj.y - pi.y) - sy;
-                const double dz = (pj.z - pi.z) - sz;
+    const int st = tile_seg(h, u);
+    const double4 p = xs_ref[h.gstart[st] + (u - h.off[st])];
+    return make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+}
+
+template <bool STAGED>
+__device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ lds,
+                                 const double4* __restrict__ xs_ref, int first, int a1, int h0,
+                                 int lane, double rl2, double rin2, int cap,
+                                 uint16_t* __restrict__ nlist, int2* __restrict__ ncount,
+                                 unsigned long long* __restrict__ stats)
+{
+#pragma clang fp contract(off)
+    const int total = h.off[27];
+    for (int s = first; s < a1; s += kTileWaves) {
+        if (total > 65535) {
+            if (lane == 0) {
+                ncount[s] = make_int2(cap + 1, 0);
+                atomicAdd(&stats[2], 1ull);
+                atomicMax(&stats[4], (unsigned long long)(cap + 1));
+            }
+            continue;
+        }
+        const int ui = h.off[kHomeSeg] + (s - h0);
+        const double4 pi = STAGED ? lds[ui] : xs_ref[s];
+        int n_in = 0, n_out = 0;
+        uint16_t* __restrict__ row = nlist + (size_t)s * cap;
+        for (int ub = 0; ub < total; ub += 128) {
+            double4 pj[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = ub + 64 * k + lane;
+                const int uu = u < total ? u : 0;
+                pj[k] = STAGED ? lds[uu] : stage_ref(h, xs_ref, uu);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = ub + 64 * k + lane;
+                const double dx = pj[k].x - pi.x;
+                const double dy = pj[k].y - pi.y;
+                const double dz = pj[k].z - pi.z;
                 const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                hit = ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
-                inner = d2 < rin2;
-            }
-            const uint64_t mi = __ballot(hit && inner);
-            const uint64_t mo = __ballot(hit && !inner);
-            if (hit) {
-                if (inner) {
-                    const int p = n_in + (int)mbcnt64(mi);
-                    if (p < cap) row[p] = j;
-                } else {
-                    const int p = n_out + (int)mbcnt64(mo);
-                    if (p < cap) row[cap - 1 - p] = j;
+                const bool hit = (u < total) && ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
+                const bool inner = d2 < rin2;
+                const uint64_t mi = __ballot(hit && inner);
+                const uint64_t mo = __ballot(hit && !inner);
+                if (hit) {
+                    if (inner) {
+                        const int p = n_in + (int)mbcnt64(mi);
+                        if (p < cap) row[p] = (uint16_t)u;
+                    } else {
+                        const int p = n_out + (int)mbcnt64(mo);
+                        if (p < cap) row[cap - 1 - p] = (uint16_t)u;
+                    }
                 }
+                n_in += __popcll(mi);
+                n_out += __popcll(mo);
             }
-            n_in += __popcll(mi);
-            n_out += __popcll(mo);
+        }
+        if (lane == 0) {
+            ncount[s] = make_int2(n_in, n_out);
+            if (n_in + n_out > cap) {
+                atomicAdd(&stats[2], 1ull);
+                atomicMax(&stats[4], (unsigned long long)(n_in + n_out));
+            }
         }
     }
-    if (lane == 0) {
-        ncount[s] = make_int2(n_in, n_out);
-        atomicAdd(&stats[1], (unsigned long long)(n_in + n_out));
-        atomicAdd(&stats[3], (unsigned long long)n_in);
-        if (n_in + n_out > cap) atomicAdd(&stats[2], 1ull);
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_list_tile(
+    const double4* __restrict__ xs_ref, const int32_t* __restrict__ cell_start, int split,
+    int3 dims, Box box, double rl2, double rin2, int cap, int stage_cap,
+    uint16_t* __restrict__ nlist, int2* __restrict__ ncount, const uint32_t* __restrict__ flags,
+    unsigned long long* __restrict__ stats, int32_t* __restrict__ big_tiles)
+{
+    if (flags[0] == 0u) return;
+    extern __shared__ double4 lds[];
+    __shared__ TileHdr h;
+    const int c = blockIdx.x / split, part = blockIdx.x % split;
+    tile_setup(c, dims, box, cell_start, h);
+    const int total = h.off[27];
+    const bool staged = total <= stage_cap;
+    if (!staged && part == 0 && threadIdx.x == 0)
+        big_tiles[atomicAdd(&stats[6], 1ull)] = c;  // served by the global-memory pair pass
+    if (staged) {
+        for (int u = threadIdx.x; u < total; u += kTileThreads) lds[u] = stage_ref(h, xs_ref, u);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wv = uniform(threadIdx.x >> 6);
+    const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+    const int a0 = h0 + (hc * part) / split, a1 = h0 + (hc * (part + 1)) / split;
+    if (staged)
+        list_wave<true>(h, lds, xs_ref, a0 + wv, a1, h0, lane, rl2, rin2, cap, nlist, ncount,
+                        stats);
+    else
+        list_wave<false>(h, lds, xs_ref, a0 + wv, a1, h0, lane, rl2, rin2, cap, nlist, ncount,
+                         stats);
+}
+
+// List statistics on demand: entries, LJ-range entries, longest row.
+__global__ __launch_bounds__(256) void k_list_stats(const int2* __restrict__ ncount, int n,
+                                                    unsigned long long* __restrict__ out)
+{
+    __shared__ unsigned long long se[256], si[256], sm[256];
+    unsigned long long e = 0, in = 0, mx = 0;
+    for (int s = threadIdx.x; s < n; s += 256) {
+        const int2 c = ncount[s];
+        e += (unsigned long long)(c.x + c.y);
+        in += (unsigned long long)c.x;
+        mx = mx > (unsigned long long)(c.x + c.y) ? mx : (unsigned long long)(c.x + c.y);
+    }
+    se[threadIdx.x] = e;
+    si[threadIdx.x] = in;
+    sm[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            se[threadIdx.x] += se[threadIdx.x + o];
+            si[threadIdx.x] += si[threadIdx.x + o];
+            sm[threadIdx.x] = sm[threadIdx.x] > sm[threadIdx.x + o] ? sm[threadIdx.x]
+                                                                    : sm[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = se[0];
+        out[1] = si[0];
+        out[2] = sm[0];
     }
 }
 
@@ -276,6 +355,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(
 }
 
 // ---------------------------------------------------------------- integrator
+constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sampler
+
+// First half of the velocity-Verlet step: v += dt/2 F/m, x += v dt, wrap with
+// image flags, displacement check against the list build (and zeroing of the
+// cell histogram for a possible rebuild).  The first blocks also draw this
+// step's k-vectors (the sampler of k_sample) so that no extra launch is needed.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_kick_drift_check(
     double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
@@ -283,10 +368,32 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift_check(
     const T* __restrict__ fx, const T* __restrict__ fy, const T* __restrict__ fz,
     int32_t* __restrict__ img, const double* __restrict__ mass, double dt, int n, Box box,
     const double* __restrict__ xr, const double* __restrict__ yr, const double* __restrict__ zr,
-    double thr2, uint32_t* __restrict__ flags, int32_t* __restrict__ cell_count, int ncell)
+    double thr2, uint32_t* __restrict__ flags, int32_t* __restrict__ cell_count, int ncell,
+    SampleArgs sa)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ncell) cell_count[i] = 0;
+    if (blockIdx.x < (unsigned)((sa.P + 63) / 64)) {
+        // the first ceil(P/64) blocks also draw this step's k-vectors (64 per
+        // block, one wave), with the CDF tables staged in LDS
+        __shared__ double cdf[kSamplerLds];
+        const int len = 2 * (sa.M.x + sa.M.y + sa.M.z) + 3;
+        const bool in_lds = len <= kSamplerLds;
+        if (in_lds)
+            for (int k = threadIdx.x; k < len; k += blockDim.x) cdf[k] = sa.cdf[k];
+        __syncthreads();
+        const int l = blockIdx.x * 64 + threadIdx.x;
+        if (threadIdx.x < 64 && l < sa.P) {
+            const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
+            const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
+            int m[3];
+            sample_kvector(sa.seed, step, (uint32_t)l, in_lds ? cdf : sa.cdf, Ms, m);
+            sa.km[3 * l] = m[0];
+            sa.km[3 * l + 1] = m[1];
+            sa.km[3 * l + 2] = m[2];
+            sa.kw[l] = sa.w;
+        }
+    }
     if (i >= n) return;
     const double h = 0.5 * dt / (mass ? mass[i] : 1.0);
     double v[3] = {vx[i] + h * (double)fx[i], vy[i] + h * (double)fy[i], vz[i] + h * (double)fz[i]};
@@ -386,10 +493,15 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
         const double rl2 = e.r_list * e.r_list;
         const double rin2 = e.r_in * e.r_in;
         ScopedTimer t(e, P_LIST, s);
-        k_list<<<nblk(n, kBlock / 64), kBlock, 0, s>>>(e.xs_ref, e.scell, e.cell_start, n, dims,
-                                                         e.box, rl2, rin2, e.cap, e.nlist,
-                                                         e.ncount, e.flags, e.stats);
+        k_list_tile<<<e.ncell * e.split, kTileThreads, (size_t)e.stage_cap * sizeof(double4), s>>>(
+            e.xs_ref, e.cell_start, e.split, dims, e.box, rl2, rin2, e.cap, e.stage_cap, e.nlist,
+            e.ncount, e.flags, e.stats, e.big_tiles);
     }
+}
+
+void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s)
+{
+    k_list_stats<<<1, 256, 0, s>>>(e.ncount, (int)e.n, out);
 }
 
 void launch_gather(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
@@ -402,19 +514,23 @@ void launch_gather(Engine& e, const double* x, const double* y, const double* z,
 
 void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double* vx, double* vy,
                              double* vz, const void* fx, const void* fy, const void* fz,
-                             int32_t* img, const double* mass, double dt, hipStream_t s)
+                             int32_t* img, const double* mass, double dt, const SampleArgs& sa,
+                             hipStream_t s)
 {
     const double h = 0.5 * e.skin;
     ScopedTimer t(e, P_CHECK, s);
     int64_t cover = e.n > e.ncell ? e.n : e.ncell;
+    if (cover < (int64_t)sa.P * kBlock / 64) cover = (int64_t)sa.P * kBlock / 64;
+    if (cover < 1) cover = 1;
     if (e.precision == 32)
         k_kick_drift_check<float><<<nblk(cover), kBlock, 0, s>>>(
             x, y, z, vx, vy, vz, (const float*)fx, (const float*)fy, (const float*)fz, img, mass,
-            dt, (int)e.n, e.box, e.xr, e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell);
+            dt, (int)e.n, e.box, e.xr, e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell, sa);
     else
         k_kick_drift_check<double><<<nblk(cover), kBlock, 0, s>>>(
             x, y, z, vx, vy, vz, (const double*)fx, (const double*)fy, (const double*)fz, img,
-            mass, dt, (int)e.n, e.box, e.xr, e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell);
+            mass, dt, (int)e.n, e.box, e.xr, e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell,
+            sa);
 }
 
 void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,

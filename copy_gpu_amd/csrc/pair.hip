@@ -6,42 +6,34 @@
 //   per-atom full-list loop  rbmd/framework/run/worklet/RunWorklet.cpp:170-237
 //   virial convention        rbmd-gpu src/force/src/ljforce_op/rocm/ljforce_op.hip.cu:555-575
 //
-// Mapping: one wave64 per cell-sorted atom; the lanes walk the Verlet row 64
-// entries at a time (the row's LJ-range entries come first, so only the first
-// iterations evaluate LJ), then a wave reduction.  Pair displacements are formed
-// in FP64 (minimum image) and the force maths runs in T (float or double).
-// Cutoff decisions use the FP64 squared distance.
+// Mapping: one workgroup per cell tile (home cell + 27-cell stencil staged in
+// LDS at the home cell's periodic image), one wave64 per home atom; the lanes
+// walk the atom's uint16 Verlet row 64 entries at a time (LJ-range entries come
+// first, so only the first iterations evaluate LJ), then a wave reduction.
+// Pair displacements are FP64 differences of the staged positions (no per-pair
+// minimum image), the force maths runs in T (float or double), and cutoff
+// decisions use the FP64 squared distance.
 #include "state.h"
+#include "tile.h"
 
 namespace rbmd {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / 64;
 
-__device__ inline double min_image(double t, double L, double invL)
+// erfc(z) e^{z^2} for z >= 0 in FP32, Abramowitz & Stegun 7.1.26 (Hastings):
+//   erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) e^{-z^2} + eps,
+//   t = 1 / (1 + p z),  |eps| <= 1.5e-7 absolute.
+// The caller multiplies by e^{-z^2}, which the Gaussian term of G shares.
+__device__ inline float erfc_scaled_as(float z)
 {
-    return t - L * rint(t * invL);
-}
-
-// erfc(z) for z >= 0 in FP32: Chebyshev-fitted form of Numerical Recipes
-// (Press et al., 2nd ed., section 6.2, "erfcc"), fractional error < 1.2e-7:
-//   t = 1/(1 + z/2),  erfc(z) = t exp(-z^2 + p(t)).
-__device__ inline float erfc_nr(float z, float mz2)
-{
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
-    float p = 0.17087277f;
-    p = fmaf(p, t, -0.82215223f);
-    p = fmaf(p, t, 1.48851587f);
-    p = fmaf(p, t, -1.13520398f);
-    p = fmaf(p, t, 0.27886807f);
-    p = fmaf(p, t, -0.18628806f);
-    p = fmaf(p, t, 0.09678418f);
-    p = fmaf(p, t, 0.37409196f);
-    p = fmaf(p, t, 1.00002368f);
-    p = fmaf(p, t, -1.26551223f);
-    return t * __expf(mz2 + p);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+    float p = 1.061405429f;
+    p = fmaf(p, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    return p * t;
 }
 
 struct PairScalars {
@@ -58,15 +50,15 @@ struct PairMath;
 
 template <>
 struct PairMath<float> {
-    __device__ static inline float eval(double d2, bool lj_range, double qq_d, float eps24,
-                                        float sig2, const PairScalars& ps, float* e_lj,
-                                        float* e_re, bool ev)
+    template <bool LJ>
+    __device__ static inline float eval(double d2, double qq_d, float eps24, float sig2,
+                                        const PairScalars& ps, float* e_lj, float* e_re, bool ev)
     {
         const float r2 = (float)d2;
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         float fs = 0.0f;
-        if (lj_range && d2 < ps.rcl2) {
+        if (LJ && d2 < ps.rcl2) {
             const float sr2 = sig2 * r2inv;
             const float sr6 = sr2 * sr2 * sr2;
             fs = -eps24 * (2.0f * sr6 - 1.0f) * sr6 * r2inv;
@@ -75,11 +67,9 @@ struct PairMath<float> {
         if (d2 < ps.rcc2) {
             const float qq = (float)qq_d;
             const float r = r2 * rinv;
-            const float alpha = (float)ps.alpha;
-            const float z = (float)ps.sqrt_alpha * r;
-            const float mz2 = -alpha * r2;
-            const float erfcv = erfc_nr(z, mz2);
-            const float g = erfcv * r2inv + (float)ps.tsp * __expf(mz2) * rinv;
+            const float e2 = __expf(-(float)ps.alpha * r2);
+            const float erfcv = erfc_scaled_as((float)ps.sqrt_alpha * r) * e2;
+            const float g = erfcv * r2inv + (float)ps.tsp * e2 * rinv;
             fs -= qq * g * rinv;
             if (ev) *e_re += 0.5f * qq * erfcv * rinv;
         }
@@ -89,12 +79,13 @@ struct PairMath<float> {
 
 template <>
 struct PairMath<double> {
-    __device__ static inline double eval(double d2, bool lj_range, double qq, double eps24,
-                                         double sig2, const PairScalars& ps, double* e_lj,
-                                         double* e_re, bool ev)
+    template <bool LJ>
+    __device__ static inline double eval(double d2, double qq, double eps24, double sig2,
+                                         const PairScalars& ps, double* e_lj, double* e_re,
+                                         bool ev)
     {
         double fs = 0.0;
-        if (lj_range && d2 < ps.rcl2) {
+        if (LJ && d2 < ps.rcl2) {
             const double sr2 = sig2 / d2;
             const double sr6 = sr2 * sr2 * sr2;
             fs = -eps24 * (2.0 * sr6 - 1.0) * sr6 / d2;
@@ -111,28 +102,28 @@ struct PairMath<double> {
     }
 };
 
-template <typename T, bool EV>
-__device__ inline void accumulate_pair(const double4& pi, const double4& pj, int ti, int tj,
-                                       bool lj_range, const Box& box, const PairScalars& ps,
-                                       const PairParams* __restrict__ pp, T& fx, T& fy, T& fz,
-                                       double* acc)
+// One pair: fs from PairMath, force accumulated in T (fx += fs * dx), and with
+// EV the energies and the virial W_ab += -1/2 fs dx_a dx_b in FP64.  qq already
+// carries the Coulomb unit factor.
+template <typename T, bool EV, bool MULTI, bool LJ>
+__device__ inline void accumulate_pair(double dx, double dy, double dz, double qq, int ti, int tj,
+                                       const PairScalars& ps, const PairParams* __restrict__ pp,
+                                       T& fx, T& fy, T& fz, double* acc)
 {
-    const double dx = min_image(pj.x - pi.x, box.L[0], box.invL[0]);
-    const double dy = min_image(pj.y - pi.y, box.L[1], box.invL[1]);
-    const double dz = min_image(pj.z - pi.z, box.L[2], box.invL[2]);
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (!(d2 > 1e-4)) return;
-    T eps24, sig2;
-    if (ps.ntypes == 1) {
-        eps24 = (T)ps.eps24;
-        sig2 = (T)ps.sig2;
-    } else {
-        eps24 = (T)pp->eps24[ti * kMaxTypes + tj];
-        sig2 = (T)pp->sig2[ti * kMaxTypes + tj];
+    T eps24 = 0, sig2 = 0;
+    if (LJ) {
+        if (!MULTI) {
+            eps24 = (T)ps.eps24;
+            sig2 = (T)ps.sig2;
+        } else {
+            eps24 = (T)pp->eps24[ti * kMaxTypes + tj];
+            sig2 = (T)pp->sig2[ti * kMaxTypes + tj];
+        }
     }
     T e_lj = 0, e_re = 0;
-    const double qq = pi.w * pj.w * ps.qqr2e;
-    const T fs = PairMath<T>::eval(d2, lj_range, qq, eps24, sig2, ps, &e_lj, &e_re, EV);
+    const T fs = PairMath<T>::template eval<LJ>(d2, qq, eps24, sig2, ps, &e_lj, &e_re, EV);
     fx += fs * (T)dx;
     fy += fs * (T)dy;
     fz += fs * (T)dz;
@@ -149,82 +140,239 @@ __device__ inline void accumulate_pair(const double4& pi, const double4& pj, int
     }
 }
 
-template <typename T, bool EV>
-__global__ __launch_bounds__(kBlock) void k_pair(
-    const double4* __restrict__ xs, const int32_t* __restrict__ ts,
-    const int32_t* __restrict__ perm, const int32_t* __restrict__ nlist,
-    const int2* __restrict__ ncount, const int32_t* __restrict__ scell,
-    const int32_t* __restrict__ cell_start, int n, int cap, int3 dims, Box box, PairScalars ps,
-    const PairParams* __restrict__ pp, T* __restrict__ fx, T* __restrict__ fy,
-    T* __restrict__ fz, double* __restrict__ ev_part)
+// Position of tile-local candidate u, already moved to the image next to the
+// home cell (from LDS when the tile is staged, else from global memory).
+template <bool STAGED>
+__device__ inline double4 tile_pos(const TileHdr& h, const double4* __restrict__ lds,
+                                   const double4* __restrict__ xs, int u)
 {
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int s = uniform(blockIdx.x * kWavesPerBlock + wv);
-    T ax = 0, ay = 0, az = 0;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (s < n) {
-        const double4 pi = xs[s];
-        const int ti = ps.ntypes == 1 ? 0 : ts[s];
-        const int2 nc = ncount[s];
-        const int n_in = uniform(nc.x);
-        const int tot = uniform(nc.x + nc.y);
-        if (tot <= cap) {
-            const int32_t* __restrict__ row = nlist + (size_t)s * cap;
-            for (int base = 0; base < tot; base += 64) {
-                const int v = base + lane;
-                if (v < tot) {
-                    const int idx = v < n_in ? v : cap - 1 - (v - n_in);
-                    const int j = row[idx];
-                    const double4 pj = xs[j];
-                    const int tj = ps.ntypes == 1 ? 0 : ts[j];
-                    accumulate_pair<T, EV>(pi, pj, ti, tj, base < n_in, box, ps, pp, ax, ay, az,
-                                           acc);
-                }
+    if (STAGED) return lds[u];
+    const int st = tile_seg(h, u);
+    const double4 p = xs[h.gstart[st] + (u - h.off[st])];
+    return make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+}
+
+template <bool STAGED>
+__device__ inline int tile_type(const TileHdr& h, const int32_t* __restrict__ ltype,
+                                const int32_t* __restrict__ ts, int u)
+{
+    if (STAGED) return ltype[u];
+    const int st = tile_seg(h, u);
+    return ts[h.gstart[st] + (u - h.off[st])];
+}
+
+constexpr int kRound = 2;    // 16-entry chunks per atom per round
+constexpr int kBigGrid = 64; // workgroups of the big-tile pass
+
+struct PairArgs {
+    const double4* xs;
+    const int32_t* ts;
+    const int32_t* perm;
+    const uint16_t* nlist;
+    const int2* ncount;
+    int cap;
+};
+
+// Sum over the 16 lanes of each quarter-wave, delivered to all 16: quad xor 1,
+// quad xor 2, half-row mirror, row mirror (DPP lane moves, no LDS).
+template <typename T>
+__device__ inline T group16_sum(T v);
+
+template <>
+__device__ inline float group16_sum<float>(float v)
+{
+    v += __int_as_float(dpp_i<0xB1>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x4E>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x141>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x140>(__float_as_int(v)));
+    return v;
+}
+
+template <>
+__device__ inline double group16_sum<double>(double v)
+{
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return v;
+}
+
+// Maximum of an int over the wave (all lanes get it).
+__device__ inline int wave_max_int(int v)
+{
+    const int a = max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16));
+    const int b = max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
+    return max(a, b);
+}
+
+// One wave = four quarter-waves of 16 lanes, each walking one home atom's row
+// (atoms s = a0 + 4 wv + g + 32 t of the workgroup's range).  A round covers
+// 4 x 16 entries per atom: all index loads, then all LDS gathers, then the
+// maths; rounds that can still hold LJ-range entries of any of the four atoms
+// evaluate LJ, later rounds skip it.  Rows that overflowed their capacity scan
+// every tile candidate instead.
+template <typename T, bool EV, bool MULTI, bool STAGED>
+__device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
+                                 const double4* __restrict__ lds,
+                                 const int32_t* __restrict__ ltype, int a0, int a1, int h0, int wv,
+                                 int lane, const PairScalars& ps,
+                                 const PairParams* __restrict__ pp, T* __restrict__ fx,
+                                 T* __restrict__ fy, T* __restrict__ fz, double* acc)
+{
+    const int total = h.off[27];
+    const int cap = a.cap;
+    const int g = lane >> 4, gl = lane & 15;
+    for (int t0 = a0 + 4 * wv; t0 < a1; t0 += 4 * kTileWaves) {
+        const int s = t0 + g;
+        const bool live = s < a1;
+        const int2 nc = live ? a.ncount[s] : make_int2(0, 0);
+        const int n_in = nc.x, tot = nc.x + nc.y;
+        const bool over = tot > cap;
+        const int tl = over ? 0 : tot;
+        const int tmax = wave_max_int(tl);
+        const int nin_max = wave_max_int(over ? 0 : n_in);
+        const int ui = h.off[kHomeSeg] + (live ? s - h0 : 0);
+        const double4 pi = STAGED ? lds[ui] : a.xs[live ? s : h0];
+        const int ti = MULTI ? (STAGED ? ltype[ui] : a.ts[live ? s : h0]) : 0;
+        const double qi = pi.w * ps.qqr2e;
+        const uint16_t* __restrict__ row = a.nlist + (size_t)(live ? s : h0) * cap;
+        T ax = 0, ay = 0, az = 0;
+        for (int base = 0; base < tmax; base += 16 * kRound) {
+            int u[kRound];
+#pragma unroll
+            for (int k = 0; k < kRound; ++k) {
+                const int v = base + 16 * k + gl;
+                const int idx = v < n_in ? v : cap - 1 - (v - n_in);
+                u[k] = v < tl ? (int)row[idx] : -1;
             }
-        } else {
-            // Row overflowed its capacity: scan the 27-cell stencil directly.
-            const int c = uniform(scell[s]);
-            const int cz = c / (dims.x * dims.y);
-            const int cy = (c / dims.x) % dims.y;
-            const int cx = c % dims.x;
-            for (int st = 0; st < 27; ++st) {
-                const int bx = (cx + st / 9 - 1 + dims.x) % dims.x;
-                const int by = (cy + (st / 3) % 3 - 1 + dims.y) % dims.y;
-                const int bz = (cz + st % 3 - 1 + dims.z) % dims.z;
-                const int cc = bx + by * dims.x + bz * dims.x * dims.y;
-                const int j0 = cell_start[cc], j1 = cell_start[cc + 1];
-                for (int j = j0 + lane; j < j1; j += 64) {
-                    const double4 pj = xs[j];
-                    const int tj = ps.ntypes == 1 ? 0 : ts[j];
-                    accumulate_pair<T, EV>(pi, pj, ti, tj, true, box, ps, pp, ax, ay, az, acc);
+            double4 pj[kRound];
+            int tj[kRound] = {};
+#pragma unroll
+            for (int k = 0; k < kRound; ++k) {
+                const int uu = u[k] < 0 ? 0 : u[k];
+                pj[k] = tile_pos<STAGED>(h, lds, a.xs, uu);
+                if (MULTI) tj[k] = tile_type<STAGED>(h, ltype, a.ts, uu);
+            }
+#pragma unroll
+            for (int k = 0; k < kRound; ++k) {
+                if (base + 16 * k < nin_max) {
+                    if (u[k] >= 0)
+                        accumulate_pair<T, EV, MULTI, true>(pj[k].x - pi.x, pj[k].y - pi.y,
+                                                            pj[k].z - pi.z, qi * pj[k].w, ti,
+                                                            tj[k], ps, pp, ax, ay, az, acc);
+                } else if (base + 16 * k < tmax) {
+                    if (u[k] >= 0)
+                        accumulate_pair<T, EV, MULTI, false>(pj[k].x - pi.x, pj[k].y - pi.y,
+                                                             pj[k].z - pi.z, qi * pj[k].w, ti,
+                                                             tj[k], ps, pp, ax, ay, az, acc);
                 }
             }
         }
-        ax = wave_sum(ax);
-        ay = wave_sum(ay);
-        az = wave_sum(az);
-        if (lane == 0) {
-            const int i = perm[s];
+        if (__ballot(live && over) != 0ull) {
+            for (int uu = gl; uu < total; uu += 16) {
+                if (live && over) {
+                    const double4 pj = tile_pos<STAGED>(h, lds, a.xs, uu);
+                    const int tj = MULTI ? tile_type<STAGED>(h, ltype, a.ts, uu) : 0;
+                    accumulate_pair<T, EV, MULTI, true>(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
+                                                        qi * pj.w, ti, tj, ps, pp, ax, ay, az,
+                                                        acc);
+                }
+            }
+        }
+        ax = group16_sum(ax);
+        ay = group16_sum(ay);
+        az = group16_sum(az);
+        if (live && gl == 0) {
+            const int i = a.perm[s];
             fx[i] = ax;
             fy[i] = ay;
             fz[i] = az;
         }
     }
-    if (EV) {
-        __shared__ double red[kWavesPerBlock][8];
+}
+
+// One workgroup per (home cell, part).  The tile's current positions are staged
+// in LDS shifted to the home cell's image (and types when there are several);
+// each 16-lane quarter-wave walks one home atom's uint16 row -- LJ-range
+// entries first, so later rounds skip the LJ code -- then reduces across its 16
+// lanes.  Rows that overflowed their capacity scan every tile candidate instead.
+// Per-workgroup FP64 energy / virial partial: wave sums, then the 8 waves in order.
+__device__ inline void block_ev(const double* acc, int lane, int wv, double* __restrict__ out)
+{
+    __shared__ double red[kTileWaves][8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const double v = wave_sum(acc[k]);
-            if (lane == 0) red[wv][k] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x < 8) {
-            double v = 0.0;
-            for (int w = 0; w < kWavesPerBlock; ++w) v += red[w][threadIdx.x];
-            ev_part[(size_t)blockIdx.x * 8 + threadIdx.x] = v;
+    for (int k = 0; k < 8; ++k) {
+        const double v = wave_sum_dpp(acc[k]);
+        if (lane == 0) red[wv][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double v = 0.0;
+        for (int w = 0; w < kTileWaves; ++w) v += red[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+}
+
+// Staged pass: one workgroup per (tile, part); tiles larger than the LDS stage
+// are skipped here and served by k_pair_big.
+template <typename T, bool EV, bool MULTI>
+__global__ __launch_bounds__(kTileThreads) void k_pair_tile(
+    PairArgs a, const int32_t* __restrict__ cell_start, int split, int3 dims, Box box,
+    int stage_cap, PairScalars ps, const PairParams* __restrict__ pp, T* __restrict__ fx,
+    T* __restrict__ fy, T* __restrict__ fz, double* __restrict__ ev_part)
+{
+    extern __shared__ double4 lds[];
+    int32_t* ltype = (int32_t*)(lds + stage_cap);
+    __shared__ TileHdr h;
+    const int c = blockIdx.x / split, part = blockIdx.x % split;
+    tile_setup(c, dims, box, cell_start, h);
+    const int total = h.off[27];
+    const bool mine = total <= stage_cap;
+    if (mine) {
+        for (int u = threadIdx.x; u < total; u += kTileThreads) {
+            const int st = tile_seg(h, u);
+            const int j = h.gstart[st] + (u - h.off[st]);
+            const double4 p = a.xs[j];
+            lds[u] = make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+            if (MULTI) ltype[u] = a.ts[j];
         }
     }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wv = uniform(threadIdx.x >> 6);
+    const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+    const int a0 = h0 + (hc * part) / split, a1 = h0 + (hc * (part + 1)) / split;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (mine)
+        pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, ps, pp, fx, fy, fz,
+                                      acc);
+    if (EV) block_ev(acc, lane, wv, ev_part + (size_t)blockIdx.x * 8);
+}
+
+// Global-memory pass over the tiles the list build found larger than the LDS
+// stage (big_tiles[0 .. stats[6])), a fixed small grid looping over them.
+template <typename T, bool EV, bool MULTI>
+__global__ __launch_bounds__(kTileThreads) void k_pair_big(
+    PairArgs a, const int32_t* __restrict__ cell_start, const int32_t* __restrict__ big_tiles,
+    const unsigned long long* __restrict__ stats, int split, int3 dims, Box box, PairScalars ps,
+    const PairParams* __restrict__ pp, T* __restrict__ fx, T* __restrict__ fy,
+    T* __restrict__ fz, double* __restrict__ ev_part)
+{
+    __shared__ TileHdr h;
+    const int nbig = (int)stats[6];
+    const int lane = threadIdx.x & 63;
+    const int wv = uniform(threadIdx.x >> 6);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int c = big_tiles[b];
+        __syncthreads();
+        tile_setup(c, dims, box, cell_start, h);
+        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+        pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, h0, h0 + hc, h0, wv, lane, ps, pp,
+                                       fx, fy, fz, acc);
+    }
+    if (EV) block_ev(acc, lane, wv, ev_part + (size_t)blockIdx.x * 8);
 }
 
 // Deterministic final reduction: pair partials [nb][8] + per-k terms [P][7]
@@ -281,33 +429,40 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
     ps.sig2 = e.pp.sig2[0];
     ps.ntypes = e.pp.ntypes;
     const int3 dims = make_int3(e.dims[0], e.dims[1], e.dims[2]);
-    const int blocks = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int blocks = e.ncell * e.split;
+    const size_t lds = (size_t)e.stage_cap * (sizeof(double4) + (ps.ntypes > 1 ? 4 : 0));
     ScopedTimer t(e, P_PAIR, s);
-#define RBMD_PAIR_ARGS                                                                        \
-    e.xs, e.ts, e.perm, e.nlist, e.ncount, e.scell, e.cell_start, n, e.cap, dims, e.box, ps,  \
-        e.d_pp
+    PairArgs a{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap};
+    const bool multi = ps.ntypes > 1;
+#define RBMD_PAIR(T, EV, M)                                                                   \
+    do {                                                                                      \
+        k_pair_tile<T, EV, M><<<blocks, kTileThreads, lds, s>>>(                              \
+            a, e.cell_start, e.split, dims, e.box, e.stage_cap, ps, e.d_pp, (T*)fx, (T*)fy,   \
+            (T*)fz, e.ev_part);                                                               \
+        k_pair_big<T, EV, M><<<kBigGrid, kTileThreads, 0, s>>>(                               \
+            a, e.cell_start, e.big_tiles, e.stats, e.split, dims, e.box, ps, e.d_pp, (T*)fx,  \
+            (T*)fy, (T*)fz, e.ev_part + (size_t)blocks * 8);                                  \
+    } while (0)
+#define RBMD_PAIR_M(T, EV)                                                                    \
+    do {                                                                                      \
+        if (multi) RBMD_PAIR(T, EV, true);                                                    \
+        else RBMD_PAIR(T, EV, false);                                                         \
+    } while (0)
     if (e.precision == 32) {
-        if (ev)
-            k_pair<float, true><<<blocks, kBlock, 0, s>>>(RBMD_PAIR_ARGS, (float*)fx, (float*)fy,
-                                                           (float*)fz, e.ev_part);
-        else
-            k_pair<float, false><<<blocks, kBlock, 0, s>>>(RBMD_PAIR_ARGS, (float*)fx,
-                                                            (float*)fy, (float*)fz, e.ev_part);
+        if (ev) RBMD_PAIR_M(float, true);
+        else RBMD_PAIR_M(float, false);
     } else {
-        if (ev)
-            k_pair<double, true><<<blocks, kBlock, 0, s>>>(RBMD_PAIR_ARGS, (double*)fx,
-                                                            (double*)fy, (double*)fz, e.ev_part);
-        else
-            k_pair<double, false><<<blocks, kBlock, 0, s>>>(RBMD_PAIR_ARGS, (double*)fx,
-                                                             (double*)fy, (double*)fz, e.ev_part);
+        if (ev) RBMD_PAIR_M(double, true);
+        else RBMD_PAIR_M(double, false);
     }
-#undef RBMD_PAIR_ARGS
+#undef RBMD_PAIR_M
+#undef RBMD_PAIR
 }
 
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s)
 {
     ScopedTimer t(e, P_EV_FINAL, s);
-    const int nb = with_pair ? (int)((e.n + kWavesPerBlock - 1) / kWavesPerBlock) : 0;
+    const int nb = with_pair ? e.ncell * e.split + kBigGrid : 0;  // staged + big-tile kernels
     const int P = (e.kspace != 0) ? e.P : 0;
     k_ev_final<<<1, 256, 0, s>>>(e.ev_part, nb, e.kev, P, e.e_self, e.ev);
 }

@@ -21,6 +21,19 @@ struct KCoef {
     T A, B;
 };
 
+// k-sampler arguments of the fused kick/drift kernel (P = 0: no sampling).
+struct SampleArgs {
+    int P = 0;
+    uint64_t seed = 0;
+    uint64_t step = 0;
+    const unsigned long long* step_dev = nullptr;  // overrides step when set
+    const double* cdf = nullptr;
+    int3 M{};
+    double w = 0;
+    int32_t* km = nullptr;
+    double* kw = nullptr;
+};
+
 struct Engine {
     // ---- parameters
     Box box{};
@@ -32,7 +45,13 @@ struct Engine {
     int dims[3] = {0, 0, 0};
     double cell[3] = {0, 0, 0};
     int ncell = 0;
-    int cap = 0;  // list row capacity, multiple of 64
+    int cap = 0;        // list row capacity, multiple of 64
+    int split = 1;      // workgroups per cell tile
+    int stage_cap = 0;  // LDS tile capacity (candidates)
+    int stage_override = 0;
+    int lds_max = 0;    // device LDS bytes per workgroup
+    int lds_cu = 0;     // device LDS bytes per CU
+    int num_cu = 0;
     double e_self = 0;
     bool atoms_set = false;
 
@@ -42,10 +61,11 @@ struct Engine {
     int32_t *perm = nullptr, *scell = nullptr, *ts = nullptr;
     int32_t *cell_count = nullptr, *cell_start = nullptr, *atom_cell = nullptr,
             *atom_rank = nullptr;
-    int32_t* nlist = nullptr;
+    uint16_t* nlist = nullptr;  // tile-local indices
+    int32_t* big_tiles = nullptr;  // tiles larger than the LDS stage (count in stats[6])
     int2* ncount = nullptr;
     uint32_t* flags = nullptr;              // [0] rebuild request, [1] NaN seen
-    unsigned long long* stats = nullptr;    // [0] builds [1] entries [2] overflow rows [3] inner
+    unsigned long long* stats = nullptr;    // [0] builds [1] entries [2] overflow rows [3] inner [4] longest overflowing row [6] big tiles [8..10] list stats
     double* q = nullptr;
     int32_t* type = nullptr;
     PairParams* d_pp = nullptr;
@@ -76,6 +96,7 @@ struct Engine {
     unsigned long long* h_stats = nullptr;
 
     hipStream_t own_stream = nullptr;
+    hipEvent_t order_event = nullptr;  // orders own_stream after the caller's null stream
 
     // timing
     bool timing = false;
@@ -103,7 +124,9 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
 void launch_gather(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
 void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double* vx, double* vy,
                              double* vz, const void* fx, const void* fy, const void* fz,
-                             int32_t* img, const double* mass, double dt, hipStream_t s);
+                             int32_t* img, const double* mass, double dt, const SampleArgs& sa,
+                             hipStream_t s);
+void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s);
 void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,
                  const void* fz, const double* mass, double dt, hipStream_t s);
 void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s);
@@ -117,7 +140,8 @@ void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, h
 void launch_kweights_default(Engine& e, hipStream_t s);
 void launch_rho(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
 void launch_kforce(Engine& e, const double* x, const double* y, const double* z, void* fx,
-                   void* fy, void* fz, hipStream_t s);
+                   void* fy, void* fz, double* vx, double* vy, double* vz, const double* mass,
+                   double dt, hipStream_t s);
 
 // timing helpers (api.hip)
 void timer_begin(Engine& e, int pass, hipStream_t s, hipEvent_t* ev);

@@ -62,6 +62,7 @@ class ForceConfig:
     ewald_tol: float = 1e-16
     n_max: int = 0
     list_capacity: int = 0        # Verlet row capacity; 0 = automatic
+    tile_stage: int = 0           # LDS stage of a cell tile (atoms); 0 = automatic
     extra: dict = field(default_factory=dict)
 
     @property
@@ -95,6 +96,7 @@ class LJCutCoulKspaceForce:
         p.p_max = cfg.coulomb_sample_num if cfg.kspace.upper() == "RBE" else 0
         p.ewald_tol = cfg.ewald_tol
         p.list_capacity = cfg.list_capacity
+        p.tile_stage = cfg.tile_stage
         h = C.c_void_p()
         rc = self.lib.rbmd_force_create(C.byref(p), int(cfg.n_max), C.byref(h))
         if rc != 0:
@@ -191,11 +193,11 @@ class LJCutCoulKspaceForce:
         return offs, idx[:cnt.value]
 
     def stats(self) -> dict:
-        out = np.zeros(8, np.int64)
+        out = np.zeros(12, np.int64)
         self._check(self.lib.rbmd_force_get_stats(self.h, out.ctypes.data_as(C.c_void_p)),
                     "get_stats")
         keys = ["builds", "entries", "cap", "overflow_rows", "cells", "inner_entries",
-                "ewald_k", "n"]
+                "ewald_k", "n", "big_tiles", "tile_stage", "largest_tile", "split"]
         return dict(zip(keys, (int(v) for v in out)))
 
     def set_timing(self, enable: bool):
@@ -212,7 +214,7 @@ class LJCutCoulKspaceForce:
     def md_run(self, x, y, z, vx, vy, vz, fx, fy, fz, dt, nsteps, seed=0, step0=0, img=None,
                mass=None, use_graph=True, stream=None):
         """nsteps of kick-drift-force-kick (ExecutionNVT::Solve order) updating all arrays."""
-        st = None if (use_graph and stream is None) else _stream(stream)
+        st = _stream(stream)
         self._check(self.lib.rbmd_md_run(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(vx), _ptr(vy),
                                          _ptr(vz), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(img),
                                          _ptr(mass), dt, nsteps, seed, step0,

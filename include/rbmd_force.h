@@ -58,6 +58,8 @@ typedef struct {
     int32_t p_max;      /* RBE: largest P used with this handle               */
     double ewald_tol;   /* EWALD: keep k with exp(-k^2/4alpha) >= tol (e.g. 1e-16) */
     int32_t list_capacity; /* Verlet row capacity (multiple of 64); 0 = automatic */
+    int32_t tile_stage;    /* LDS stage of a cell tile, in atoms (>= 64); 0 = automatic.
+                              Tiles with more candidates read global memory instead. */
 } rbmd_force_params;
 
 /* FP64 energies (LJ units x qqr2e for Coulomb terms) and virial
@@ -142,10 +144,12 @@ int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int
 int rbmd_force_get_kspace(rbmd_force_t h, int32_t* P, int32_t* m, double* w, double* re,
                           double* im);
 
-/* Diagnostic counters (HOST out[8], synchronising): [0] list builds so far,
+/* Diagnostic counters (HOST out[12], synchronising): [0] list builds so far,
  * [1] list entries (full, at the list radius), [2] row capacity, [3] rows
  * that overflowed at the last build, [4] cells, [5] inner (LJ-range) entries,
- * [6] Ewald k-vectors (EWALD mode), [7] n. */
+ * [6] Ewald k-vectors (EWALD mode), [7] n, [8] cell tiles larger than the LDS
+ * stage at the last build, [9] LDS stage (atoms), [10] largest tile (atoms),
+ * [11] workgroups per tile. */
 int rbmd_force_get_stats(rbmd_force_t h, int64_t* out);
 
 /* Per-kernel device time (HIP events on the launch stream).  enable != 0

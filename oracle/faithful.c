@@ -69,7 +69,11 @@ int orc_faithful_run(int64_t n, float* pos, float* vel, float* force, const floa
     const float alpha = (float)alpha_d, dt = (float)dt_d;
     const float V = L[0] * L[1] * L[2];
     const float rho_sys = (float)((double)n / ((double)L[0] * L[1] * L[2]));
-    const int64_t max_j = (int64_t)(rho_sys * ceilf(4.0f / 3.0f * (float)ORC_PI * rc * rc * rc)) + 1;
+    /* The reference sizes the fixed-stride rows as rho*ceil(4/3 pi rc^3) + 1
+     * (ExecutionMD.cpp:676-691) without an overflow check; at these densities
+     * that is below the longest rows, so the baseline doubles it to do the
+     * full work (overflow is still reported). */
+    const int64_t max_j = 2 * ((int64_t)(rho_sys * ceilf(4.0f / 3.0f * (float)ORC_PI * rc * rc * rc)) + 1);
     int dims[3];
     float dxyz[3];
     for (int d = 0; d < 3; ++d) {

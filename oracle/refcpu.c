@@ -21,10 +21,11 @@
  *     std::random_device, math/Utiles.h:32-39);
  *   - FP64 energy and a 6-component virial.
  *
- * Pair-distance convention shared bit-for-bit with the GPU list builder:
- *   t  = x_j - x_i               (FP64)
- *   k  = rint(t / L)             (image index, in {-1,0,1} for wrapped input)
- *   dx = t - k*L                 (k*L is exact)
+ * Pair-distance convention shared bit-for-bit with the GPU list builder, in
+ * the reference's own order (RunWorklet.cpp:77-79: p_j = position(j) -
+ * coord_offset, r_ij = p_j - p_i):
+ *   k  = rint((x_j - x_i) / L)   (image index, in {-1,0,1} for wrapped input)
+ *   dx = (x_j - k*L) - x_i       (k*L is exact)
  *   d2 = fma(dz,dz, fma(dy,dy, dx*dx))
  * and the neighbour criterion of RunWorklet.cpp:83
  *   (r_list^2 - d2 > 1e-4) && (d2 > 1e-4).
@@ -44,14 +45,15 @@
 /* periodic helpers                                                     */
 /* ------------------------------------------------------------------ */
 
-/* Minimum image of one displacement component.
- * Follows ExecPointLocator::MinDistanceVec (locator/ExecPointLocator.h:225-260):
- * a single shift by the box length when |t| > L/2; written with rint so that
- * it also covers |t| > L and reproduces the GPU's (t - s) bits exactly. */
-static inline double orc_mi(double t, double L)
+/* Minimum-image displacement x_j - x_i along one axis.  Follows
+ * ExecPointLocator::MinDistanceVec (locator/ExecPointLocator.h:225-260): a
+ * single shift by the box length when |x_j - x_i| > L/2, written with rint so
+ * that it also covers |x_j - x_i| > L; the shift is applied to x_j before the
+ * difference, as the reference's list build does (RunWorklet.cpp:77-79). */
+static inline double orc_mi(double xj, double xi, double L)
 {
-    double k = nearbyint(t / L);
-    return t - k * L;
+    double k = nearbyint((xj - xi) / L);
+    return (xj - k * L) - xi;
 }
 
 static inline double orc_d2(double dx, double dy, double dz)
@@ -99,9 +101,9 @@ int64_t orc_neighbors_brute(int64_t n, const double* x, const double* y, const d
     offsets[0] = 0;
     for (int64_t i = 0; i < n; ++i) {
         for (int64_t j = 0; j < n; ++j) {
-            double dx = orc_mi(x[j] - x[i], L[0]);
-            double dy = orc_mi(y[j] - y[i], L[1]);
-            double dz = orc_mi(z[j] - z[i], L[2]);
+            double dx = orc_mi(x[j], x[i], L[0]);
+            double dy = orc_mi(y[j], y[i], L[1]);
+            double dz = orc_mi(z[j], z[i], L[2]);
             double d2 = orc_d2(dx, dy, dz);
             if ((rl2 - d2) > 1e-4 && d2 > 1e-4) {
                 if (e < cap) idx[e] = (int32_t)j;
@@ -208,9 +210,9 @@ int64_t orc_neighbors_cells(int64_t n, const double* x, const double* y, const d
                     int64_t cc = cx + cy * (int64_t)c.dims[0] + cz * (int64_t)c.dims[0] * c.dims[1];
                     for (int64_t s = c.start[cc]; s < c.start[cc + 1]; ++s) {
                         int32_t j = c.ids[s];
-                        double dx = orc_mi(x[j] - x[i], L[0]);
-                        double dy = orc_mi(y[j] - y[i], L[1]);
-                        double dz = orc_mi(z[j] - z[i], L[2]);
+                        double dx = orc_mi(x[j], x[i], L[0]);
+                        double dy = orc_mi(y[j], y[i], L[1]);
+                        double dz = orc_mi(z[j], z[i], L[2]);
                         double d2 = orc_d2(dx, dy, dz);
                         if ((rl2 - d2) > 1e-4 && d2 > 1e-4) {
                             if (e < cap) idx[e] = j;
@@ -288,9 +290,9 @@ void orc_pair_forces(int64_t n, const double* x, const double* y, const double* 
             }
             for (int64_t s = s0; s < s1; ++s) {
                 int64_t j = use_cells ? c.ids[s] : s;
-                double dx = orc_mi(x[j] - x[i], L[0]);
-                double dy = orc_mi(y[j] - y[i], L[1]);
-                double dz = orc_mi(z[j] - z[i], L[2]);
+                double dx = orc_mi(x[j], x[i], L[0]);
+                double dy = orc_mi(y[j], y[i], L[1]);
+                double dz = orc_mi(z[j], z[i], L[2]);
                 double d2 = orc_d2(dx, dy, dz);
                 if (!(d2 > 1e-4)) continue;
                 double fs = 0.0;

@@ -29,13 +29,14 @@ def _dev(a, dtype=torch.float64):
 
 def _make(Lb, n, precision=32, kspace="NONE", P=100, rc_lj=2.5, rc_coul=5.0, alpha=0.5,
           skin=0.3, lo=(0.0, 0.0, 0.0), eps=(1.0,), sigma=(1.0,), list_capacity=0,
-          ewald_tol=1e-16):
+          ewald_tol=1e-16, tile_stage=0):
     from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
 
     cfg = ForceConfig(box_len=tuple(float(v) for v in Lb), box_lo=lo, cut_off=rc_lj,
                       coulomb_cut=rc_coul, skin=skin, alpha=alpha, kspace=kspace,
                       coulomb_sample_num=P, precision=precision, eps=eps, sigma=sigma,
-                      n_max=n, list_capacity=list_capacity, ewald_tol=ewald_tol)
+                      n_max=n, list_capacity=list_capacity, ewald_tol=ewald_tol,
+                      tile_stage=tile_stage)
     return LJCutCoulKspaceForce(cfg)
 
 
@@ -184,6 +185,26 @@ def test_overflowed_rows_use_stencil_fallback(orc, gpu):
     st = f.stats()
     assert st["overflow_rows"] == 0 and st["cap"] > 64
     assert _rel(F2, Fo) < 1e-12
+
+
+@pytest.mark.parametrize("stage", [64, 1216])
+def test_tiles_beyond_lds_stage_read_global_memory(orc, gpu, stage):
+    """Tiles with more candidates than the LDS stage take the global-memory
+    paths of the list build and the pair pass: stage 64 sends every tile there,
+    stage 1216 (about the mean tile of this system) splits the tiles between the
+    two pair kernels.  Sets and forces must not change."""
+    _log(f"case: LDS stage {stage} atoms, FP64, 6000 ions")
+    x, y, z, q, L = _ions(6000, seed=13)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, precision=64, tile_stage=stage)
+    f.Init(_dev(q))
+    F = _compute(f, x, y, z, ev=True)
+    _check_sets(orc, f, x, y, z, np.zeros(3), Lb, 5.3)
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fo, evo = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, 0.5)
+    assert _rel(F, Fo) < 1e-12
+    ev = f.energy_virial()
+    assert abs(ev["e_real"] - evo[1]) < 1e-11 * abs(evo[1])
 
 
 # ----------------------------------------------------------------------------- k-space
