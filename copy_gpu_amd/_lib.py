@@ -81,6 +81,22 @@ _SIGS = {
     "rbmd_force_get_timing": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]),
     "rbmd_util_copy": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "rbmd_proposal_create": (C.c_int, [C.c_int32, _P, C.c_double, C.c_double, _P,
+                                       C.POINTER(C.c_void_p)]),
+    "rbmd_proposal_destroy": (None, [_P]),
+    "rbmd_proposal_sample_host": (C.c_int, [_P, C.c_int32, C.c_uint64, C.c_uint64, _P, _P]),
+    "rbmd_force_set_proposal": (C.c_int, [_P, _P]),
+    "rbmd_mlp_nparams": (C.c_int, []),
+    "rbmd_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "rbmd_mlp_destroy": (None, [_P]),
+    "rbmd_mlp_last_error": (C.c_char_p, [_P]),
+    "rbmd_mlp_get_params": (C.c_int, [_P, _P]),
+    "rbmd_mlp_set_params": (C.c_int, [_P, _P]),
+    "rbmd_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "rbmd_mlp_loss_grad": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.POINTER(C.c_double), _P,
+                                     _P]),
+    "rbmd_mlp_train": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int32, C.c_double,
+                                 C.POINTER(C.c_double), _P]),
     "rbmd_philox4x32": (None, [_P, _P, _P]),
     "rbmd_sys_fcc": (C.c_int, [C.c_int32, C.c_double, _P, _P, _P, C.POINTER(C.c_double)]),
     "rbmd_sys_random": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_uint64, _P, _P, _P]),

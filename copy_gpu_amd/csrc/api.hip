@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/rbmd_force.h"
+#include "proposal.h"
 #include "state.h"
 
 namespace rbmd {
@@ -105,7 +106,7 @@ static void free_all(Engine& E)
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
-                    E.ev_part,  E.ev,    E.big_tiles};
+                    E.ev_part,  E.ev,    E.big_tiles, E.gp_cdf, E.gp_ratio, E.gp_guide};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (E.h_stats) hipHostFree(E.h_stats);
@@ -431,6 +432,49 @@ int rbmd_force_set_kvectors(rbmd_force_t h, int32_t P, const int32_t* m, const d
     return RBMD_OK;
 }
 
+int rbmd_force_set_proposal(rbmd_force_t h, rbmd_proposal_t p)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (E.kspace != RBMD_KSPACE_RBE) return fail(E, RBMD_ERR_STATE, "handle has no RBE k-space");
+    E.gp_gen += 1;
+    if (!p) {
+        E.gp = GridProposal{};
+        return RBMD_OK;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (p->L[d] != E.box.L[d])
+            return fail(E, RBMD_ERR_INVALID, "proposal box differs from the handle's box");
+    if (p->alpha != E.alpha) return fail(E, RBMD_ERR_INVALID, "proposal alpha differs");
+    const int64_t G = (int64_t)p->cdf.size();
+    const int64_t K = (int64_t)p->guide.size();
+    HIP_TRY(hipDeviceSynchronize());
+    if (G > E.gp_cap || K > E.gp_guide_cap) {
+        if (E.gp_cdf) hipFree(E.gp_cdf);
+        if (E.gp_ratio) hipFree(E.gp_ratio);
+        if (E.gp_guide) hipFree(E.gp_guide);
+        E.gp_cdf = E.gp_ratio = nullptr;
+        E.gp_guide = nullptr;
+        E.gp_cap = E.gp_guide_cap = 0;
+        E.gp = GridProposal{};
+        HIP_TRY(hipMalloc((void**)&E.gp_cdf, (size_t)G * 8));
+        HIP_TRY(hipMalloc((void**)&E.gp_ratio, (size_t)G * 8));
+        HIP_TRY(hipMalloc((void**)&E.gp_guide, (size_t)K * 4));
+        E.gp_cap = G;
+        E.gp_guide_cap = K;
+    }
+    HIP_TRY(hipMemcpy(E.gp_cdf, p->cdf.data(), (size_t)G * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E.gp_ratio, p->ratio.data(), (size_t)G * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E.gp_guide, p->guide.data(), (size_t)K * 4, hipMemcpyHostToDevice));
+    E.gp.cdf = E.gp_cdf;
+    E.gp.ratio = E.gp_ratio;
+    E.gp.guide = E.gp_guide;
+    E.gp.guide_bits = p->guide_bits;
+    E.gp.Mg = p->Mg;
+    E.gp.lambda = p->lambda;
+    return RBMD_OK;
+}
+
 int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_t step,
                                void* stream)
 {
@@ -524,17 +568,7 @@ static void md_step(Engine& E, double* x, double* y, double* z, double* vx, doub
                     double dt, uint64_t seed, const uint64_t* step_host, hipStream_t s)
 {
     SampleArgs sa;
-    if (E.kspace == RBMD_KSPACE_RBE) {
-        sa.P = E.P;
-        sa.seed = seed;
-        sa.step = step_host ? *step_host : 0ull;
-        sa.step_dev = step_host ? nullptr : E.d_step;
-        sa.cdf = E.kcdf;
-        sa.M = make_int3(E.kM[0], E.kM[1], E.kM[2]);
-        sa.w = E.S / (double)E.P;
-        sa.km = E.km;
-        sa.kw = E.kw;
-    }
+    if (E.kspace == RBMD_KSPACE_RBE) sa = sample_args(E, E.P, seed, step_host);
     launch_kick_drift_check(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa, s);
     const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
     if (ks) {
@@ -583,6 +617,8 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         memcpy(&dtbits, &dt, 8);
         key.push_back((const void*)(uintptr_t)dtbits);
         key.push_back((const void*)(uintptr_t)seed);
+        // the proposal's table pointers, lambda and P-dependent weights are baked too
+        key.push_back((const void*)(uintptr_t)E.gp_gen);
         if (!E.graph_exec || key != E.graph_key) {
             if (E.graph_exec) hipGraphExecDestroy(E.graph_exec);
             if (E.graph) hipGraphDestroy(E.graph);

@@ -118,6 +118,63 @@ RBMD_HD inline void sample_kvector(uint64_t seed, uint64_t step, uint32_t l, con
     }
 }
 
+// Tabulated (learned) k-space proposal mixed with the analytic sampler:
+// q_mix(m) = lambda * p_grid(m) [|m_d| <= Mg] + (1 - lambda) * a(m) / S,
+// a(m) = exp(-k^2 / 4 alpha).  cdf is the inclusive running sum of p_grid over
+// the (2Mg+1)^3 box in x-fastest order (closing at exactly 1), ratio[i] =
+// a(m_i) / q_mix(m_i), and w_out = S / ((1 - lambda) P) is the weight of a draw
+// outside the box.  cdf == nullptr selects the analytic sampler alone.
+// guide[j] (j < 2^guide_bits) is the first index i with cdf[i] > j / 2^bits, so
+// the inverse-CDF lookup starts there and walks ~1 entry (guide-table method,
+// Chen & Asau 1974); with a power-of-two table u * 2^bits is exact.
+struct GridProposal {
+    const double* cdf = nullptr;
+    const double* ratio = nullptr;
+    const int32_t* guide = nullptr;
+    int guide_bits = 0;
+    int Mg = 0;
+    double lambda = 0;
+};
+
+// First index i with u < cdf[i] (u in [0, 1), cdf closing at 1).
+RBMD_HD inline int guided_lookup(const GridProposal& g, double u)
+{
+    int i = g.guide[(int)(u * (double)(1 << g.guide_bits))];
+    while (u >= g.cdf[i]) ++i;
+    return i;
+}
+
+// Draw of sample l (of P) and its importance weight w = a(m) / (P q(m)).
+// Mixture choice and grid index come from counter (l, 0x80000000, step_lo,
+// step_hi ^ 0x4C524E44); analytic draws use sample_kvector's counters.
+RBMD_HD inline double draw_kvector(uint64_t seed, uint64_t step, uint32_t l, const double* cdf,
+                                   const int* M, const GridProposal& g, int P, double w_analytic,
+                                   double w_out, int m_out[3])
+{
+    if (!g.cdf) {
+        sample_kvector(seed, step, l, cdf, M, m_out);
+        return w_analytic;
+    }
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t c[4] = {l, 0x80000000u, (uint32_t)step, (uint32_t)(step >> 32) ^ 0x4C524E44u};
+    uint32_t r[4];
+    philox4x32_10(c, key, r);
+    const int side = 2 * g.Mg + 1;
+    if (u53(r[0], r[1]) < g.lambda) {
+        const int idx = guided_lookup(g, u53(r[2], r[3]));
+        m_out[0] = idx % side - g.Mg;
+        m_out[1] = (idx / side) % side - g.Mg;
+        m_out[2] = idx / (side * side) - g.Mg;
+        return g.ratio[idx] / (double)P;
+    }
+    sample_kvector(seed, step, l, cdf, M, m_out);
+    if (m_out[0] < -g.Mg || m_out[0] > g.Mg || m_out[1] < -g.Mg || m_out[1] > g.Mg ||
+        m_out[2] < -g.Mg || m_out[2] > g.Mg)
+        return w_out;
+    const int idx = (m_out[0] + g.Mg) + side * ((m_out[1] + g.Mg) + side * (m_out[2] + g.Mg));
+    return g.ratio[idx] / (double)P;
+}
+
 #ifdef __HIPCC__
 // ---------------------------------------------------------------- wave64 helpers
 __device__ inline uint32_t lane_id() { return __lane_id(); }

@@ -22,21 +22,19 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------------------- sampler
-__global__ __launch_bounds__(256) void k_sample(int P, uint64_t seed, uint64_t step_arg,
-                                                const unsigned long long* __restrict__ step_dev,
-                                                const double* __restrict__ cdf, int3 M, double w,
-                                                int32_t* __restrict__ km, double* __restrict__ kw)
+__global__ __launch_bounds__(256) void k_sample(SampleArgs sa)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= P) return;
-    const uint64_t step = step_dev ? (uint64_t)(*step_dev) : step_arg;
-    const int Ms[3] = {M.x, M.y, M.z};
+    if (l >= sa.P) return;
+    const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
+    const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
     int m[3];
-    sample_kvector(seed, step, (uint32_t)l, cdf, Ms, m);
-    km[3 * l] = m[0];
-    km[3 * l + 1] = m[1];
-    km[3 * l + 2] = m[2];
-    kw[l] = w;
+    const double w =
+        draw_kvector(sa.seed, step, (uint32_t)l, sa.cdf, Ms, sa.gp, sa.P, sa.w, sa.w_out, m);
+    sa.km[3 * l] = m[0];
+    sa.km[3 * l + 1] = m[1];
+    sa.km[3 * l + 2] = m[2];
+    sa.kw[l] = w;
 }
 
 __global__ void k_fill(double* __restrict__ a, int n, double v)
@@ -292,13 +290,27 @@ __global__ __launch_bounds__(kKfThreads) void k_kforce(
 
 // ================================================================= launchers
 
+SampleArgs sample_args(const Engine& e, int P, uint64_t seed, const uint64_t* step_host)
+{
+    SampleArgs sa;
+    sa.P = P;
+    sa.seed = seed;
+    sa.step = step_host ? *step_host : 0ull;
+    sa.step_dev = step_host ? nullptr : e.d_step;
+    sa.cdf = e.kcdf;
+    sa.M = make_int3(e.kM[0], e.kM[1], e.kM[2]);
+    sa.w = e.S / (double)P;
+    sa.gp = e.gp;
+    sa.w_out = e.gp.cdf ? e.S / ((1.0 - e.gp.lambda) * (double)P) : 0.0;
+    sa.km = e.km;
+    sa.kw = e.kw;
+    return sa;
+}
+
 void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, hipStream_t s)
 {
     ScopedTimer t(e, P_SAMPLE, s);
-    const double w = e.S / (double)P;
-    k_sample<<<(P + 255) / 256, 256, 0, s>>>(P, seed, step_host ? *step_host : 0ull,
-                                              step_host ? nullptr : e.d_step, e.kcdf,
-                                              make_int3(e.kM[0], e.kM[1], e.kM[2]), w, e.km, e.kw);
+    k_sample<<<(P + 255) / 256, 256, 0, s>>>(sample_args(e, P, seed, step_host));
 }
 
 void launch_kweights_default(Engine& e, hipStream_t s)

@@ -387,11 +387,12 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift_check(
             const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
             const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
             int m[3];
-            sample_kvector(sa.seed, step, (uint32_t)l, in_lds ? cdf : sa.cdf, Ms, m);
+            const double w = draw_kvector(sa.seed, step, (uint32_t)l, in_lds ? cdf : sa.cdf, Ms,
+                                          sa.gp, sa.P, sa.w, sa.w_out, m);
             sa.km[3 * l] = m[0];
             sa.km[3 * l + 1] = m[1];
             sa.km[3 * l + 2] = m[2];
-            sa.kw[l] = sa.w;
+            sa.kw[l] = w;
         }
     }
     if (i >= n) return;

@@ -29,7 +29,9 @@ struct SampleArgs {
     const unsigned long long* step_dev = nullptr;  // overrides step when set
     const double* cdf = nullptr;
     int3 M{};
-    double w = 0;
+    double w = 0;       // analytic weight S / P
+    double w_out = 0;   // weight of a mixture draw outside the grid
+    GridProposal gp{};  // learned proposal (gp.cdf == nullptr: analytic only)
     int32_t* km = nullptr;
     double* kw = nullptr;
 };
@@ -86,6 +88,14 @@ struct Engine {
     double S = 0;             // RBE normaliser H_x H_y H_z - 1
     unsigned long long* d_step = nullptr;  // device step counter for graph replays
     bool kvec_weights_default = true;
+    // learned proposal tables (device), active when gp.cdf != nullptr
+    GridProposal gp{};
+    double* gp_cdf = nullptr;
+    double* gp_ratio = nullptr;
+    int32_t* gp_guide = nullptr;
+    int64_t gp_cap = 0;  // entries allocated
+    int64_t gp_guide_cap = 0;
+    uint64_t gp_gen = 0; // bumped by every set_proposal (keys the MD graph)
 
     // energy / virial
     double* ev_part = nullptr;  // [blocks][8]
@@ -136,6 +146,7 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s);
 
 // ---- launchers (kspace.hip)
+SampleArgs sample_args(const Engine& e, int P, uint64_t seed, const uint64_t* step_host);
 void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, hipStream_t s);
 void launch_kweights_default(Engine& e, hipStream_t s);
 void launch_rho(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);

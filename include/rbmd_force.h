@@ -164,6 +164,59 @@ int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int
  * traffic counters against a kernel of known traffic. */
 int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream);
 
+/* ---- learned k-space proposal ----
+ *
+ * Replaces the analytic k-sampler RBEPSAMPLE (rbmd/framework/model/RBEPSample.h:143-175,
+ * called from ExecutionNVT::PreForce, rbmd/framework/run/ExecutionNVT.cpp:331-353) by an
+ * importance proposal: with probability lambda a draw from a tabulated density on the
+ * lattice box |m_d| <= Mg, otherwise a draw from the analytic sampler.  Each sampled k
+ * carries w = a(m) / (P q_mix(m)), a(m) = exp(-k^2/4 alpha), in place of S/P, so the RBE
+ * force and energy estimators stay unbiased.  The reference has no learned sampler. */
+typedef struct rbmd_proposal_s* rbmd_proposal_t;
+
+/* density: HOST array of (2 Mg + 1)^3 values >= 0, x fastest, m_d = -Mg..Mg (the entry
+ * at m = 0 is ignored).  lambda in (0, 1).  alpha and box_len[3] must be those of the
+ * force handles the proposal is used with.  Returns RBMD_ERR_INVALID for a density
+ * that is negative, non-finite or all zero. */
+int rbmd_proposal_create(int32_t Mg, const double* density, double lambda, double alpha,
+                         const double* box_len, rbmd_proposal_t* out);
+void rbmd_proposal_destroy(rbmd_proposal_t p);
+/* The proposal's draws on the host (bit-identical m and w to the device sampler of a
+ * handle using it): m[3P], w[P] HOST. */
+int rbmd_proposal_sample_host(rbmd_proposal_t p, int32_t P, uint64_t seed, uint64_t step,
+                              int32_t* m, double* w);
+/* Use the proposal for all later sample_kvectors / md_run draws of this handle
+ * (p == NULL: back to the analytic sampler).  Copies the tables to the device
+ * (synchronising); the proposal may be destroyed afterwards. */
+int rbmd_force_set_proposal(rbmd_force_t h, rbmd_proposal_t p);
+
+/* ---- proposal network: 4 -> 64 -> 64 -> 1 MLP, SiLU, softplus output ----
+ *
+ * No counterpart in the reference.  Features (|kx|, |ky|, |kz|, |k|) / k_scale in FP32,
+ * BF16 weights with FP32 accumulation in the forward pass, FP32 master weights updated
+ * by Adam.  Parameter vector (rbmd_mlp_nparams() = 4545 FP32 values): W1[64][4], b1[64],
+ * W2[64][64], b2[64], w3[64], b3.  Training loss over a batch: sum_s w_s (log h_s - y_s)^2. */
+typedef struct rbmd_mlp_s* rbmd_mlp_t;
+int rbmd_mlp_nparams(void);
+/* Glorot-uniform weights from Philox(seed), zero biases, b3 = log(e - 1). */
+int rbmd_mlp_create(uint64_t seed, rbmd_mlp_t* out);
+void rbmd_mlp_destroy(rbmd_mlp_t h);
+const char* rbmd_mlp_last_error(rbmd_mlp_t h);
+/* FP32 master weights, HOST arrays of rbmd_mlp_nparams() values (synchronising).
+ * set also resets the Adam moments. */
+int rbmd_mlp_get_params(rbmd_mlp_t h, float* params);
+int rbmd_mlp_set_params(rbmd_mlp_t h, const float* params);
+/* h(f) for feat[4B] (DEVICE) into out[B] (DEVICE). */
+int rbmd_mlp_forward(rbmd_mlp_t h, const float* feat, int64_t B, float* out, void* stream);
+/* Loss and its gradient w.r.t. the parameters at the BF16 weights (synchronising);
+ * feat[4B], y[B], w[B] DEVICE; loss, grad[nparams] HOST (either may be NULL). */
+int rbmd_mlp_loss_grad(rbmd_mlp_t h, const float* feat, const float* y, const float* w,
+                       int64_t B, double* loss, float* grad, void* stream);
+/* iters full-batch Adam steps (beta 0.9/0.999, eps 1e-8) at learning rate lr;
+ * final_loss (HOST, nullable, synchronising) = loss after the last step. */
+int rbmd_mlp_train(rbmd_mlp_t h, const float* feat, const float* y, const float* w, int64_t B,
+                   int32_t iters, double lr, double* final_loss, void* stream);
+
 /* ---- host-side system generators (CPU, deterministic, Philox4x32-10) ---- */
 
 /* Philox4x32-10 block (Salmon et al., SC'11): out[4] for counter ctr[4], key[2]. */
