@@ -42,6 +42,12 @@ WORKLOADS = {
     "c3": dict(n=1048576, rho=0.3, rc_lj=2.5, rc_coul=6.0, alpha=0.3, P=500, dt=0.002,
                steps=200, desc="large charged LJ, plain RBE: 1048576 ions, density 0.3, "
                            "rc_LJ 2.5, rc_coul 6.0, alpha 0.3, P=500, dt 0.002"),
+    "c4": dict(n=100000, rho=0.3, rc_lj=2.5, rc_coul=5.0, alpha=0.5, P=50, dt=0.002,
+               steps=200, learned=True, Mg=32, snapshots=50, snap_every=10,
+               plain_P=(50, 100, 200, 400),
+               desc="learned k-space proposal: 100000 ions, density 0.3, rc_LJ 2.5, "
+                    "rc_coul 5.0, alpha 0.5; MLP 4-64-64-1 trained on FP64 |rho(k)|^2 of 50 "
+                    "MD snapshots on |m_d| <= 32; learned P=50 vs analytic P=50/100/200/400"),
 }
 
 
@@ -94,6 +100,221 @@ def cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, target_s):
                       f"workload, OpenMP over atoms, {el:.1f} s"}
 
 
+def make_force(wl, L, P, kspace="RBE", precision=32, tol=1e-14):
+    from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
+
+    return LJCutCoulKspaceForce(ForceConfig(
+        box_len=(L, L, L), cut_off=wl["rc_lj"], coulomb_cut=wl["rc_coul"], skin=0.3,
+        alpha=wl["alpha"], kspace=kspace, coulomb_sample_num=P, precision=precision,
+        n_max=wl["n"], ewald_tol=tol))
+
+
+class State:
+    """Device MD state (FP64 positions and velocities, FP32 forces)."""
+
+    def __init__(self, X, Y, Z, VX, VY, VZ, F):
+        self.X, self.Y, self.Z, self.VX, self.VY, self.VZ, self.F = X, Y, Z, VX, VY, VZ, F
+
+    def clone(self):
+        return State(*(t.clone() for t in (self.X, self.Y, self.Z, self.VX, self.VY, self.VZ)),
+                     [t.clone() for t in self.F])
+
+    def run(self, f, wl, nsteps, step0, use_graph=True):
+        f.md_run(self.X, self.Y, self.Z, self.VX, self.VY, self.VZ, *self.F, dt=wl["dt"],
+                 nsteps=nsteps, seed=SEED, step0=step0, use_graph=use_graph)
+
+
+def timed(f, st, wl, W, K, step, use_graph):
+    """W warm-up then K timed MD steps; returns (seconds, list builds, next step)."""
+    import torch
+
+    st.run(f, wl, W, step, use_graph)
+    step += W
+    torch.cuda.synchronize()
+    b0 = f.stats()["builds"]
+    t0 = time.perf_counter()
+    st.run(f, wl, K, step, use_graph)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return el, f.stats()["builds"] - b0, step + K
+
+
+def kernel_times(f, st, wl, Kt, step):
+    """Per-kernel device ms per step from HIP events over Kt eager steps, and the
+    pair-pass roofline numbers."""
+    import torch
+
+    f.set_timing(True)
+    st.run(f, wl, Kt, step, use_graph=False)
+    torch.cuda.synchronize()
+    names = ["check", "bin", "scan", "scatter", "cellsort", "list", "gather", "sample", "rho",
+             "rho_final", "pair", "kforce", "kick"]
+    per = {}
+    for nm in names:
+        ms, cnt = f.timing(nm)
+        per[nm] = {"mean_ms": ms, "launches": cnt}
+    f.set_timing(False)
+    E = f.stats()["entries"]
+    n = wl["n"]
+    pair_bytes = 4.0 * E + 40.0 * n
+    pair_ms = per["pair"]["mean_ms"]
+    per_step_ms = {k: v["mean_ms"] * v["launches"] / Kt for k, v in per.items()}
+    log("per-step device ms: " + ", ".join(f"{k} {v:.4f}" for k, v in per_step_ms.items()))
+    return per_step_ms, pair_bytes, pair_ms, E, step + Kt
+
+
+def ewald_refs(wl, L, Q, configs):
+    """FP64 full-Ewald total forces and the Ewald k-space part for each configuration."""
+    import torch
+
+    n = wl["n"]
+    ew = make_force(wl, L, 0, kspace="EWALD", precision=64, tol=1e-14)
+    ew.Init(Q)
+    nk = make_force(wl, L, 0, kspace="NONE", precision=64)
+    nk.Init(Q)
+    refs = []
+    for (X, Y, Z) in configs:
+        Fe = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
+        Fr = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
+        ew.Execute(X, Y, Z, *Fe)
+        nk.Execute(X, Y, Z, *Fr)
+        Fe_t = torch.stack(Fe, 1)
+        refs.append((Fe_t, Fe_t - torch.stack(Fr, 1)))
+    nkv = ew.stats()["ewald_k"]
+    ew.close()
+    nk.close()
+    return refs, nkv
+
+
+def force_error(f, P, configs, refs, lists):
+    """eps_F over configurations x k-lists: RMS of F - F_Ewald relative to the
+    total Ewald force and to its k-space part (same real-space part, so the
+    difference is the k-space estimator's error)."""
+    import torch
+
+    num = den = denk = 0.0
+    n = configs[0][0].numel()
+    F = [torch.zeros(n, device="cuda", dtype=torch.float32) for _ in range(3)]
+    for c, ((X, Y, Z), (Fe, Fek)) in enumerate(zip(configs, refs)):
+        for s in range(lists):
+            f.sample_kvectors(P, SEED + 1, 1000 * c + s)
+            f.Execute(X, Y, Z, *F)
+            num += float(((torch.stack(F, 1).double() - Fe) ** 2).sum())
+            den += float((Fe ** 2).sum())
+            denk += float((Fek ** 2).sum())
+    return (num / den) ** 0.5, (num / denk) ** 0.5
+
+
+def learned_eval(wl, L, Q, st, step, args):
+    """Train the proposal on FP64 spectra of MD snapshots, then measure eps_F and
+    atom-steps/s of learned P vs analytic P values on held-out configurations."""
+    import torch
+
+    from copy_gpu_amd import proposal as PR
+
+    n, alpha, Mg = wl["n"], wl["alpha"], wl["Mg"]
+    f = make_force(wl, L, 100)
+    f.Init(Q)
+    f.sample_kvectors(100, SEED, 0)
+    f.Execute(st.X, st.Y, st.Z, *st.F)
+    st.run(f, wl, args.warmup, step)
+    step += args.warmup
+    snaps = []
+    log(f"trajectory: {wl['snapshots']} snapshots every {wl['snap_every']} steps "
+        "(analytic RBE, P=100)")
+    for i in range(wl["snapshots"]):
+        st.run(f, wl, wl["snap_every"], step)
+        step += wl["snap_every"]
+        snaps.append((st.X.clone(), st.Y.clone(), st.Z.clone()))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    S2 = PR.spectrum(snaps, Q, (L, L, L), alpha, Mg)
+    torch.cuda.synchronize()
+    t_spec = time.perf_counter() - t
+    log(f"FP64 spectra of {len(snaps)} snapshots on {(2 * Mg + 1) ** 3} lattice points: "
+        f"{t_spec:.2f} s")
+    del snaps
+    t = time.perf_counter()
+    prop, mlp, hist = PR.learned_proposal(S2, (L, L, L), alpha, Mg, lam=0.9)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    log(f"training: loss {hist[0]:.4e} -> {hist[-1]:.4e} in {t_train:.2f} s")
+    m = PR.lattice_grid(Mg)
+    k = PR.kvecs(m, (L, L, L))
+    k2 = (k * k).sum(1)
+    a = PR.analytic_weight(m, (L, L, L), alpha)
+    opt = np.where(k2 > 0, a * np.sqrt(S2 / np.where(k2 > 0, k2, 1.0)), 0.0)
+    va = PR.variance_factor(S2, None, (L, L, L), alpha, Mg)
+    g_learned = va / PR.variance_factor(S2, prop.density, (L, L, L), alpha, Mg, prop.lam)
+    g_opt = va / PR.variance_factor(S2, opt, (L, L, L), alpha, Mg, 1.0)
+    log(f"expected variance gain on the training spectra: learned {g_learned:.4f}, "
+        f"optimal k-only proposal {g_opt:.4f}")
+
+    # held-out configurations: 5 more snapshots, 20 steps apart
+    configs = []
+    for i in range(5):
+        st.run(f, wl, 20, step)
+        step += 20
+        configs.append((st.X.clone(), st.Y.clone(), st.Z.clone()))
+    f.close()
+    refs, nkv = ([], 0) if args.skip_ewald else ewald_refs(wl, L, Q, configs)
+    if refs:
+        log(f"FP64 Ewald references: {len(configs)} configurations, {nkv} half-space k-vectors")
+
+    runs = []
+    K = args.steps if args.steps is not None else wl["steps"]
+    settings = [("learned", wl["P"])] + [("analytic", p) for p in wl["plain_P"]]
+    base = st.clone()
+    for kind, P in settings:
+        g = make_force(wl, L, P)
+        g.Init(Q)
+        if kind == "learned":
+            prop.attach(g)
+        eps = epsk = None
+        if refs:
+            eps, epsk = force_error(g, P, configs, refs, 20)
+        s2 = base.clone()
+        g.sample_kvectors(P, SEED, step)
+        g.Execute(s2.X, s2.Y, s2.Z, *s2.F)
+        el, rebuilds, _ = timed(g, s2, wl, 20, K, step + 1, not args.no_graph)
+        rate = n * K / el
+        log(f"{kind:8s} P={P:4d}: eps_F {eps if eps is None else f'{eps:.4e}'} "
+            f"(k-space part {epsk if epsk is None else f'{epsk:.4e}'}), "
+            f"{rate:.4e} atom-steps/s, {el * 1e3 / K:.4f} ms/step")
+        runs.append({"sampler": kind, "P": P, "eps_F": eps, "eps_F_kspace_part": epsk,
+                     "atom_steps_per_s": rate, "ms_per_step": el * 1e3 / K,
+                     "list_builds": rebuilds})
+        if kind == "learned":
+            learned_state = (g, s2)
+        else:
+            g.close()
+    out = {"runs": runs, "train": {
+        "snapshots": wl["snapshots"], "snapshot_spacing_steps": wl["snap_every"],
+        "grid": f"|m_d| <= {Mg} ({(2 * Mg + 1) ** 3} lattice points)",
+        "samples": int((Mg + 1) ** 3 - 1), "adam_iters": 2400,
+        "loss_initial": hist[0], "loss_final": hist[-1],
+        "spectra_s": t_spec, "train_s": t_train, "mixture_lambda": prop.lam},
+        "expected_variance_gain": g_learned, "optimal_konly_variance_gain": g_opt}
+    # analytic P giving the learned eps_F (eps^2 ~ c / P, least squares over the
+    # analytic runs), its throughput by log-log interpolation, and the ratio
+    an = [r for r in runs if r["sampler"] == "analytic"]
+    lr = runs[0]
+    if lr["eps_F"] is not None:
+        Ps = np.array([r["P"] for r in an], float)
+        e2 = np.array([r["eps_F"] ** 2 for r in an])
+        c = float((e2 * Ps).mean())
+        P_eq = c / lr["eps_F"] ** 2
+        rates = np.array([r["atom_steps_per_s"] for r in an])
+        rate_eq = float(np.exp(np.interp(np.log(P_eq), np.log(Ps), np.log(rates))))
+        out.update({"analytic_P_at_learned_eps": P_eq,
+                    "analytic_atom_steps_per_s_at_learned_eps": rate_eq,
+                    "speedup_at_equal_eps": lr["atom_steps_per_s"] / rate_eq,
+                    "variance_gain_measured": P_eq / lr["P"]})
+        log(f"equal eps_F: analytic needs P = {P_eq:.1f}; learned speedup "
+            f"{out['speedup_at_equal_eps']:.3f}x")
+    return out, learned_state, prop, step
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,7 +336,7 @@ def main():
 
     import torch
 
-    from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce, systems
+    from copy_gpu_amd import systems
 
     if args.copy_calibration:
         import ctypes
@@ -135,97 +356,47 @@ def main():
     wl = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else wl["steps"]
     W = args.warmup
-    n, P, alpha = wl["n"], wl["P"], wl["alpha"]
+    n, P = wl["n"], wl["P"]
     dev = torch.device("cuda:0")
     log(f"workload {args.workload}: generating {n} ions")
     x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED)
     vx, vy, vz = systems.velocities(n, 1.0, seed=SEED)
     t64 = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to(dev, torch.float64)  # noqa
-    cfg = ForceConfig(box_len=(L, L, L), cut_off=wl["rc_lj"], coulomb_cut=wl["rc_coul"],
-                      skin=0.3, alpha=alpha, kspace="RBE", coulomb_sample_num=P,
-                      precision=32, n_max=n)
-    f = LJCutCoulKspaceForce(cfg)
-    f.Init(t64(q))
-    X, Y, Z, VX, VY, VZ = (t64(a) for a in (x, y, z, vx, vy, vz))
-    F = [torch.zeros(n, device=dev, dtype=torch.float32) for _ in range(3)]
-    f.sample_kvectors(P, SEED, 0)
-    f.Execute(X, Y, Z, *F)
+    Q = t64(q)
+    st = State(*(t64(a) for a in (x, y, z, vx, vy, vz)),
+               [torch.zeros(n, device=dev, dtype=torch.float32) for _ in range(3)])
     use_graph = not args.no_graph
-    step = 1
-    log(f"warmup {W} steps (graph={use_graph})")
-    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=wl["dt"], nsteps=W, seed=SEED, step0=step,
-             use_graph=use_graph)
-    step += W
-    torch.cuda.synchronize()
-    b0 = f.stats()["builds"]
-    t0 = time.perf_counter()
-    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=wl["dt"], nsteps=K, seed=SEED, step0=step,
-             use_graph=use_graph)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    step += K
-    st = f.stats()
-    rebuilds = st["builds"] - b0
-    value = n * K / el
-    log(f"timed {K} steps: {el * 1e3:.2f} ms -> {value:.4e} atom-steps/s, {rebuilds} list builds")
-    log(f"list/tile stats: {st}")
+    learned = None
+    if wl.get("learned"):
+        learned, (f, st), prop, step = learned_eval(wl, L, Q, st, 1, args)
+        lr = learned["runs"][0]
+        value, el, rebuilds = lr["atom_steps_per_s"], lr["ms_per_step"] * K / 1e3, \
+            lr["list_builds"]
+        eps_total, eps_k = lr["eps_F"], lr["eps_F_kspace_part"]
+    else:
+        f = make_force(wl, L, P)
+        f.Init(Q)
+        f.sample_kvectors(P, SEED, 0)
+        f.Execute(st.X, st.Y, st.Z, *st.F)
+        log(f"warmup {W} steps, timed {K} steps (graph={use_graph})")
+        el, rebuilds, step = timed(f, st, wl, W, K, 1, use_graph)
+        value = n * K / el
+        log(f"timed {K} steps: {el * 1e3:.2f} ms -> {value:.4e} atom-steps/s, "
+            f"{rebuilds} list builds")
+    log(f"list/tile stats: {f.stats()}")
 
-    # per-kernel device times from HIP events (eager launches of the same step)
-    f.set_timing(True)
-    Kt = min(K, 100)
-    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=wl["dt"], nsteps=Kt, seed=SEED, step0=step,
-             use_graph=False)
-    step += Kt
-    torch.cuda.synchronize()
-    names = ["check", "bin", "scan", "scatter", "cellsort", "list", "gather", "sample", "rho",
-             "rho_final", "pair", "kforce", "kick"]
-    per = {}
-    for nm in names:
-        ms, cnt = f.timing(nm)
-        per[nm] = {"mean_ms": ms, "launches": cnt}
-    f.set_timing(False)
-    st = f.stats()
-    E = st["entries"]
-    pair_bytes = 4.0 * E + 40.0 * n
-    pair_ms = per["pair"]["mean_ms"]
+    per_step_ms, pair_bytes, pair_ms, E, step = kernel_times(f, st, wl, min(K, 100), step)
     achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
-    per_step_ms = {k: v["mean_ms"] * v["launches"] / Kt for k, v in per.items()}
     dominant = max(per_step_ms, key=per_step_ms.get)
-    log("per-step device ms: " + ", ".join(f"{k} {v:.4f}" for k, v in per_step_ms.items()))
 
-    # force RMS error vs FP64 full Ewald on the final configuration
-    eps_total = eps_k = None
-    if not args.skip_ewald:
-        log("FP64 Ewald reference on the final configuration")
-        ew = LJCutCoulKspaceForce(ForceConfig(box_len=(L, L, L), cut_off=wl["rc_lj"],
-                                              coulomb_cut=wl["rc_coul"], skin=0.3, alpha=alpha,
-                                              kspace="EWALD", precision=64, n_max=n,
-                                              ewald_tol=1e-14))
-        ew.Init(t64(q))
-        Fe = [torch.zeros(n, device=dev, dtype=torch.float64) for _ in range(3)]
-        ew.Execute(X, Y, Z, *Fe)
-        nk = LJCutCoulKspaceForce(ForceConfig(box_len=(L, L, L), cut_off=wl["rc_lj"],
-                                              coulomb_cut=wl["rc_coul"], skin=0.3, alpha=alpha,
-                                              kspace="NONE", precision=64, n_max=n))
-        nk.Init(t64(q))
-        Fr = [torch.zeros(n, device=dev, dtype=torch.float64) for _ in range(3)]
-        nk.Execute(X, Y, Z, *Fr)
-        Fe_t = torch.stack(Fe, 1)
-        Fek = Fe_t - torch.stack(Fr, 1)
-        # the real-space and LJ parts are identical, so the same difference
-        # is normalised by the total force and by the Ewald k-space force
-        num = 0.0
-        for s in range(20):
-            f.sample_kvectors(P, SEED + 1, s)
-            f.Execute(X, Y, Z, *F)
-            Fg = torch.stack(F, 1).double()
-            num += float(((Fg - Fe_t) ** 2).sum())
-        eps_total = (num / (float((Fe_t ** 2).sum()) * 20)) ** 0.5
-        eps_k = (num / (float((Fek ** 2).sum()) * 20)) ** 0.5
-        log(f"force RMS error vs FP64 Ewald: total {eps_total:.3e}, relative to k-space part "
-            f"{eps_k:.3e} ({ew.stats()['ewald_k']} half-space k-vectors)")
-        ew.close()
-        nk.close()
+    if not learned:
+        eps_total = eps_k = None
+        if not args.skip_ewald:
+            log("FP64 Ewald reference on the final configuration")
+            refs, nkv = ewald_refs(wl, L, Q, [(st.X, st.Y, st.Z)])
+            eps_total, eps_k = force_error(f, P, [(st.X, st.Y, st.Z)], refs, 20)
+            log(f"force RMS error vs FP64 Ewald: total {eps_total:.3e}, relative to k-space "
+                f"part {eps_k:.3e} ({nkv} half-space k-vectors)")
 
     cpu = None
     if not args.skip_cpu:
@@ -249,7 +420,8 @@ def main():
         "config": {
             "workload": wl["desc"],
             "n_atoms": n,
-            "kspace": "plain RBE (analytic Gaussian sampler)",
+            "kspace": ("learned MLP proposal (mixture 0.9 learned / 0.1 analytic)"
+                       if learned else "plain RBE (analytic Gaussian sampler)"),
             "P": P,
             "precision": "FP32 pair and k-space maths; FP64 positions, integrator, energy",
             "hip_graph": use_graph,
@@ -274,6 +446,8 @@ def main():
         "dominant_kernel": dominant,
         "cpu_baseline": cpu,
     }
+    if learned:
+        out["learned_vs_plain"] = learned
     print(json.dumps(out), flush=True)
 
 

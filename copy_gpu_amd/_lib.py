@@ -63,6 +63,13 @@ class EnergyVirial(C.Structure):
 
 # name -> (restype, argtypes); the symbol test checks this table against the header
 _P = C.c_void_p
+class ReplicaState(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+                ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+                ("fx", C.c_void_p), ("fy", C.c_void_p), ("fz", C.c_void_p),
+                ("img", C.c_void_p), ("mass", C.c_void_p)]
+
+
 _SIGS = {
     "rbmd_force_create": (C.c_int, [C.POINTER(ForceParams), C.c_int64, C.POINTER(C.c_void_p)]),
     "rbmd_force_destroy": (None, [_P]),
@@ -81,6 +88,12 @@ _SIGS = {
     "rbmd_force_get_timing": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]),
     "rbmd_util_copy": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "rbmd_batch_create": (C.c_int, [C.c_int32, _P, _P, C.POINTER(C.c_void_p)]),
+    "rbmd_batch_destroy": (None, [_P]),
+    "rbmd_batch_last_error": (C.c_char_p, [_P]),
+    "rbmd_batch_replica": (C.c_void_p, [_P, C.c_int32]),
+    "rbmd_batch_md_run": (C.c_int, [_P, _P, C.c_double, C.c_int64, _P, C.c_uint64, C.c_int32,
+                                    _P]),
     "rbmd_proposal_create": (C.c_int, [C.c_int32, _P, C.c_double, C.c_double, _P,
                                        C.POINTER(C.c_void_p)]),
     "rbmd_proposal_destroy": (None, [_P]),

@@ -801,6 +801,208 @@ int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int
     return fail(E, RBMD_ERR_INVALID, "unknown pass name");
 }
 
+// ---------------------------------------------------------------- replica batches
+
+}  // extern "C"
+
+struct rbmd_batch_s {
+    std::vector<rbmd_force_t> reps;
+    Batch b;
+    hipStream_t own_stream = nullptr;
+    hipEvent_t order_event = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<const void*> graph_key;
+    std::string err;
+};
+
+static int bfail(rbmd_batch_t b, int code, const std::string& msg)
+{
+    b->err = msg;
+    return code;
+}
+
+#define BHIP_TRY(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t _st = (expr);                                                         \
+        if (_st != hipSuccess) return bfail(b, RBMD_ERR_HIP, std::string(#expr) + ": " + \
+                                                                 hipGetErrorString(_st)); \
+    } while (0)
+
+// One batched velocity-Verlet step: every phase is one launch over all replicas.
+static void batch_step(Batch& B, hipStream_t s)
+{
+    const bool ks = B.kspace == RBMD_KSPACE_RBE || B.kspace == RBMD_KSPACE_EWALD;
+    nlist_batch_launch(B, BP_KICKDRIFT, s);
+    for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT, BP_LIST, BP_GATHER})
+        nlist_batch_launch(B, ph, s);
+    if (ks) {
+        kspace_batch_launch(B, BP_RHO, s);
+        kspace_batch_launch(B, BP_RHO_FINAL, s);
+    }
+    pair_batch_launch(B, BP_PAIR, s);
+    pair_batch_launch(B, BP_PAIR_BIG, s);
+    if (ks)
+        kspace_batch_launch(B, BP_KFORCE, s);
+    else
+        nlist_batch_launch(B, BP_KICK, s);
+}
+
+extern "C" {
+
+int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64_t* n_max,
+                      rbmd_batch_t* out)
+{
+    if (!out || nrep < 1 || !params || !n_max) return RBMD_ERR_INVALID;
+    *out = nullptr;
+    rbmd_batch_t b = new rbmd_batch_s();
+    for (int32_t r = 0; r < nrep; ++r) {
+        const rbmd_force_params& p = params[r];
+        if (p.precision != params[0].precision || p.kspace != params[0].kspace ||
+            (p.ntypes > 1) != (params[0].ntypes > 1)) {
+            g_create_err = "replicas must share precision, k-space mode and single/multi type";
+            rbmd_batch_destroy(b);
+            return RBMD_ERR_INVALID;
+        }
+        rbmd_force_t h = nullptr;
+        const int rc = rbmd_force_create(&p, n_max[r], &h);
+        if (rc != RBMD_OK) {
+            rbmd_batch_destroy(b);
+            return rc;
+        }
+        // tiles beyond the LDS stage are rare at these densities: a few
+        // workgroups per replica serve them
+        h->e.big_grid = 4;
+        b->reps.push_back(h);
+        b->b.es.push_back(&h->e);
+    }
+    b->b.precision = params[0].precision;
+    b->b.kspace = params[0].kspace;
+    b->b.multi = params[0].ntypes > 1;
+    if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->order_event, hipEventDisableTiming) != hipSuccess) {
+        g_create_err = "stream/event creation failed";
+        rbmd_batch_destroy(b);
+        return RBMD_ERR_HIP;
+    }
+    *out = b;
+    return RBMD_OK;
+}
+
+void rbmd_batch_destroy(rbmd_batch_t b)
+{
+    if (!b) return;
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto& ph : b->b.ph) ph.release();
+    for (rbmd_force_t h : b->reps) rbmd_force_destroy(h);
+    if (b->own_stream) hipStreamDestroy(b->own_stream);
+    if (b->order_event) hipEventDestroy(b->order_event);
+    delete b;
+}
+
+const char* rbmd_batch_last_error(rbmd_batch_t b)
+{
+    if (!b) return g_create_err.c_str();
+    return b->err.c_str();
+}
+
+rbmd_force_t rbmd_batch_replica(rbmd_batch_t b, int32_t r)
+{
+    if (!b || r < 0 || r >= (int32_t)b->reps.size()) return nullptr;
+    return b->reps[(size_t)r];
+}
+
+int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, int64_t nsteps,
+                      const uint64_t* seeds, uint64_t step0, int32_t use_graph, void* stream)
+{
+    if (!b || !st || !seeds) return RBMD_ERR_INVALID;
+    Batch& B = b->b;
+    const size_t R = b->reps.size();
+    for (size_t r = 0; r < R; ++r) {
+        const Engine& E = *B.es[r];
+        const rbmd_replica_state& q = st[r];
+        if (!E.atoms_set) return bfail(b, RBMD_ERR_STATE, "replica without set_atoms");
+        if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
+            return bfail(b, RBMD_ERR_STATE, "replica without k-vectors: sample once to fix P");
+        if (E.n > 0 && (!q.x || !q.y || !q.z || !q.vx || !q.vy || !q.vz || !q.fx || !q.fy ||
+                        !q.fz))
+            return bfail(b, RBMD_ERR_INVALID, "null state pointer");
+    }
+    if (nsteps <= 0) return RBMD_OK;
+    hipStream_t cs = (hipStream_t)stream;
+    hipStream_t s = cs;
+    if (use_graph && cs == nullptr) {
+        s = b->own_stream;
+        BHIP_TRY(hipEventRecord(b->order_event, cs));
+        BHIP_TRY(hipStreamWaitEvent(s, b->order_event, 0));
+    }
+    for (size_t r = 0; r < R; ++r) {
+        int rc = maintain(*B.es[r], s);
+        if (rc != RBMD_OK) return bfail(b, rc, "replica " + std::to_string(r) + ": " + B.es[r]->err);
+    }
+    // arguments are read from device memory by the batched kernels: wait for
+    // earlier work on the stream before overwriting them
+    BHIP_TRY(hipStreamSynchronize(s));
+    B.st.resize(R);
+    B.sa.assign(R, SampleArgs{});
+    B.dt = dt;
+    for (size_t r = 0; r < R; ++r) {
+        const rbmd_replica_state& q = st[r];
+        RepState& o = B.st[r];
+        o.x = q.x; o.y = q.y; o.z = q.z;
+        o.vx = q.vx; o.vy = q.vy; o.vz = q.vz;
+        o.fx = q.fx; o.fy = q.fy; o.fz = q.fz;
+        o.img = q.img;
+        o.mass = q.mass;
+        Engine& E = *B.es[r];
+        if (E.kspace == RBMD_KSPACE_RBE) B.sa[r] = sample_args(E, E.P, seeds[r], nullptr);
+        const unsigned long long s0 = step0;
+        BHIP_TRY(hipMemcpy(E.d_step, &s0, 8, hipMemcpyHostToDevice));
+    }
+    BHIP_TRY(nlist_batch_prepare(B));
+    BHIP_TRY(pair_batch_prepare(B));
+    if (B.kspace != RBMD_KSPACE_NONE) BHIP_TRY(kspace_batch_prepare(B));
+    if (!use_graph) {
+        for (int64_t k = 0; k < nsteps; ++k) batch_step(B, s);
+        BHIP_TRY(hipGetLastError());
+    } else {
+        std::vector<const void*> key;
+        for (const BatchPhase& ph : B.ph) {
+            key.push_back(ph.args);
+            key.push_back(ph.map);
+            key.push_back((const void*)(uintptr_t)ph.items);
+            key.push_back((const void*)(uintptr_t)ph.lds);
+        }
+        if (!b->graph_exec || key != b->graph_key) {
+            if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+            if (b->graph) hipGraphDestroy(b->graph);
+            b->graph_exec = nullptr;
+            b->graph = nullptr;
+            BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            batch_step(B, s);
+            hipError_t e = hipStreamEndCapture(s, &b->graph);
+            if (e != hipSuccess)
+                return bfail(b, RBMD_ERR_HIP, std::string("graph capture failed: ") +
+                                                  hipGetErrorString(e));
+            BHIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            b->graph_key = key;
+        }
+        for (int64_t k = 0; k < nsteps; ++k) BHIP_TRY(hipGraphLaunch(b->graph_exec, s));
+    }
+    BHIP_TRY(hipStreamSynchronize(s));
+    for (size_t r = 0; r < R; ++r) {
+        Engine& E = *B.es[r];
+        uint32_t fl[2];
+        BHIP_TRY(hipMemcpy(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost));
+        BHIP_TRY(hipMemcpy(fl, E.flags, 8, hipMemcpyDeviceToHost));
+        if (fl[1])
+            return bfail(b, RBMD_ERR_INVALID,
+                         "non-finite positions in replica " + std::to_string(r));
+    }
+    return RBMD_OK;
+}
+
 int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream)
 {
     if (!dst || !src || nbytes < 0 || (nbytes % 16) != 0) return RBMD_ERR_INVALID;

@@ -22,20 +22,24 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------------------- sampler
-__global__ __launch_bounds__(256) void k_sample(SampleArgs sa)
-{
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= sa.P) return;
-    const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
-    const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
-    int m[3];
-    const double w =
-        draw_kvector(sa.seed, step, (uint32_t)l, sa.cdf, Ms, sa.gp, sa.P, sa.w, sa.w_out, m);
-    sa.km[3 * l] = m[0];
-    sa.km[3 * l + 1] = m[1];
-    sa.km[3 * l + 2] = m[2];
-    sa.kw[l] = w;
-}
+struct Sample {
+    static constexpr int kThreads = 256;
+    using Args = SampleArgs;
+    __device__ static void run(const SampleArgs& sa, int blk, int)
+    {
+        const int l = blk * 256 + threadIdx.x;
+        if (l >= sa.P) return;
+        const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
+        const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
+        int m[3];
+        const double w =
+            draw_kvector(sa.seed, step, (uint32_t)l, sa.cdf, Ms, sa.gp, sa.P, sa.w, sa.w_out, m);
+        sa.km[3 * l] = m[0];
+        sa.km[3 * l + 1] = m[1];
+        sa.km[3 * l + 2] = m[2];
+        sa.kw[l] = w;
+    }
+};
 
 __global__ void k_fill(double* __restrict__ a, int n, double v)
 {
@@ -87,135 +91,161 @@ struct Kahan<double> {
 };
 
 // ---------------------------------------------------------------- rho partials
-// grid = (atom tiles, k tiles of 64).  Lane <-> k within the tile, the 4 waves
-// split the tile's atoms; atoms are staged through LDS 256 at a time.
+// Workgroups = (atom tiles) x (k tiles of 64), atom tile fastest.  Lane <-> k
+// within the tile, the 4 waves split the tile's atoms; atoms are staged
+// through LDS 256 at a time.
 constexpr int kRhoBlock = 256;
 constexpr int kRhoStage = 256;
 
 template <typename T>
-__global__ __launch_bounds__(kRhoBlock) void k_rho_partial(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ q, int n, int atoms_per_block, Box box,
-    const int32_t* __restrict__ km, int P, double* __restrict__ kpart)
-{
+struct RhoPartial {
+    static constexpr int kThreads = kRhoBlock;
+    struct Args {
+        const double *x, *y, *z, *q;
+        int n, atoms_per_block, nbx;
+        Box box;
+        const int32_t* km;
+        int P;
+        double* kpart;
+    };
+    __device__ static void run(const Args& A, int blk, int)
+    {
 #pragma clang fp contract(off)
-    __shared__ double4 st[kRhoStage];
-    __shared__ double red[kRhoBlock / 64][64][2];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int kidx = blockIdx.y * 64 + lane;
-    const bool kok = kidx < P;
-    double mx = 0, my = 0, mz = 0;
-    if (kok) {
-        mx = (double)km[3 * kidx];
-        my = (double)km[3 * kidx + 1];
-        mz = (double)km[3 * kidx + 2];
-    }
-    Kahan<T> re, im;
-    const int a0 = blockIdx.x * atoms_per_block;
-    int a1 = a0 + atoms_per_block;
-    a1 = a1 < n ? a1 : n;
-    for (int base = a0; base < a1; base += kRhoStage) {
-        __syncthreads();
-        const int i = base + threadIdx.x;
-        if (threadIdx.x < kRhoStage) {
-            if (i < a1)
-                st[threadIdx.x] = make_double4((x[i] - box.lo[0]) * box.invL[0],
-                                               (y[i] - box.lo[1]) * box.invL[1],
-                                               (z[i] - box.lo[2]) * box.invL[2], q[i]);
-            else
-                st[threadIdx.x] = make_double4(0, 0, 0, 0);
-        }
-        __syncthreads();
-        const int cnt = (a1 - base) < kRhoStage ? (a1 - base) : kRhoStage;
+        __shared__ double4 st[kRhoStage];
+        __shared__ double red[kRhoBlock / 64][64][2];
+        const int bx = blk % A.nbx, ky = blk / A.nbx;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int kidx = ky * 64 + lane;
+        const int P = A.P;
+        const bool kok = kidx < P;
+        double mx = 0, my = 0, mz = 0;
         if (kok) {
-            for (int a = wv; a < cnt; a += kRhoBlock / 64) {
-                const double4 p = st[a];
-                const double phi = fma(mz, p.z, fma(my, p.y, mx * p.x));
-                T sn, cs;
-                sincos2pi<T>(phi, &sn, &cs);
-                const T qa = (T)p.w;
-                re.add(qa * cs);
-                im.add(qa * sn);
+            mx = (double)A.km[3 * kidx];
+            my = (double)A.km[3 * kidx + 1];
+            mz = (double)A.km[3 * kidx + 2];
+        }
+        Kahan<T> re, im;
+        const int a0 = bx * A.atoms_per_block;
+        int a1 = a0 + A.atoms_per_block;
+        a1 = a1 < A.n ? a1 : A.n;
+        const Box& box = A.box;
+        for (int base = a0; base < a1; base += kRhoStage) {
+            __syncthreads();
+            const int i = base + threadIdx.x;
+            if (threadIdx.x < kRhoStage) {
+                if (i < a1)
+                    st[threadIdx.x] = make_double4((A.x[i] - box.lo[0]) * box.invL[0],
+                                                   (A.y[i] - box.lo[1]) * box.invL[1],
+                                                   (A.z[i] - box.lo[2]) * box.invL[2], A.q[i]);
+                else
+                    st[threadIdx.x] = make_double4(0, 0, 0, 0);
+            }
+            __syncthreads();
+            const int cnt = (a1 - base) < kRhoStage ? (a1 - base) : kRhoStage;
+            if (kok) {
+                for (int a = wv; a < cnt; a += kRhoBlock / 64) {
+                    const double4 p = st[a];
+                    const double phi = fma(mz, p.z, fma(my, p.y, mx * p.x));
+                    T sn, cs;
+                    sincos2pi<T>(phi, &sn, &cs);
+                    const T qa = (T)p.w;
+                    re.add(qa * cs);
+                    im.add(qa * sn);
+                }
             }
         }
-    }
-    red[wv][lane][0] = re.value();
-    red[wv][lane][1] = im.value();
-    __syncthreads();
-    if (wv == 0 && kok) {
-        double r = 0.0, m = 0.0;
-        for (int w = 0; w < kRhoBlock / 64; ++w) {
-            r += red[w][lane][0];
-            m += red[w][lane][1];
+        red[wv][lane][0] = re.value();
+        red[wv][lane][1] = im.value();
+        __syncthreads();
+        if (wv == 0 && kok) {
+            double r = 0.0, m = 0.0;
+            for (int w = 0; w < kRhoBlock / 64; ++w) {
+                r += red[w][lane][0];
+                m += red[w][lane][1];
+            }
+            const size_t o = ((size_t)bx * P + kidx) * 2;
+            A.kpart[o] = r;
+            A.kpart[o + 1] = m;
         }
-        const size_t o = ((size_t)blockIdx.x * P + kidx) * 2;
-        kpart[o] = r;
-        kpart[o + 1] = m;
     }
-}
+};
 
 // ---------------------------------------------------------------- rho final
-// One block per k: FP64 tree over the atom-tile partials, then the force
+// One workgroup per k: FP64 tree over the atom-tile partials, then the force
 // coefficients and the energy / virial terms of this k.
 template <typename T>
-__global__ __launch_bounds__(256) void k_rho_final(
-    const double* __restrict__ kpart, int nbx, int P, const int32_t* __restrict__ km,
-    const double* __restrict__ kw, Box box, double alpha, double qqr2e,
-    double* __restrict__ rho, KCoef<T>* __restrict__ coef, double* __restrict__ kev,
-    unsigned long long* __restrict__ step_dev)
-{
-    __shared__ double sr[256], si[256];
-    const int k = blockIdx.x, t = threadIdx.x;
-    double r = 0.0, m = 0.0;
-    for (int b = t; b < nbx; b += 256) {
-        r += kpart[((size_t)b * P + k) * 2];
-        m += kpart[((size_t)b * P + k) * 2 + 1];
-    }
-    sr[t] = r;
-    si[t] = m;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) {
-            sr[t] += sr[t + o];
-            si[t] += si[t + o];
+struct RhoFinal {
+    static constexpr int kThreads = 256;
+    struct Args {
+        const double* kpart;
+        int nbx, P;
+        const int32_t* km;
+        const double* kw;
+        Box box;
+        double alpha, qqr2e;
+        double* rho;
+        KCoef<T>* coef;
+        double* kev;
+        unsigned long long* step_dev;
+    };
+    __device__ static void run(const Args& A, int blk, int)
+    {
+        __shared__ double sr[256], si[256];
+        const int k = blk, t = threadIdx.x;
+        const int P = A.P;
+        double r = 0.0, m = 0.0;
+        for (int b = t; b < A.nbx; b += 256) {
+            r += A.kpart[((size_t)b * P + k) * 2];
+            m += A.kpart[((size_t)b * P + k) * 2 + 1];
         }
+        sr[t] = r;
+        si[t] = m;
         __syncthreads();
-    }
-    if (t == 0) {
-        const double re = sr[0], im = si[0];
-        rho[2 * k] = re;
-        rho[2 * k + 1] = im;
-        const double mm[3] = {(double)km[3 * k], (double)km[3 * k + 1], (double)km[3 * k + 2]};
-        double kv[3];
-        for (int d = 0; d < 3; ++d) kv[d] = 2.0 * kPi * mm[d] / box.L[d];
-        const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
-        const double V = box.L[0] * box.L[1] * box.L[2];
-        const double w = kw[k];
-        const double c = w * (-4.0 * kPi) / (V * k2) * qqr2e;
-        KCoef<T> kc;
-        for (int d = 0; d < 3; ++d) {
-            kc.m[d] = mm[d];
-            kc.k[d] = (T)kv[d];
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) {
+                sr[t] += sr[t + o];
+                si[t] += si[t + o];
+            }
+            __syncthreads();
         }
-        kc.A = (T)(c * im);
-        kc.B = (T)(c * re);
-        coef[k] = kc;
-        const double e = w * (2.0 * kPi / V) * (re * re + im * im) / k2 * qqr2e;
-        const double b = 2.0 * (1.0 / k2 + 1.0 / (4.0 * alpha));
-        double* o = kev + (size_t)k * 7;
-        o[0] = e;
-        o[1] = e * (1.0 - b * kv[0] * kv[0]);
-        o[2] = e * (1.0 - b * kv[1] * kv[1]);
-        o[3] = e * (1.0 - b * kv[2] * kv[2]);
-        o[4] = -e * b * kv[0] * kv[1];
-        o[5] = -e * b * kv[0] * kv[2];
-        o[6] = -e * b * kv[1] * kv[2];
-        if (k == 0 && step_dev) *step_dev += 1ull;
+        if (t == 0) {
+            const Box& box = A.box;
+            const double re = sr[0], im = si[0];
+            A.rho[2 * k] = re;
+            A.rho[2 * k + 1] = im;
+            const double mm[3] = {(double)A.km[3 * k], (double)A.km[3 * k + 1],
+                                  (double)A.km[3 * k + 2]};
+            double kv[3];
+            for (int d = 0; d < 3; ++d) kv[d] = 2.0 * kPi * mm[d] / box.L[d];
+            const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+            const double V = box.L[0] * box.L[1] * box.L[2];
+            const double w = A.kw[k];
+            const double c = w * (-4.0 * kPi) / (V * k2) * A.qqr2e;
+            KCoef<T> kc;
+            for (int d = 0; d < 3; ++d) {
+                kc.m[d] = mm[d];
+                kc.k[d] = (T)kv[d];
+            }
+            kc.A = (T)(c * im);
+            kc.B = (T)(c * re);
+            A.coef[k] = kc;
+            const double e = w * (2.0 * kPi / V) * (re * re + im * im) / k2 * A.qqr2e;
+            const double b = 2.0 * (1.0 / k2 + 1.0 / (4.0 * A.alpha));
+            double* o = A.kev + (size_t)k * 7;
+            o[0] = e;
+            o[1] = e * (1.0 - b * kv[0] * kv[0]);
+            o[2] = e * (1.0 - b * kv[1] * kv[1]);
+            o[3] = e * (1.0 - b * kv[2] * kv[2]);
+            o[4] = -e * b * kv[0] * kv[1];
+            o[5] = -e * b * kv[0] * kv[2];
+            o[6] = -e * b * kv[1] * kv[2];
+            if (k == 0 && A.step_dev) *A.step_dev += 1ull;
+        }
     }
-}
+};
 
 // ---------------------------------------------------------------- k-space force
-// Block = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes
+// Workgroup = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes
 // k = w, w+8, ... of the coefficients staged in LDS (every lane reads the same
 // k: broadcast), then the 8 partial forces of each atom are summed in a fixed
 // order.  F_i += q_i sum_k k (cos(phi) A_k - sin(phi) B_k).  With KICK the
@@ -226,64 +256,145 @@ constexpr int kKfThreads = kKfAtoms * kKfSlices;
 constexpr int kKfStage = 512;
 
 template <typename T, bool KICK>
-__global__ __launch_bounds__(kKfThreads) void k_kforce(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ q, int n, Box box, const KCoef<T>* __restrict__ coef, int P,
-    T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ fz, double* __restrict__ vx,
-    double* __restrict__ vy, double* __restrict__ vz, const double* __restrict__ mass, double dt)
-{
+struct KForce {
+    static constexpr int kThreads = kKfThreads;
+    struct Args {
+        const double *x, *y, *z, *q;
+        int n;
+        Box box;
+        const KCoef<T>* coef;
+        int P;
+        T *fx, *fy, *fz;
+        double *vx, *vy, *vz;
+        const double* mass;
+        double dt;
+    };
+    __device__ static void run(const Args& A, int blk, int)
+    {
 #pragma clang fp contract(off)
-    __shared__ KCoef<T> st[kKfStage];
-    __shared__ T red[kKfSlices][3][kKfAtoms];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = blockIdx.x * kKfAtoms + lane;
-    double sx = 0, sy = 0, sz = 0;
-    if (i < n) {
-        sx = (x[i] - box.lo[0]) * box.invL[0];
-        sy = (y[i] - box.lo[1]) * box.invL[1];
-        sz = (z[i] - box.lo[2]) * box.invL[2];
-    }
-    T ax = 0, ay = 0, az = 0;
-    for (int base = 0; base < P; base += kKfStage) {
+        __shared__ KCoef<T> st[kKfStage];
+        __shared__ T red[kKfSlices][3][kKfAtoms];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int i = blk * kKfAtoms + lane;
+        const int n = A.n, P = A.P;
+        const Box& box = A.box;
+        double sx = 0, sy = 0, sz = 0;
+        if (i < n) {
+            sx = (A.x[i] - box.lo[0]) * box.invL[0];
+            sy = (A.y[i] - box.lo[1]) * box.invL[1];
+            sz = (A.z[i] - box.lo[2]) * box.invL[2];
+        }
+        T ax = 0, ay = 0, az = 0;
+        for (int base = 0; base < P; base += kKfStage) {
+            __syncthreads();
+            for (int k = threadIdx.x; k < kKfStage && base + k < P; k += kKfThreads)
+                st[k] = A.coef[base + k];
+            __syncthreads();
+            const int cnt = (P - base) < kKfStage ? (P - base) : kKfStage;
+            for (int k = wv; k < cnt; k += kKfSlices) {
+                const KCoef<T>& c = st[k];
+                const double phi = fma(c.m[2], sz, fma(c.m[1], sy, c.m[0] * sx));
+                T sn, cs;
+                sincos2pi<T>(phi, &sn, &cs);
+                const T g = cs * c.A - sn * c.B;
+                ax += g * c.k[0];
+                ay += g * c.k[1];
+                az += g * c.k[2];
+            }
+        }
+        red[wv][0][lane] = ax;
+        red[wv][1][lane] = ay;
+        red[wv][2][lane] = az;
         __syncthreads();
-        for (int k = threadIdx.x; k < kKfStage && base + k < P; k += kKfThreads)
-            st[k] = coef[base + k];
-        __syncthreads();
-        const int cnt = (P - base) < kKfStage ? (P - base) : kKfStage;
-        for (int k = wv; k < cnt; k += kKfSlices) {
-            const KCoef<T>& c = st[k];
-            const double phi = fma(c.m[2], sz, fma(c.m[1], sy, c.m[0] * sx));
-            T sn, cs;
-            sincos2pi<T>(phi, &sn, &cs);
-            const T g = cs * c.A - sn * c.B;
-            ax += g * c.k[0];
-            ay += g * c.k[1];
-            az += g * c.k[2];
+        if (wv == 0 && i < n) {
+            T s0 = 0, s1 = 0, s2 = 0;
+            for (int w = 0; w < kKfSlices; ++w) {
+                s0 += red[w][0][lane];
+                s1 += red[w][1][lane];
+                s2 += red[w][2][lane];
+            }
+            const T qi = (T)A.q[i];
+            const T f0 = A.fx[i] + qi * s0, f1 = A.fy[i] + qi * s1, f2 = A.fz[i] + qi * s2;
+            A.fx[i] = f0;
+            A.fy[i] = f1;
+            A.fz[i] = f2;
+            if (KICK) {
+                const double h = 0.5 * A.dt / (A.mass ? A.mass[i] : 1.0);
+                A.vx[i] += h * (double)f0;
+                A.vy[i] += h * (double)f1;
+                A.vz[i] += h * (double)f2;
+            }
         }
     }
-    red[wv][0][lane] = ax;
-    red[wv][1][lane] = ay;
-    red[wv][2][lane] = az;
-    __syncthreads();
-    if (wv == 0 && i < n) {
-        T s0 = 0, s1 = 0, s2 = 0;
-        for (int w = 0; w < kKfSlices; ++w) {
-            s0 += red[w][0][lane];
-            s1 += red[w][1][lane];
-            s2 += red[w][2][lane];
-        }
-        const T qi = (T)q[i];
-        const T f0 = fx[i] + qi * s0, f1 = fy[i] + qi * s1, f2 = fz[i] + qi * s2;
-        fx[i] = f0;
-        fy[i] = f1;
-        fz[i] = f2;
-        if (KICK) {
-            const double h = 0.5 * dt / (mass ? mass[i] : 1.0);
-            vx[i] += h * (double)f0;
-            vy[i] += h * (double)f1;
-            vz[i] += h * (double)f2;
-        }
+};
+
+void rho_geometry(const Engine& e, int* atoms_per_block, int* nbx, int* nkt)
+{
+    const int n = (int)e.n;
+    *nkt = (e.P + 63) / 64;
+    int want = 2048 / *nkt;
+    if (want < 1) want = 1;
+    int apb = (n + want - 1) / want;
+    apb = ((apb + 63) / 64) * 64;
+    if (apb < 64) apb = 64;
+    *atoms_per_block = apb;
+    *nbx = (n + apb - 1) / apb;
+    if (*nbx < 1) *nbx = 1;
+}
+
+template <typename T>
+typename RhoPartial<T>::Args rho_args(Engine& e, const double* x, const double* y,
+                                      const double* z, int* blocks)
+{
+    int apb, nbx, nkt;
+    rho_geometry(e, &apb, &nbx, &nkt);
+    *blocks = nbx * nkt;
+    return typename RhoPartial<T>::Args{x, y, z, e.q, (int)e.n, apb, nbx, e.box, e.km, e.P,
+                                        e.kpart};
+}
+
+template <typename T>
+typename RhoFinal<T>::Args rho_final_args(Engine& e, unsigned long long* step_dev)
+{
+    int apb, nbx, nkt;
+    rho_geometry(e, &apb, &nbx, &nkt);
+    return typename RhoFinal<T>::Args{e.kpart, nbx, e.P, e.km, e.kw, e.box, e.alpha, e.qqr2e,
+                                      e.rho, (KCoef<T>*)e.kcoef, e.kev, step_dev};
+}
+
+template <typename T, bool K>
+typename KForce<T, K>::Args kforce_args(Engine& e, const double* x, const double* y,
+                                        const double* z, void* fx, void* fy, void* fz,
+                                        double* vx, double* vy, double* vz, const double* mass,
+                                        double dt)
+{
+    return typename KForce<T, K>::Args{x,        y,      z,        e.q,    (int)e.n, e.box,
+                                       (const KCoef<T>*)e.kcoef, e.P, (T*)fx, (T*)fy, (T*)fz,
+                                       vx,       vy,     vz,       mass,   dt};
+}
+
+template <typename T>
+hipError_t kspace_prepare_t(Batch& b)
+{
+    const size_t R = b.es.size();
+    std::vector<typename RhoPartial<T>::Args> ra(R);
+    std::vector<typename RhoFinal<T>::Args> fa(R);
+    std::vector<typename KForce<T, true>::Args> ka(R);
+    std::vector<int> nr(R), nf(R), nk(R);
+    for (size_t r = 0; r < R; ++r) {
+        Engine& e = *b.es[r];
+        const RepState& st = b.st[r];
+        ra[r] = rho_args<T>(e, st.x, st.y, st.z, &nr[r]);
+        fa[r] = rho_final_args<T>(e, e.d_step);
+        ka[r] = kforce_args<T, true>(e, st.x, st.y, st.z, st.fx, st.fy, st.fz, st.vx, st.vy,
+                                     st.vz, st.mass, b.dt);
+        nf[r] = e.P;
+        nk[r] = (int)((e.n + kKfAtoms - 1) / kKfAtoms);
     }
+    hipError_t err;
+    if ((err = phase_upload(b.ph[BP_RHO], ra, nr, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_RHO_FINAL], fa, nf, 0)) != hipSuccess) return err;
+    return phase_upload(b.ph[BP_KFORCE], ka, nk, 0);
 }
 
 }  // namespace
@@ -310,7 +421,7 @@ SampleArgs sample_args(const Engine& e, int P, uint64_t seed, const uint64_t* st
 void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, hipStream_t s)
 {
     ScopedTimer t(e, P_SAMPLE, s);
-    k_sample<<<(P + 255) / 256, 256, 0, s>>>(sample_args(e, P, seed, step_host));
+    k_one<Sample><<<(P + 255) / 256, 256, 0, s>>>(sample_args(e, P, seed, step_host));
 }
 
 void launch_kweights_default(Engine& e, hipStream_t s)
@@ -318,44 +429,26 @@ void launch_kweights_default(Engine& e, hipStream_t s)
     k_fill<<<(e.P + 255) / 256, 256, 0, s>>>(e.kw, e.P, e.S / (double)e.P);
 }
 
-static void rho_geometry(const Engine& e, int* atoms_per_block, int* nbx, int* nkt)
-{
-    const int n = (int)e.n;
-    *nkt = (e.P + 63) / 64;
-    int want = 2048 / *nkt;
-    if (want < 1) want = 1;
-    int apb = (n + want - 1) / want;
-    apb = ((apb + 63) / 64) * 64;
-    if (apb < 64) apb = 64;
-    *atoms_per_block = apb;
-    *nbx = (n + apb - 1) / apb;
-    if (*nbx < 1) *nbx = 1;
-}
-
 void launch_rho(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
 {
-    int apb, nbx, nkt;
-    rho_geometry(e, &apb, &nbx, &nkt);
+    int blocks;
+    unsigned long long* step_dev = e.capturing ? e.d_step : nullptr;
     {
         ScopedTimer t(e, P_RHO, s);
-        dim3 grid(nbx, nkt);
-        if (e.precision == 32)
-            k_rho_partial<float><<<grid, kRhoBlock, 0, s>>>(x, y, z, e.q, (int)e.n, apb, e.box,
-                                                             e.km, e.P, e.kpart);
-        else
-            k_rho_partial<double><<<grid, kRhoBlock, 0, s>>>(x, y, z, e.q, (int)e.n, apb, e.box,
-                                                              e.km, e.P, e.kpart);
+        if (e.precision == 32) {
+            const auto a = rho_args<float>(e, x, y, z, &blocks);
+            k_one<RhoPartial<float>><<<blocks, kRhoBlock, 0, s>>>(a);
+        } else {
+            const auto a = rho_args<double>(e, x, y, z, &blocks);
+            k_one<RhoPartial<double>><<<blocks, kRhoBlock, 0, s>>>(a);
+        }
     }
     {
         ScopedTimer t(e, P_RHO_FINAL, s);
         if (e.precision == 32)
-            k_rho_final<float><<<e.P, 256, 0, s>>>(e.kpart, nbx, e.P, e.km, e.kw, e.box, e.alpha,
-                                                    e.qqr2e, e.rho, (KCoef<float>*)e.kcoef, e.kev,
-                                                    e.capturing ? e.d_step : nullptr);
+            k_one<RhoFinal<float>><<<e.P, 256, 0, s>>>(rho_final_args<float>(e, step_dev));
         else
-            k_rho_final<double><<<e.P, 256, 0, s>>>(e.kpart, nbx, e.P, e.km, e.kw, e.box, e.alpha,
-                                                     e.qqr2e, e.rho, (KCoef<double>*)e.kcoef,
-                                                     e.kev, e.capturing ? e.d_step : nullptr);
+            k_one<RhoFinal<double>><<<e.P, 256, 0, s>>>(rho_final_args<double>(e, step_dev));
     }
 }
 
@@ -364,13 +457,11 @@ void launch_kforce(Engine& e, const double* x, const double* y, const double* z,
                    double dt, hipStream_t s)
 {
     ScopedTimer t(e, P_KFORCE, s);
-    const int n = (int)e.n;
-    const int blocks = (n + kKfAtoms - 1) / kKfAtoms;
+    const int blocks = (int)((e.n + kKfAtoms - 1) / kKfAtoms);
     const bool kick = vx != nullptr;
 #define RBMD_KF(T, K)                                                                          \
-    k_kforce<T, K><<<blocks, kKfThreads, 0, s>>>(x, y, z, e.q, n, e.box,                       \
-                                                 (const KCoef<T>*)e.kcoef, e.P, (T*)fx, (T*)fy, \
-                                                 (T*)fz, vx, vy, vz, mass, dt)
+    k_one<KForce<T, K>><<<blocks, kKfThreads, 0, s>>>(                                        \
+        kforce_args<T, K>(e, x, y, z, fx, fy, fz, vx, vy, vz, mass, dt))
     if (e.precision == 32) {
         if (kick) RBMD_KF(float, true);
         else RBMD_KF(float, false);
@@ -379,6 +470,31 @@ void launch_kforce(Engine& e, const double* x, const double* y, const double* z,
         else RBMD_KF(double, false);
     }
 #undef RBMD_KF
+}
+
+hipError_t kspace_batch_prepare(Batch& b)
+{
+    return b.precision == 32 ? kspace_prepare_t<float>(b) : kspace_prepare_t<double>(b);
+}
+
+void kspace_batch_launch(Batch& b, int phase, hipStream_t s)
+{
+    const bool f32 = b.precision == 32;
+    switch (phase) {
+    case BP_RHO:
+        if (f32) phase_launch<RhoPartial<float>>(b.ph[phase], s);
+        else phase_launch<RhoPartial<double>>(b.ph[phase], s);
+        break;
+    case BP_RHO_FINAL:
+        if (f32) phase_launch<RhoFinal<float>>(b.ph[phase], s);
+        else phase_launch<RhoFinal<double>>(b.ph[phase], s);
+        break;
+    case BP_KFORCE:
+        if (f32) phase_launch<KForce<float, true>>(b.ph[phase], s);
+        else phase_launch<KForce<double, true>>(b.ph[phase], s);
+        break;
+    default: break;
+    }
 }
 
 }  // namespace rbmd

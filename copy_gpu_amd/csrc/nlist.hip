@@ -25,26 +25,31 @@ __device__ inline double min_image(double t, double L, double invL)
 // ---------------------------------------------------------------- per-step check
 // Max-displacement test against the positions of the last build; zeroes the
 // cell histogram so that a rebuild can follow.
-__global__ __launch_bounds__(kBlock) void k_check(const double* __restrict__ x,
-                                                  const double* __restrict__ y,
-                                                  const double* __restrict__ z,
-                                                  const double* __restrict__ xr,
-                                                  const double* __restrict__ yr,
-                                                  const double* __restrict__ zr, int n, Box box,
-                                                  double thr2, uint32_t* __restrict__ flags,
-                                                  int32_t* __restrict__ cell_count, int ncell)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncell) cell_count[i] = 0;
-    if (i < n) {
-        double dx = min_image(x[i] - xr[i], box.L[0], box.invL[0]);
-        double dy = min_image(y[i] - yr[i], box.L[1], box.invL[1]);
-        double dz = min_image(z[i] - zr[i], box.L[2], box.invL[2]);
-        double d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 > thr2) flags[0] = 1u;
-        if (d2 != d2) flags[1] = 1u;
+struct Check {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const double *x, *y, *z, *xr, *yr, *zr;
+        int n;
+        Box box;
+        double thr2;
+        uint32_t* flags;
+        int32_t* cell_count;
+        int ncell;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i < a.ncell) a.cell_count[i] = 0;
+        if (i < a.n) {
+            double dx = min_image(a.x[i] - a.xr[i], a.box.L[0], a.box.invL[0]);
+            double dy = min_image(a.y[i] - a.yr[i], a.box.L[1], a.box.invL[1]);
+            double dz = min_image(a.z[i] - a.zr[i], a.box.L[2], a.box.invL[2]);
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > a.thr2) a.flags[0] = 1u;
+            if (d2 != d2) a.flags[1] = 1u;
+        }
     }
-}
+};
 
 __global__ void k_set_flag(uint32_t* flags) { flags[0] = 1u; }
 
@@ -52,98 +57,114 @@ __global__ void k_set_flag(uint32_t* flags) { flags[0] = 1u; }
 // Wrap into the box (x - L floor((x-lo)/L), no contraction: the CPU oracle
 // computes the same bits), cell = trunc((x-lo)/cell) clamped, rank inside the
 // cell from an atomic histogram.
-__global__ __launch_bounds__(kBlock) void k_bin(const double* __restrict__ x,
-                                                const double* __restrict__ y,
-                                                const double* __restrict__ z,
-                                                double* __restrict__ xr, double* __restrict__ yr,
-                                                double* __restrict__ zr, int n, Box box,
-                                                int3 dims, double3 cell,
-                                                int32_t* __restrict__ cell_count,
-                                                int32_t* __restrict__ atom_cell,
-                                                int32_t* __restrict__ atom_rank,
-                                                const uint32_t* __restrict__ flags)
-{
+struct Bin {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const double *x, *y, *z;
+        double *xr, *yr, *zr;
+        int n;
+        Box box;
+        int3 dims;
+        double3 cell;
+        int32_t *cell_count, *atom_cell, *atom_rank;
+        const uint32_t* flags;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
 #pragma clang fp contract(off)
-    if (flags[0] == 0u) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double p[3] = {x[i], y[i], z[i]};
-    const int dm[3] = {dims.x, dims.y, dims.z};
-    const double cs[3] = {cell.x, cell.y, cell.z};
-    double w[3];
-    int c[3];
-    for (int d = 0; d < 3; ++d) {
-        double f = floor((p[d] - box.lo[d]) / box.L[d]);
-        double fl = f * box.L[d];
-        w[d] = p[d] - fl;
-        int v = (int)((w[d] - box.lo[d]) / cs[d]);
-        v = v < 0 ? 0 : v;
-        v = v > dm[d] - 1 ? dm[d] - 1 : v;
-        c[d] = v;
+        if (a.flags[0] == 0u) return;
+        const int i = blk * kBlock + threadIdx.x;
+        if (i >= a.n) return;
+        const double p[3] = {a.x[i], a.y[i], a.z[i]};
+        const int dm[3] = {a.dims.x, a.dims.y, a.dims.z};
+        const double cs[3] = {a.cell.x, a.cell.y, a.cell.z};
+        double w[3];
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+            double f = floor((p[d] - a.box.lo[d]) / a.box.L[d]);
+            double fl = f * a.box.L[d];
+            w[d] = p[d] - fl;
+            int v = (int)((w[d] - a.box.lo[d]) / cs[d]);
+            v = v < 0 ? 0 : v;
+            v = v > dm[d] - 1 ? dm[d] - 1 : v;
+            c[d] = v;
+        }
+        a.xr[i] = w[0];
+        a.yr[i] = w[1];
+        a.zr[i] = w[2];
+        const int cid = c[0] + c[1] * a.dims.x + c[2] * a.dims.x * a.dims.y;
+        a.atom_cell[i] = cid;
+        a.atom_rank[i] = atomicAdd(&a.cell_count[cid], 1);
     }
-    xr[i] = w[0];
-    yr[i] = w[1];
-    zr[i] = w[2];
-    const int cid = c[0] + c[1] * dims.x + c[2] * dims.x * dims.y;
-    atom_cell[i] = cid;
-    atom_rank[i] = atomicAdd(&cell_count[cid], 1);
-}
+};
 
 // ---------------------------------------------------------------- scan
 // Exclusive scan of the cell histogram by one workgroup (wave prefix sums
 // combined through LDS, chunks of 1024 with a running carry).
-__global__ __launch_bounds__(1024) void k_scan(const int32_t* __restrict__ count,
-                                               int32_t* __restrict__ start, int ncell,
-                                               const uint32_t* __restrict__ flags,
-                                               unsigned long long* __restrict__ stats)
-{
-    if (flags[0] == 0u) return;
-    __shared__ int32_t wsum[16];
-    __shared__ int32_t carry;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (t == 0) {
-        carry = 0;
-        stats[0] += 1ull;
-        stats[1] = 0ull;
-        stats[2] = 0ull;
-        stats[3] = 0ull;
-        stats[4] = 0ull;
-        stats[6] = 0ull;
-    }
-    __syncthreads();
-    for (int base = 0; base < ncell; base += 1024) {
-        const int i = base + t;
-        const int v = i < ncell ? count[i] : 0;
-        int incl = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int u = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += u;
+struct Scan {
+    static constexpr int kThreads = 1024;
+    struct Args {
+        const int32_t* count;
+        int32_t* start;
+        int ncell;
+        const uint32_t* flags;
+        unsigned long long* stats;
+    };
+    __device__ static void run(const Args& a, int, int)
+    {
+        if (a.flags[0] == 0u) return;
+        __shared__ int32_t wsum[16];
+        __shared__ int32_t carry;
+        const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+        if (t == 0) {
+            carry = 0;
+            a.stats[0] += 1ull;
+            a.stats[1] = 0ull;
+            a.stats[2] = 0ull;
+            a.stats[3] = 0ull;
+            a.stats[4] = 0ull;
+            a.stats[6] = 0ull;
         }
-        if (lane == 63) wsum[w] = incl;
         __syncthreads();
-        int woff = 0;
-        for (int k = 0; k < w; ++k) woff += wsum[k];
-        const int c0 = carry;
-        if (i < ncell) start[i] = c0 + woff + incl - v;
-        __syncthreads();
-        if (t == 1023) carry = c0 + woff + incl;
-        __syncthreads();
+        for (int base = 0; base < a.ncell; base += 1024) {
+            const int i = base + t;
+            const int v = i < a.ncell ? a.count[i] : 0;
+            int incl = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int u = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += u;
+            }
+            if (lane == 63) wsum[w] = incl;
+            __syncthreads();
+            int woff = 0;
+            for (int k = 0; k < w; ++k) woff += wsum[k];
+            const int c0 = carry;
+            if (i < a.ncell) a.start[i] = c0 + woff + incl - v;
+            __syncthreads();
+            if (t == 1023) carry = c0 + woff + incl;
+            __syncthreads();
+        }
+        if (t == 0) a.start[a.ncell] = carry;
     }
-    if (t == 0) start[ncell] = carry;
-}
+};
 
-__global__ __launch_bounds__(kBlock) void k_scatter(const int32_t* __restrict__ atom_cell,
-                                                    const int32_t* __restrict__ atom_rank,
-                                                    const int32_t* __restrict__ start,
-                                                    int32_t* __restrict__ perm, int n,
-                                                    const uint32_t* __restrict__ flags)
-{
-    if (flags[0] == 0u) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    perm[start[atom_cell[i]] + atom_rank[i]] = i;
-}
+struct Scatter {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const int32_t *atom_cell, *atom_rank, *start;
+        int32_t* perm;
+        int n;
+        const uint32_t* flags;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        if (a.flags[0] == 0u) return;
+        const int i = blk * kBlock + threadIdx.x;
+        if (i >= a.n) return;
+        a.perm[a.start[a.atom_cell[i]] + a.atom_rank[i]] = i;
+    }
+};
 
 // ---------------------------------------------------------------- per-cell sort
 // One wave per cell: order the cell's atoms by original id (rank sort in LDS)
@@ -151,41 +172,48 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int32_t* __restrict__ 
 // charges, types, cells and the inverse permutation.
 constexpr int kCellSortMax = 2048;
 
-__global__ __launch_bounds__(64) void k_cellsort(
-    int32_t* __restrict__ perm, int32_t* __restrict__ inv_perm, const int32_t* __restrict__ start,
-    int ncell, const double* __restrict__ xr, const double* __restrict__ yr,
-    const double* __restrict__ zr, const double* __restrict__ q, const int32_t* __restrict__ type,
-    double4* __restrict__ xs_ref, int32_t* __restrict__ scell, int32_t* __restrict__ ts,
-    const uint32_t* __restrict__ flags)
-{
-    if (flags[0] == 0u) return;
-    __shared__ int32_t buf[kCellSortMax];
-    const int c = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int s0 = start[c], cnt = start[c + 1] - s0;
-    const bool sortable = cnt <= kCellSortMax;
-    if (sortable) {
-        for (int k = lane; k < cnt; k += 64) buf[k] = perm[s0 + k];
-        __syncthreads();
-    }
-    for (int k = lane; k < cnt; k += 64) {
-        int v, r;
+struct CellSort {
+    static constexpr int kThreads = 64;
+    struct Args {
+        int32_t *perm, *inv_perm;
+        const int32_t* start;
+        const double *xr, *yr, *zr, *q;
+        const int32_t* type;
+        double4* xs_ref;
+        int32_t *scell, *ts;
+        const uint32_t* flags;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        if (a.flags[0] == 0u) return;
+        __shared__ int32_t buf[kCellSortMax];
+        const int c = blk;
+        const int lane = threadIdx.x;
+        const int s0 = a.start[c], cnt = a.start[c + 1] - s0;
+        const bool sortable = cnt <= kCellSortMax;
         if (sortable) {
-            v = buf[k];
-            r = 0;
-            for (int t = 0; t < cnt; ++t) r += (buf[t] < v) ? 1 : 0;
-        } else {
-            v = perm[s0 + k];
-            r = k;
+            for (int k = lane; k < cnt; k += 64) buf[k] = a.perm[s0 + k];
+            __syncthreads();
         }
-        const int s = s0 + r;
-        if (sortable) perm[s] = v;
-        inv_perm[v] = s;
-        xs_ref[s] = make_double4(xr[v], yr[v], zr[v], q[v]);
-        scell[s] = c;
-        ts[s] = type ? type[v] : 0;
+        for (int k = lane; k < cnt; k += 64) {
+            int v, r;
+            if (sortable) {
+                v = buf[k];
+                r = 0;
+                for (int t = 0; t < cnt; ++t) r += (buf[t] < v) ? 1 : 0;
+            } else {
+                v = a.perm[s0 + k];
+                r = k;
+            }
+            const int s = s0 + r;
+            if (sortable) a.perm[s] = v;
+            a.inv_perm[v] = s;
+            a.xs_ref[s] = make_double4(a.xr[v], a.yr[v], a.zr[v], a.q[v]);
+            a.scell[s] = c;
+            a.ts[s] = a.type ? a.type[v] : 0;
+        }
     }
-}
+};
 
 // ---------------------------------------------------------------- Verlet list
 // One workgroup per (home cell, part): the tile's reference positions are
@@ -271,36 +299,50 @@ __device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ l
     }
 }
 
-__global__ __launch_bounds__(kTileThreads) void k_list_tile(
-    const double4* __restrict__ xs_ref, const int32_t* __restrict__ cell_start, int split,
-    int3 dims, Box box, double rl2, double rin2, int cap, int stage_cap,
-    uint16_t* __restrict__ nlist, int2* __restrict__ ncount, const uint32_t* __restrict__ flags,
-    unsigned long long* __restrict__ stats, int32_t* __restrict__ big_tiles)
-{
-    if (flags[0] == 0u) return;
-    extern __shared__ double4 lds[];
-    __shared__ TileHdr h;
-    const int c = blockIdx.x / split, part = blockIdx.x % split;
-    tile_setup(c, dims, box, cell_start, h);
-    const int total = h.off[27];
-    const bool staged = total <= stage_cap;
-    if (!staged && part == 0 && threadIdx.x == 0)
-        big_tiles[atomicAdd(&stats[6], 1ull)] = c;  // served by the global-memory pair pass
-    if (staged) {
-        for (int u = threadIdx.x; u < total; u += kTileThreads) lds[u] = stage_ref(h, xs_ref, u);
+struct ListTile {
+    static constexpr int kThreads = kTileThreads;
+    struct Args {
+        const double4* xs_ref;
+        const int32_t* cell_start;
+        int split;
+        int3 dims;
+        Box box;
+        double rl2, rin2;
+        int cap, stage_cap;
+        uint16_t* nlist;
+        int2* ncount;
+        const uint32_t* flags;
+        unsigned long long* stats;
+        int32_t* big_tiles;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        if (a.flags[0] == 0u) return;
+        extern __shared__ double4 lds[];
+        __shared__ TileHdr h;
+        const int c = blk / a.split, part = blk % a.split;
+        tile_setup(c, a.dims, a.box, a.cell_start, h);
+        const int total = h.off[27];
+        const bool staged = total <= a.stage_cap;
+        if (!staged && part == 0 && threadIdx.x == 0)
+            a.big_tiles[atomicAdd(&a.stats[6], 1ull)] = c;  // served by the global-memory pair pass
+        if (staged) {
+            for (int u = threadIdx.x; u < total; u += kTileThreads)
+                lds[u] = stage_ref(h, a.xs_ref, u);
+        }
+        __syncthreads();
+        const int lane = threadIdx.x & 63;
+        const int wv = uniform(threadIdx.x >> 6);
+        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+        const int a0 = h0 + (hc * part) / a.split, a1 = h0 + (hc * (part + 1)) / a.split;
+        if (staged)
+            list_wave<true>(h, lds, a.xs_ref, a0 + wv, a1, h0, lane, a.rl2, a.rin2, a.cap,
+                            a.nlist, a.ncount, a.stats);
+        else
+            list_wave<false>(h, lds, a.xs_ref, a0 + wv, a1, h0, lane, a.rl2, a.rin2, a.cap,
+                             a.nlist, a.ncount, a.stats);
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wv = uniform(threadIdx.x >> 6);
-    const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
-    const int a0 = h0 + (hc * part) / split, a1 = h0 + (hc * (part + 1)) / split;
-    if (staged)
-        list_wave<true>(h, lds, xs_ref, a0 + wv, a1, h0, lane, rl2, rin2, cap, nlist, ncount,
-                        stats);
-    else
-        list_wave<false>(h, lds, xs_ref, a0 + wv, a1, h0, lane, rl2, rin2, cap, nlist, ncount,
-                         stats);
-}
+};
 
 // List statistics on demand: entries, LJ-range entries, longest row.
 __global__ __launch_bounds__(256) void k_list_stats(const int2* __restrict__ ncount, int n,
@@ -338,21 +380,28 @@ __global__ __launch_bounds__(256) void k_list_stats(const int2* __restrict__ nco
 // Current positions into the sorted packed array, unwrapped against the
 // reference positions (x_ref + min_image(x - x_ref)), so that the image of
 // every list pair stays the one seen at the build.  Consumes the rebuild flag.
-__global__ __launch_bounds__(kBlock) void k_gather(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ xr, const double* __restrict__ yr, const double* __restrict__ zr,
-    const double* __restrict__ q, const int32_t* __restrict__ inv_perm, int n, Box box,
-    double4* __restrict__ xs, uint32_t* __restrict__ flags)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) flags[0] = 0u;
-    if (i >= n) return;
-    const double a = xr[i], b = yr[i], c = zr[i];
-    const double dx = min_image(x[i] - a, box.L[0], box.invL[0]);
-    const double dy = min_image(y[i] - b, box.L[1], box.invL[1]);
-    const double dz = min_image(z[i] - c, box.L[2], box.invL[2]);
-    xs[inv_perm[i]] = make_double4(a + dx, b + dy, c + dz, q[i]);
-}
+struct Gather {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const double *x, *y, *z, *xr, *yr, *zr, *q;
+        const int32_t* inv_perm;
+        int n;
+        Box box;
+        double4* xs;
+        uint32_t* flags;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i == 0) a.flags[0] = 0u;
+        if (i >= a.n) return;
+        const double p = a.xr[i], b = a.yr[i], c = a.zr[i];
+        const double dx = min_image(a.x[i] - p, a.box.L[0], a.box.invL[0]);
+        const double dy = min_image(a.y[i] - b, a.box.L[1], a.box.invL[1]);
+        const double dz = min_image(a.z[i] - c, a.box.L[2], a.box.invL[2]);
+        a.xs[a.inv_perm[i]] = make_double4(p + dx, b + dy, c + dz, a.q[i]);
+    }
+};
 
 // ---------------------------------------------------------------- integrator
 constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sampler
@@ -362,83 +411,103 @@ constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sam
 // cell histogram for a possible rebuild).  The first blocks also draw this
 // step's k-vectors (the sampler of k_sample) so that no extra launch is needed.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_kick_drift_check(
-    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-    const T* __restrict__ fx, const T* __restrict__ fy, const T* __restrict__ fz,
-    int32_t* __restrict__ img, const double* __restrict__ mass, double dt, int n, Box box,
-    const double* __restrict__ xr, const double* __restrict__ yr, const double* __restrict__ zr,
-    double thr2, uint32_t* __restrict__ flags, int32_t* __restrict__ cell_count, int ncell,
-    SampleArgs sa)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncell) cell_count[i] = 0;
-    if (blockIdx.x < (unsigned)((sa.P + 63) / 64)) {
-        // the first ceil(P/64) blocks also draw this step's k-vectors (64 per
-        // block, one wave), with the CDF tables staged in LDS
-        __shared__ double cdf[kSamplerLds];
-        const int len = 2 * (sa.M.x + sa.M.y + sa.M.z) + 3;
-        const bool in_lds = len <= kSamplerLds;
-        if (in_lds)
-            for (int k = threadIdx.x; k < len; k += blockDim.x) cdf[k] = sa.cdf[k];
-        __syncthreads();
-        const int l = blockIdx.x * 64 + threadIdx.x;
-        if (threadIdx.x < 64 && l < sa.P) {
-            const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
-            const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
-            int m[3];
-            const double w = draw_kvector(sa.seed, step, (uint32_t)l, in_lds ? cdf : sa.cdf, Ms,
-                                          sa.gp, sa.P, sa.w, sa.w_out, m);
-            sa.km[3 * l] = m[0];
-            sa.km[3 * l + 1] = m[1];
-            sa.km[3 * l + 2] = m[2];
-            sa.kw[l] = w;
+struct KickDrift {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        double *x, *y, *z, *vx, *vy, *vz;
+        const T *fx, *fy, *fz;
+        int32_t* img;
+        const double* mass;
+        double dt;
+        int n;
+        Box box;
+        const double *xr, *yr, *zr;
+        double thr2;
+        uint32_t* flags;
+        int32_t* cell_count;
+        int ncell;
+        SampleArgs sa;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const SampleArgs& sa = a.sa;
+        const int i = blk * kBlock + threadIdx.x;
+        if (i < a.ncell) a.cell_count[i] = 0;
+        if (blk < (sa.P + 63) / 64) {
+            // the first ceil(P/64) blocks also draw this step's k-vectors (64 per
+            // block, one wave), with the CDF tables staged in LDS
+            __shared__ double cdf[kSamplerLds];
+            const int len = 2 * (sa.M.x + sa.M.y + sa.M.z) + 3;
+            const bool in_lds = len <= kSamplerLds;
+            if (in_lds)
+                for (int k = threadIdx.x; k < len; k += kBlock) cdf[k] = sa.cdf[k];
+            __syncthreads();
+            const int l = blk * 64 + threadIdx.x;
+            if (threadIdx.x < 64 && l < sa.P) {
+                const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
+                const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
+                int m[3];
+                const double w = draw_kvector(sa.seed, step, (uint32_t)l, in_lds ? cdf : sa.cdf,
+                                              Ms, sa.gp, sa.P, sa.w, sa.w_out, m);
+                sa.km[3 * l] = m[0];
+                sa.km[3 * l + 1] = m[1];
+                sa.km[3 * l + 2] = m[2];
+                sa.kw[l] = w;
+            }
         }
+        if (i >= a.n) return;
+        const Box& box = a.box;
+        const double dt = a.dt;
+        const double h = 0.5 * dt / (a.mass ? a.mass[i] : 1.0);
+        double v[3] = {a.vx[i] + h * (double)a.fx[i], a.vy[i] + h * (double)a.fy[i],
+                       a.vz[i] + h * (double)a.fz[i]};
+        double p[3] = {a.x[i] + v[0] * dt, a.y[i] + v[1] * dt, a.z[i] + v[2] * dt};
+        int fl[3];
+        for (int d = 0; d < 3; ++d) {
+            double f = floor((p[d] - box.lo[d]) * box.invL[d]);
+            p[d] -= f * box.L[d];
+            fl[d] = (int)f;
+        }
+        a.vx[i] = v[0];
+        a.vy[i] = v[1];
+        a.vz[i] = v[2];
+        a.x[i] = p[0];
+        a.y[i] = p[1];
+        a.z[i] = p[2];
+        if (a.img) {
+            a.img[3 * i] += fl[0];
+            a.img[3 * i + 1] += fl[1];
+            a.img[3 * i + 2] += fl[2];
+        }
+        const double dx = min_image(p[0] - a.xr[i], box.L[0], box.invL[0]);
+        const double dy = min_image(p[1] - a.yr[i], box.L[1], box.invL[1]);
+        const double dz = min_image(p[2] - a.zr[i], box.L[2], box.invL[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > a.thr2) a.flags[0] = 1u;
+        if (d2 != d2) a.flags[1] = 1u;
     }
-    if (i >= n) return;
-    const double h = 0.5 * dt / (mass ? mass[i] : 1.0);
-    double v[3] = {vx[i] + h * (double)fx[i], vy[i] + h * (double)fy[i], vz[i] + h * (double)fz[i]};
-    double p[3] = {x[i] + v[0] * dt, y[i] + v[1] * dt, z[i] + v[2] * dt};
-    int fl[3];
-    for (int d = 0; d < 3; ++d) {
-        double f = floor((p[d] - box.lo[d]) * box.invL[d]);
-        p[d] -= f * box.L[d];
-        fl[d] = (int)f;
-    }
-    vx[i] = v[0];
-    vy[i] = v[1];
-    vz[i] = v[2];
-    x[i] = p[0];
-    y[i] = p[1];
-    z[i] = p[2];
-    if (img) {
-        img[3 * i] += fl[0];
-        img[3 * i + 1] += fl[1];
-        img[3 * i + 2] += fl[2];
-    }
-    const double dx = min_image(p[0] - xr[i], box.L[0], box.invL[0]);
-    const double dy = min_image(p[1] - yr[i], box.L[1], box.invL[1]);
-    const double dz = min_image(p[2] - zr[i], box.L[2], box.invL[2]);
-    const double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 > thr2) flags[0] = 1u;
-    if (d2 != d2) flags[1] = 1u;
-}
+};
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_kick(double* __restrict__ vx, double* __restrict__ vy,
-                                                 double* __restrict__ vz,
-                                                 const T* __restrict__ fx,
-                                                 const T* __restrict__ fy,
-                                                 const T* __restrict__ fz,
-                                                 const double* __restrict__ mass, double dt, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double h = 0.5 * dt / (mass ? mass[i] : 1.0);
-    vx[i] += h * (double)fx[i];
-    vy[i] += h * (double)fy[i];
-    vz[i] += h * (double)fz[i];
-}
+struct Kick {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        double *vx, *vy, *vz;
+        const T *fx, *fy, *fz;
+        const double* mass;
+        double dt;
+        int n;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i >= a.n) return;
+        const double h = 0.5 * a.dt / (a.mass ? a.mass[i] : 1.0);
+        a.vx[i] += h * (double)a.fx[i];
+        a.vy[i] += h * (double)a.fy[i];
+        a.vz[i] += h * (double)a.fz[i];
+    }
+};
 
 __global__ __launch_bounds__(kBlock) void k_copy(uint4* __restrict__ dst,
                                                  const uint4* __restrict__ src, int64_t n16)
@@ -450,17 +519,85 @@ __global__ __launch_bounds__(kBlock) void k_copy(uint4* __restrict__ dst,
 
 inline int nblk(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
+// ---------------------------------------------------------------- argument packing
+Check::Args check_args(Engine& e, const double* x, const double* y, const double* z)
+{
+    const double h = 0.5 * e.skin;
+    return Check::Args{x, y, z, e.xr, e.yr, e.zr, (int)e.n, e.box, h * h, e.flags,
+                       e.cell_count, e.ncell};
+}
+
+Bin::Args bin_args(Engine& e, const double* x, const double* y, const double* z)
+{
+    return Bin::Args{x, y, z, e.xr, e.yr, e.zr, (int)e.n, e.box,
+                     make_int3(e.dims[0], e.dims[1], e.dims[2]),
+                     make_double3(e.cell[0], e.cell[1], e.cell[2]), e.cell_count, e.atom_cell,
+                     e.atom_rank, e.flags};
+}
+
+Scan::Args scan_args(Engine& e) { return Scan::Args{e.cell_count, e.cell_start, e.ncell, e.flags, e.stats}; }
+
+Scatter::Args scatter_args(Engine& e)
+{
+    return Scatter::Args{e.atom_cell, e.atom_rank, e.cell_start, e.perm, (int)e.n, e.flags};
+}
+
+CellSort::Args cellsort_args(Engine& e)
+{
+    return CellSort::Args{e.perm,   e.atom_rank /* reused as inverse perm */, e.cell_start, e.xr,
+                          e.yr,     e.zr,  e.q,     e.type, e.xs_ref, e.scell, e.ts, e.flags};
+}
+
+ListTile::Args list_args(Engine& e)
+{
+    return ListTile::Args{e.xs_ref,   e.cell_start, e.split,   make_int3(e.dims[0], e.dims[1], e.dims[2]),
+                          e.box,      e.r_list * e.r_list, e.r_in * e.r_in, e.cap, e.stage_cap,
+                          e.nlist,    e.ncount, e.flags, e.stats, e.big_tiles};
+}
+
+Gather::Args gather_args(Engine& e, const double* x, const double* y, const double* z)
+{
+    return Gather::Args{x, y, z, e.xr, e.yr, e.zr, e.q, e.atom_rank, (int)e.n, e.box, e.xs,
+                        e.flags};
+}
+
+template <typename T>
+typename KickDrift<T>::Args kickdrift_args(Engine& e, double* x, double* y, double* z,
+                                           double* vx, double* vy, double* vz, const void* fx,
+                                           const void* fy, const void* fz, int32_t* img,
+                                           const double* mass, double dt, const SampleArgs& sa)
+{
+    const double h = 0.5 * e.skin;
+    return typename KickDrift<T>::Args{x,  y,    z,    vx,   vy,  vz, (const T*)fx, (const T*)fy,
+                                       (const T*)fz, img, mass, dt, (int)e.n, e.box, e.xr,
+                                       e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell, sa};
+}
+
+int kickdrift_blocks(const Engine& e, const SampleArgs& sa)
+{
+    int64_t cover = e.n > e.ncell ? e.n : e.ncell;
+    if (cover < (int64_t)sa.P * kBlock / 64) cover = (int64_t)sa.P * kBlock / 64;
+    if (cover < 1) cover = 1;
+    return nblk(cover);
+}
+
+template <typename T>
+typename Kick<T>::Args kick_args(Engine& e, double* vx, double* vy, double* vz, const void* fx,
+                                 const void* fy, const void* fz, const double* mass, double dt)
+{
+    return typename Kick<T>::Args{vx, vy, vz, (const T*)fx, (const T*)fy, (const T*)fz, mass, dt,
+                                  (int)e.n};
+}
+
 }  // namespace
 
 // ================================================================= launchers
 
 void launch_check(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
 {
-    const double h = 0.5 * e.skin;
     ScopedTimer t(e, P_CHECK, s);
     int64_t cover = e.n > e.ncell ? e.n : e.ncell;
-    k_check<<<nblk(cover), kBlock, 0, s>>>(x, y, z, e.xr, e.yr, e.zr, (int)e.n, e.box, h * h,
-                                            e.flags, e.cell_count, e.ncell);
+    k_one<Check><<<nblk(cover), kBlock, 0, s>>>(check_args(e, x, y, z));
 }
 
 void launch_force_rebuild(Engine& e, hipStream_t s) { k_set_flag<<<1, 1, 0, s>>>(e.flags); }
@@ -468,35 +605,26 @@ void launch_force_rebuild(Engine& e, hipStream_t s) { k_set_flag<<<1, 1, 0, s>>>
 void launch_build(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
 {
     const int n = (int)e.n;
-    const int3 dims = make_int3(e.dims[0], e.dims[1], e.dims[2]);
-    const double3 cell = make_double3(e.cell[0], e.cell[1], e.cell[2]);
     {
         ScopedTimer t(e, P_BIN, s);
-        k_bin<<<nblk(n), kBlock, 0, s>>>(x, y, z, e.xr, e.yr, e.zr, n, e.box, dims, cell,
-                                          e.cell_count, e.atom_cell, e.atom_rank, e.flags);
+        k_one<Bin><<<nblk(n), kBlock, 0, s>>>(bin_args(e, x, y, z));
     }
     {
         ScopedTimer t(e, P_SCAN, s);
-        k_scan<<<1, 1024, 0, s>>>(e.cell_count, e.cell_start, e.ncell, e.flags, e.stats);
+        k_one<Scan><<<1, 1024, 0, s>>>(scan_args(e));
     }
     {
         ScopedTimer t(e, P_SCATTER, s);
-        k_scatter<<<nblk(n), kBlock, 0, s>>>(e.atom_cell, e.atom_rank, e.cell_start, e.perm, n,
-                                              e.flags);
+        k_one<Scatter><<<nblk(n), kBlock, 0, s>>>(scatter_args(e));
     }
     {
         ScopedTimer t(e, P_CELLSORT, s);
-        k_cellsort<<<e.ncell, 64, 0, s>>>(e.perm, e.atom_rank /* reused as inverse perm */,
-                                           e.cell_start, e.ncell, e.xr, e.yr, e.zr, e.q, e.type,
-                                           e.xs_ref, e.scell, e.ts, e.flags);
+        k_one<CellSort><<<e.ncell, 64, 0, s>>>(cellsort_args(e));
     }
     {
-        const double rl2 = e.r_list * e.r_list;
-        const double rin2 = e.r_in * e.r_in;
         ScopedTimer t(e, P_LIST, s);
-        k_list_tile<<<e.ncell * e.split, kTileThreads, (size_t)e.stage_cap * sizeof(double4), s>>>(
-            e.xs_ref, e.cell_start, e.split, dims, e.box, rl2, rin2, e.cap, e.stage_cap, e.nlist,
-            e.ncount, e.flags, e.stats, e.big_tiles);
+        k_one<ListTile><<<e.ncell * e.split, kTileThreads, (size_t)e.stage_cap * sizeof(double4),
+                          s>>>(list_args(e));
     }
 }
 
@@ -509,8 +637,7 @@ void launch_gather(Engine& e, const double* x, const double* y, const double* z,
 {
     ScopedTimer t(e, P_GATHER, s);
     const int n = (int)e.n;
-    k_gather<<<nblk(n > 0 ? n : 1), kBlock, 0, s>>>(x, y, z, e.xr, e.yr, e.zr, e.q,
-                                                     e.atom_rank, n, e.box, e.xs, e.flags);
+    k_one<Gather><<<nblk(n > 0 ? n : 1), kBlock, 0, s>>>(gather_args(e, x, y, z));
 }
 
 void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double* vx, double* vy,
@@ -518,20 +645,14 @@ void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double*
                              int32_t* img, const double* mass, double dt, const SampleArgs& sa,
                              hipStream_t s)
 {
-    const double h = 0.5 * e.skin;
     ScopedTimer t(e, P_CHECK, s);
-    int64_t cover = e.n > e.ncell ? e.n : e.ncell;
-    if (cover < (int64_t)sa.P * kBlock / 64) cover = (int64_t)sa.P * kBlock / 64;
-    if (cover < 1) cover = 1;
+    const int blocks = kickdrift_blocks(e, sa);
     if (e.precision == 32)
-        k_kick_drift_check<float><<<nblk(cover), kBlock, 0, s>>>(
-            x, y, z, vx, vy, vz, (const float*)fx, (const float*)fy, (const float*)fz, img, mass,
-            dt, (int)e.n, e.box, e.xr, e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell, sa);
+        k_one<KickDrift<float>><<<blocks, kBlock, 0, s>>>(
+            kickdrift_args<float>(e, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa));
     else
-        k_kick_drift_check<double><<<nblk(cover), kBlock, 0, s>>>(
-            x, y, z, vx, vy, vz, (const double*)fx, (const double*)fy, (const double*)fz, img,
-            mass, dt, (int)e.n, e.box, e.xr, e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell,
-            sa);
+        k_one<KickDrift<double>><<<blocks, kBlock, 0, s>>>(
+            kickdrift_args<double>(e, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa));
 }
 
 void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,
@@ -539,13 +660,11 @@ void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, 
 {
     ScopedTimer t(e, P_KICK, s);
     if (e.precision == 32)
-        k_kick<float><<<nblk(e.n), kBlock, 0, s>>>(vx, vy, vz, (const float*)fx,
-                                                    (const float*)fy, (const float*)fz, mass, dt,
-                                                    (int)e.n);
+        k_one<Kick<float>><<<nblk(e.n), kBlock, 0, s>>>(
+            kick_args<float>(e, vx, vy, vz, fx, fy, fz, mass, dt));
     else
-        k_kick<double><<<nblk(e.n), kBlock, 0, s>>>(vx, vy, vz, (const double*)fx,
-                                                     (const double*)fy, (const double*)fz, mass,
-                                                     dt, (int)e.n);
+        k_one<Kick<double>><<<nblk(e.n), kBlock, 0, s>>>(
+            kick_args<double>(e, vx, vy, vz, fx, fy, fz, mass, dt));
 }
 
 void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s)
@@ -555,6 +674,80 @@ void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s)
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
     k_copy<<<blocks, kBlock, 0, s>>>((uint4*)dst, (const uint4*)src, n16);
+}
+
+// ================================================================= batched phases
+
+template <typename T>
+static hipError_t nlist_prepare_t(Batch& b)
+{
+    const size_t R = b.es.size();
+    std::vector<typename KickDrift<T>::Args> kd(R);
+    std::vector<Bin::Args> bi(R);
+    std::vector<Scan::Args> sc(R);
+    std::vector<Scatter::Args> sp(R);
+    std::vector<CellSort::Args> cs(R);
+    std::vector<ListTile::Args> li(R);
+    std::vector<Gather::Args> ga(R);
+    std::vector<typename Kick<T>::Args> ki(R);
+    std::vector<int> nkd(R), nat(R), none(R, 1), ncell(R), nlist(R);
+    size_t lds = 0;
+    for (size_t r = 0; r < R; ++r) {
+        Engine& e = *b.es[r];
+        const RepState& st = b.st[r];
+        kd[r] = kickdrift_args<T>(e, st.x, st.y, st.z, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz,
+                                  st.img, st.mass, b.dt, b.sa[r]);
+        bi[r] = bin_args(e, st.x, st.y, st.z);
+        sc[r] = scan_args(e);
+        sp[r] = scatter_args(e);
+        cs[r] = cellsort_args(e);
+        li[r] = list_args(e);
+        ga[r] = gather_args(e, st.x, st.y, st.z);
+        ki[r] = kick_args<T>(e, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz, st.mass, b.dt);
+        nkd[r] = kickdrift_blocks(e, b.sa[r]);
+        nat[r] = nblk(e.n > 0 ? e.n : 1);
+        ncell[r] = e.ncell;
+        nlist[r] = e.ncell * e.split;
+        const size_t l = (size_t)e.stage_cap * sizeof(double4);
+        lds = l > lds ? l : lds;
+    }
+    hipError_t err;
+    if ((err = phase_upload(b.ph[BP_KICKDRIFT], kd, nkd, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_BIN], bi, nat, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_SCAN], sc, none, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_SCATTER], sp, nat, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_CELLSORT], cs, ncell, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_LIST], li, nlist, lds)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_GATHER], ga, nat, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_KICK], ki, nat, 0)) != hipSuccess) return err;
+    return hipSuccess;
+}
+
+hipError_t nlist_batch_prepare(Batch& b)
+{
+    return b.precision == 32 ? nlist_prepare_t<float>(b) : nlist_prepare_t<double>(b);
+}
+
+void nlist_batch_launch(Batch& b, int phase, hipStream_t s)
+{
+    const bool f32 = b.precision == 32;
+    switch (phase) {
+    case BP_KICKDRIFT:
+        if (f32) phase_launch<KickDrift<float>>(b.ph[phase], s);
+        else phase_launch<KickDrift<double>>(b.ph[phase], s);
+        break;
+    case BP_BIN: phase_launch<Bin>(b.ph[phase], s); break;
+    case BP_SCAN: phase_launch<Scan>(b.ph[phase], s); break;
+    case BP_SCATTER: phase_launch<Scatter>(b.ph[phase], s); break;
+    case BP_CELLSORT: phase_launch<CellSort>(b.ph[phase], s); break;
+    case BP_LIST: phase_launch<ListTile>(b.ph[phase], s); break;
+    case BP_GATHER: phase_launch<Gather>(b.ph[phase], s); break;
+    case BP_KICK:
+        if (f32) phase_launch<Kick<float>>(b.ph[phase], s);
+        else phase_launch<Kick<double>>(b.ph[phase], s);
+        break;
+    default: break;
+    }
 }
 
 }  // namespace rbmd

@@ -162,7 +162,6 @@ __device__ inline int tile_type(const TileHdr& h, const int32_t* __restrict__ lt
 }
 
 constexpr int kRound = 2;    // 16-entry chunks per atom per round
-constexpr int kBigGrid = 64; // workgroups of the big-tile pass
 
 struct PairArgs {
     const double4* xs;
@@ -315,65 +314,89 @@ __device__ inline void block_ev(const double* acc, int lane, int wv, double* __r
 }
 
 // Staged pass: one workgroup per (tile, part); tiles larger than the LDS stage
-// are skipped here and served by k_pair_big.
+// are skipped here and served by PairBig.
 template <typename T, bool EV, bool MULTI>
-__global__ __launch_bounds__(kTileThreads) void k_pair_tile(
-    PairArgs a, const int32_t* __restrict__ cell_start, int split, int3 dims, Box box,
-    int stage_cap, PairScalars ps, const PairParams* __restrict__ pp, T* __restrict__ fx,
-    T* __restrict__ fy, T* __restrict__ fz, double* __restrict__ ev_part)
-{
-    extern __shared__ double4 lds[];
-    int32_t* ltype = (int32_t*)(lds + stage_cap);
-    __shared__ TileHdr h;
-    const int c = blockIdx.x / split, part = blockIdx.x % split;
-    tile_setup(c, dims, box, cell_start, h);
-    const int total = h.off[27];
-    const bool mine = total <= stage_cap;
-    if (mine) {
-        for (int u = threadIdx.x; u < total; u += kTileThreads) {
-            const int st = tile_seg(h, u);
-            const int j = h.gstart[st] + (u - h.off[st]);
-            const double4 p = a.xs[j];
-            lds[u] = make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
-            if (MULTI) ltype[u] = a.ts[j];
+struct PairTile {
+    static constexpr int kThreads = kTileThreads;
+    struct Args {
+        PairArgs a;
+        const int32_t* cell_start;
+        int split;
+        int3 dims;
+        Box box;
+        int stage_cap;
+        PairScalars ps;
+        const PairParams* pp;
+        T *fx, *fy, *fz;
+        double* ev_part;
+    };
+    __device__ static void run(const Args& A, int blk, int)
+    {
+        const PairArgs& a = A.a;
+        extern __shared__ double4 lds[];
+        int32_t* ltype = (int32_t*)(lds + A.stage_cap);
+        __shared__ TileHdr h;
+        const int c = blk / A.split, part = blk % A.split;
+        tile_setup(c, A.dims, A.box, A.cell_start, h);
+        const int total = h.off[27];
+        const bool mine = total <= A.stage_cap;
+        if (mine) {
+            for (int u = threadIdx.x; u < total; u += kTileThreads) {
+                const int st = tile_seg(h, u);
+                const int j = h.gstart[st] + (u - h.off[st]);
+                const double4 p = a.xs[j];
+                lds[u] = make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+                if (MULTI) ltype[u] = a.ts[j];
+            }
         }
+        __syncthreads();
+        const int lane = threadIdx.x & 63;
+        const int wv = uniform(threadIdx.x >> 6);
+        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+        const int a0 = h0 + (hc * part) / A.split, a1 = h0 + (hc * (part + 1)) / A.split;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (mine)
+            pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
+                                          A.fx, A.fy, A.fz, acc);
+        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wv = uniform(threadIdx.x >> 6);
-    const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
-    const int a0 = h0 + (hc * part) / split, a1 = h0 + (hc * (part + 1)) / split;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (mine)
-        pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, ps, pp, fx, fy, fz,
-                                      acc);
-    if (EV) block_ev(acc, lane, wv, ev_part + (size_t)blockIdx.x * 8);
-}
+};
 
 // Global-memory pass over the tiles the list build found larger than the LDS
 // stage (big_tiles[0 .. stats[6])), a fixed small grid looping over them.
 template <typename T, bool EV, bool MULTI>
-__global__ __launch_bounds__(kTileThreads) void k_pair_big(
-    PairArgs a, const int32_t* __restrict__ cell_start, const int32_t* __restrict__ big_tiles,
-    const unsigned long long* __restrict__ stats, int split, int3 dims, Box box, PairScalars ps,
-    const PairParams* __restrict__ pp, T* __restrict__ fx, T* __restrict__ fy,
-    T* __restrict__ fz, double* __restrict__ ev_part)
-{
-    __shared__ TileHdr h;
-    const int nbig = (int)stats[6];
-    const int lane = threadIdx.x & 63;
-    const int wv = uniform(threadIdx.x >> 6);
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
-        const int c = big_tiles[b];
-        __syncthreads();
-        tile_setup(c, dims, box, cell_start, h);
-        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
-        pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, h0, h0 + hc, h0, wv, lane, ps, pp,
-                                       fx, fy, fz, acc);
+struct PairBig {
+    static constexpr int kThreads = kTileThreads;
+    struct Args {
+        PairArgs a;
+        const int32_t* cell_start;
+        const int32_t* big_tiles;
+        const unsigned long long* stats;
+        int3 dims;
+        Box box;
+        PairScalars ps;
+        const PairParams* pp;
+        T *fx, *fy, *fz;
+        double* ev_part;
+    };
+    __device__ static void run(const Args& A, int blk, int nblk)
+    {
+        __shared__ TileHdr h;
+        const int nbig = (int)A.stats[6];
+        const int lane = threadIdx.x & 63;
+        const int wv = uniform(threadIdx.x >> 6);
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int b = blk; b < nbig; b += nblk) {
+            const int c = A.big_tiles[b];
+            __syncthreads();
+            tile_setup(c, A.dims, A.box, A.cell_start, h);
+            const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+            pair_wave<T, EV, MULTI, false>(A.a, h, nullptr, nullptr, h0, h0 + hc, h0, wv, lane,
+                                           A.ps, A.pp, A.fx, A.fy, A.fz, acc);
+        }
+        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
     }
-    if (EV) block_ev(acc, lane, wv, ev_part + (size_t)blockIdx.x * 8);
-}
+};
 
 // Deterministic final reduction: pair partials [nb][8] + per-k terms [P][7]
 // + self energy -> ev[10] = {e_lj, e_real, e_self, e_k, W_xx..W_yz}.
@@ -412,12 +435,8 @@ __global__ __launch_bounds__(256) void k_ev_final(const double* __restrict__ ev_
     }
 }
 
-}  // namespace
-
-void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s)
+PairScalars pair_scalars(const Engine& e)
 {
-    const int n = (int)e.n;
-    if (n == 0) return;
     PairScalars ps;
     ps.rcl2 = e.pp.rcl2;
     ps.rcc2 = e.pp.rcc2;
@@ -428,25 +447,74 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
     ps.eps24 = e.pp.eps24[0];
     ps.sig2 = e.pp.sig2[0];
     ps.ntypes = e.pp.ntypes;
-    const int3 dims = make_int3(e.dims[0], e.dims[1], e.dims[2]);
-    const int blocks = e.ncell * e.split;
-    const size_t lds = (size_t)e.stage_cap * (sizeof(double4) + (ps.ntypes > 1 ? 4 : 0));
+    return ps;
+}
+
+template <typename T, bool EV, bool M>
+typename PairTile<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void* fz)
+{
+    return typename PairTile<T, EV, M>::Args{
+        PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
+        e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box, e.stage_cap,
+        pair_scalars(e), e.d_pp, (T*)fx, (T*)fy, (T*)fz, e.ev_part};
+}
+
+template <typename T, bool EV, bool M>
+typename PairBig<T, EV, M>::Args big_args(Engine& e, void* fx, void* fy, void* fz)
+{
+    return typename PairBig<T, EV, M>::Args{
+        PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
+        e.cell_start, e.big_tiles, e.stats, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box,
+        pair_scalars(e), e.d_pp, (T*)fx, (T*)fy, (T*)fz,
+        e.ev_part + (size_t)e.ncell * e.split * 8};
+}
+
+size_t pair_lds(const Engine& e)
+{
+    return (size_t)e.stage_cap * (sizeof(double4) + (e.pp.ntypes > 1 ? 4 : 0));
+}
+
+template <typename T, bool EV, bool M>
+void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    k_one<PairTile<T, EV, M>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
+        tile_args<T, EV, M>(e, fx, fy, fz));
+    k_one<PairBig<T, EV, M>><<<e.big_grid, kTileThreads, 0, s>>>(big_args<T, EV, M>(e, fx, fy, fz));
+}
+
+template <typename T, bool M>
+hipError_t pair_prepare_t(Batch& b)
+{
+    const size_t R = b.es.size();
+    std::vector<typename PairTile<T, false, M>::Args> ta(R);
+    std::vector<typename PairBig<T, false, M>::Args> ba(R);
+    std::vector<int> nt(R), nb(R);
+    size_t lds = 0;
+    for (size_t r = 0; r < R; ++r) {
+        Engine& e = *b.es[r];
+        ta[r] = tile_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz);
+        ba[r] = big_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz);
+        nt[r] = e.ncell * e.split;
+        nb[r] = e.big_grid;
+        const size_t l = pair_lds(e);
+        lds = l > lds ? l : lds;
+    }
+    hipError_t err;
+    if ((err = phase_upload(b.ph[BP_PAIR], ta, nt, lds)) != hipSuccess) return err;
+    return phase_upload(b.ph[BP_PAIR_BIG], ba, nb, 0);
+}
+
+}  // namespace
+
+void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s)
+{
+    if (e.n == 0) return;
     ScopedTimer t(e, P_PAIR, s);
-    PairArgs a{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap};
-    const bool multi = ps.ntypes > 1;
-#define RBMD_PAIR(T, EV, M)                                                                   \
-    do {                                                                                      \
-        k_pair_tile<T, EV, M><<<blocks, kTileThreads, lds, s>>>(                              \
-            a, e.cell_start, e.split, dims, e.box, e.stage_cap, ps, e.d_pp, (T*)fx, (T*)fy,   \
-            (T*)fz, e.ev_part);                                                               \
-        k_pair_big<T, EV, M><<<kBigGrid, kTileThreads, 0, s>>>(                               \
-            a, e.cell_start, e.big_tiles, e.stats, e.split, dims, e.box, ps, e.d_pp, (T*)fx,  \
-            (T*)fy, (T*)fz, e.ev_part + (size_t)blocks * 8);                                  \
-    } while (0)
+    const bool multi = e.pp.ntypes > 1;
 #define RBMD_PAIR_M(T, EV)                                                                    \
     do {                                                                                      \
-        if (multi) RBMD_PAIR(T, EV, true);                                                    \
-        else RBMD_PAIR(T, EV, false);                                                         \
+        if (multi) pair_launch_t<T, EV, true>(e, fx, fy, fz, s);                              \
+        else pair_launch_t<T, EV, false>(e, fx, fy, fz, s);                                   \
     } while (0)
     if (e.precision == 32) {
         if (ev) RBMD_PAIR_M(float, true);
@@ -456,15 +524,37 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
         else RBMD_PAIR_M(double, false);
     }
 #undef RBMD_PAIR_M
-#undef RBMD_PAIR
 }
 
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s)
 {
     ScopedTimer t(e, P_EV_FINAL, s);
-    const int nb = with_pair ? e.ncell * e.split + kBigGrid : 0;  // staged + big-tile kernels
+    const int nb = with_pair ? e.ncell * e.split + e.big_grid : 0;  // staged + big-tile kernels
     const int P = (e.kspace != 0) ? e.P : 0;
     k_ev_final<<<1, 256, 0, s>>>(e.ev_part, nb, e.kev, P, e.e_self, e.ev);
+}
+
+hipError_t pair_batch_prepare(Batch& b)
+{
+    if (b.precision == 32) return b.multi ? pair_prepare_t<float, true>(b) : pair_prepare_t<float, false>(b);
+    return b.multi ? pair_prepare_t<double, true>(b) : pair_prepare_t<double, false>(b);
+}
+
+void pair_batch_launch(Batch& b, int phase, hipStream_t s)
+{
+#define RBMD_PB(T, M)                                                                         \
+    do {                                                                                      \
+        if (phase == BP_PAIR) phase_launch<PairTile<T, false, M>>(b.ph[phase], s);           \
+        else if (phase == BP_PAIR_BIG) phase_launch<PairBig<T, false, M>>(b.ph[phase], s);   \
+    } while (0)
+    if (b.precision == 32) {
+        if (b.multi) RBMD_PB(float, true);
+        else RBMD_PB(float, false);
+    } else {
+        if (b.multi) RBMD_PB(double, true);
+        else RBMD_PB(double, false);
+    }
+#undef RBMD_PB
 }
 
 }  // namespace rbmd

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "launch.h"
 
 namespace rbmd {
 
@@ -99,6 +100,7 @@ struct Engine {
 
     // energy / virial
     double* ev_part = nullptr;  // [blocks][8]
+    int big_grid = 64;          // workgroups of the big-tile pair pass
     int ev_part_blocks = 0;
     double* ev = nullptr;       // [10]
 
@@ -126,6 +128,53 @@ struct Engine {
 
     std::string err;
 };
+
+// ---- batched replicas: device pointers of one replica's MD state
+struct RepState {
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    void *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    int32_t* img = nullptr;
+    const double* mass = nullptr;
+};
+
+// Phases of the batched MD step, one launch each over all replicas.
+enum BatchPh {
+    BP_KICKDRIFT = 0,
+    BP_BIN,
+    BP_SCAN,
+    BP_SCATTER,
+    BP_CELLSORT,
+    BP_LIST,
+    BP_GATHER,
+    BP_RHO,
+    BP_RHO_FINAL,
+    BP_PAIR,
+    BP_PAIR_BIG,
+    BP_KFORCE,
+    BP_KICK,
+    BP_NUM
+};
+
+struct Batch {
+    std::vector<Engine*> es;
+    std::vector<RepState> st;
+    std::vector<SampleArgs> sa;  // per replica, device step counters
+    double dt = 0;
+    int precision = 32;
+    int kspace = 0;
+    bool multi = false;
+    BatchPhase ph[BP_NUM];
+};
+
+// Batched phases: prepare packs and uploads every replica's arguments (host,
+// synchronous, outside graph capture); launch issues one grid per phase.
+hipError_t nlist_batch_prepare(Batch& b);
+void nlist_batch_launch(Batch& b, int phase, hipStream_t s);
+hipError_t pair_batch_prepare(Batch& b);
+void pair_batch_launch(Batch& b, int phase, hipStream_t s);
+hipError_t kspace_batch_prepare(Batch& b);
+void kspace_batch_launch(Batch& b, int phase, hipStream_t s);
 
 // ---- launchers (nlist.hip)
 void launch_check(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);

@@ -73,9 +73,28 @@ class ForceConfig:
 class LJCutCoulKspaceForce:
     _KS = {"NONE": L.KSPACE_NONE, "RBE": L.KSPACE_RBE, "EWALD": L.KSPACE_EWALD}
 
-    def __init__(self, cfg: ForceConfig):
+    def __init__(self, cfg: ForceConfig, _borrowed=None):
         self.cfg = cfg
         self.lib = L.lib()
+        self.n = 0
+        self.fdtype = torch.float32 if cfg.precision == 32 else torch.float64
+        if _borrowed is not None:
+            # a replica handle owned by a ReplicaBatch
+            self.h = _borrowed
+            self._owned = False
+            return
+        self._owned = True
+        p, self._keep = self.params(cfg)
+        h = C.c_void_p()
+        rc = self.lib.rbmd_force_create(C.byref(p), int(cfg.n_max), C.byref(h))
+        if rc != 0:
+            raise RBMDError(f"rbmd_force_create -> {rc}: "
+                            f"{self.lib.rbmd_force_last_error(None).decode()}")
+        self.h = h
+
+    @classmethod
+    def params(cls, cfg: ForceConfig):
+        """(rbmd_force_params, arrays it points to) for a configuration."""
         p = L.ForceParams()
         lo = [float(v) for v in cfg.box_lo]
         ln = [float(v) for v in cfg.box_len]
@@ -86,30 +105,24 @@ class LJCutCoulKspaceForce:
         p.skin = cfg.skin
         p.alpha = cfg.alpha
         p.qqr2e = cfg.qqr2e
-        self._eps = (C.c_double * len(cfg.eps))(*cfg.eps)
-        self._sig = (C.c_double * len(cfg.sigma))(*cfg.sigma)
+        eps = (C.c_double * len(cfg.eps))(*cfg.eps)
+        sig = (C.c_double * len(cfg.sigma))(*cfg.sigma)
         p.ntypes = len(cfg.eps)
-        p.eps = C.cast(self._eps, C.POINTER(C.c_double))
-        p.sigma = C.cast(self._sig, C.POINTER(C.c_double))
+        p.eps = C.cast(eps, C.POINTER(C.c_double))
+        p.sigma = C.cast(sig, C.POINTER(C.c_double))
         p.precision = cfg.precision
-        p.kspace = self._KS[cfg.kspace.upper()]
+        p.kspace = cls._KS[cfg.kspace.upper()]
         p.p_max = cfg.coulomb_sample_num if cfg.kspace.upper() == "RBE" else 0
         p.ewald_tol = cfg.ewald_tol
         p.list_capacity = cfg.list_capacity
         p.tile_stage = cfg.tile_stage
-        h = C.c_void_p()
-        rc = self.lib.rbmd_force_create(C.byref(p), int(cfg.n_max), C.byref(h))
-        if rc != 0:
-            raise RBMDError(f"rbmd_force_create -> {rc}: "
-                            f"{self.lib.rbmd_force_last_error(None).decode()}")
-        self.h = h
-        self.n = 0
-        self.fdtype = torch.float32 if cfg.precision == 32 else torch.float64
+        return p, (eps, sig)
 
     # ------------------------------------------------------------ lifecycle
     def close(self):
         if getattr(self, "h", None):
-            self.lib.rbmd_force_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.rbmd_force_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -164,6 +164,43 @@ int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int
  * traffic counters against a kernel of known traffic. */
 int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream);
 
+/* ---- batched replicas ----
+ *
+ * Many independent systems of uneven size advanced together: each phase of the
+ * velocity-Verlet step (kick/drift/sample, list build, rho(k), pair pass,
+ * k-space force + kick) is ONE launch over all replicas, driven by a table of
+ * (replica, workgroup) work items with per-replica arguments in device memory,
+ * and the whole step is captured once in a HIP graph.  Each replica is an
+ * ordinary force handle (rbmd_batch_replica) with the ensemble loop of
+ * ExecutionNVT::Solve (rbmd/framework/run/ExecutionNVT.cpp:133-159) per replica;
+ * a replica advanced in a batch takes bit-identical steps to the same handle
+ * advanced alone by rbmd_md_run with the same seed.  The reference runs one
+ * system per process and has no batch interface. */
+typedef struct rbmd_batch_s* rbmd_batch_t;
+
+/* Device pointers of one replica's MD state (as for rbmd_md_run). */
+typedef struct {
+    double *x, *y, *z, *vx, *vy, *vz;
+    void *fx, *fy, *fz;   /* FP32 or FP64 by the replicas' precision */
+    int32_t* img;         /* nullable */
+    const double* mass;   /* nullable: unit masses */
+} rbmd_replica_state;
+
+/* params[nrep], n_max[nrep] HOST.  All replicas share precision, k-space mode
+ * and whether they have several atom types. */
+int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64_t* n_max,
+                      rbmd_batch_t* out);
+void rbmd_batch_destroy(rbmd_batch_t b);
+const char* rbmd_batch_last_error(rbmd_batch_t b);
+/* Borrowed handle of replica r (set_atoms, sample_kvectors, set_proposal, compute,
+ * get_* work on it as on any handle); NULL if r is out of range. */
+rbmd_force_t rbmd_batch_replica(rbmd_batch_t b, int32_t r);
+/* nsteps velocity-Verlet steps of every replica.  st[nrep], seeds[nrep] HOST; replica r
+ * draws its k-vectors with seeds[r], step numbers step0 .. step0 + nsteps - 1.
+ * Synchronising. */
+int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, int64_t nsteps,
+                      const uint64_t* seeds, uint64_t step0, int32_t use_graph, void* stream);
+
 /* ---- learned k-space proposal ----
  *
  * Replaces the analytic k-sampler RBEPSAMPLE (rbmd/framework/model/RBEPSample.h:143-175,

@@ -1,0 +1,152 @@
+"""GPU tests of batched uneven replicas: a batch step is one launch per phase over
+all replicas, and every replica must take bit-identical steps to the same system
+advanced alone (same kernels, same k-lists), in graph and eager mode."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def _cfg(L, n, kspace="RBE", P=100, precision=32):
+    from copy_gpu_amd import ForceConfig
+
+    return ForceConfig(box_len=(L, L, L), cut_off=2.5, coulomb_cut=5.0, alpha=0.5, skin=0.3,
+                       kspace=kspace, coulomb_sample_num=P, precision=precision, n_max=n)
+
+
+def _systems(sizes, seed0=100):
+    from copy_gpu_amd import systems
+
+    out = []
+    for r, n in enumerate(sizes):
+        x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=seed0 + r)
+        vx, vy, vz = systems.velocities(n, 1.0, seed=seed0 + r)
+        out.append((x, y, z, q, L, vx, vy, vz))
+    return out
+
+
+def _state(sysr, fdtype):
+    x, y, z, q, L, vx, vy, vz = sysr
+    n = x.size
+    return [_dev(a) for a in (x, y, z, vx, vy, vz)] + \
+        [torch.zeros(n, device="cuda", dtype=fdtype) for _ in range(3)] + \
+        [torch.zeros(3 * n, device="cuda", dtype=torch.int32)]
+
+
+def _prime(f, st, P, seed, kspace):
+    if kspace == "RBE":
+        f.sample_kvectors(P, seed, 0)
+    f.Execute(*st[:3], *st[6:9])
+
+
+def _run_single(sysr, seed, nsteps, kspace, P, use_graph, proposal=None):
+    from copy_gpu_amd import LJCutCoulKspaceForce
+
+    x, y, z, q, L, vx, vy, vz = sysr
+    f = LJCutCoulKspaceForce(_cfg(L, x.size, kspace, P))
+    f.Init(_dev(q))
+    if proposal is not None:
+        proposal.attach(f)
+    st = _state(sysr, f.fdtype)
+    _prime(f, st, P, seed, kspace)
+    f.md_run(*st[:9], dt=0.002, nsteps=nsteps, seed=seed, step0=1, img=st[9],
+             use_graph=use_graph)
+    torch.cuda.synchronize()
+    out = [t.cpu().numpy() for t in st]
+    if kspace == "RBE":
+        m, w, _, _ = f.kspace()
+        out += [m, w]
+    f.close()
+    return out
+
+
+def _run_batch(syss, seeds, nsteps, kspace, P, use_graph, proposals=None):
+    from copy_gpu_amd.batch import ReplicaBatch
+
+    b = ReplicaBatch([_cfg(s[4], s[0].size, kspace, P) for s in syss])
+    states = []
+    for r, (sysr, f) in enumerate(zip(syss, b.replicas)):
+        f.Init(_dev(sysr[3]))
+        if proposals and proposals[r] is not None:
+            proposals[r].attach(f)
+        st = _state(sysr, f.fdtype)
+        _prime(f, st, P, seeds[r], kspace)
+        states.append(st)
+    b.md_run([tuple(st) for st in states], dt=0.002, nsteps=nsteps, seeds=seeds, step0=1,
+             use_graph=use_graph)
+    torch.cuda.synchronize()
+    outs = []
+    for st, f in zip(states, b.replicas):
+        o = [t.cpu().numpy() for t in st]
+        if kspace == "RBE":
+            m, w, _, _ = f.kspace()
+            o += [m, w]
+        outs.append(o)
+    stats = [f.stats() for f in b.replicas]
+    b.close()
+    return outs, stats
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_rbe_equals_single_runs(gpu, use_graph):
+    sizes = [2048, 5000, 3100, 9000, 2600]
+    _log(f"case: batch of {len(sizes)} uneven RBE replicas {sizes}, 40 steps, graph={use_graph}")
+    syss = _systems(sizes)
+    seeds = [SEED + 7 * r for r in range(len(sizes))]
+    outs, stats = _run_batch(syss, seeds, 40, "RBE", 100, use_graph)
+    assert all(s["builds"] >= 2 for s in stats)  # the step includes real rebuilds
+    for r, sysr in enumerate(syss):
+        ref = _run_single(sysr, seeds[r], 40, "RBE", 100, use_graph=True)
+        for a, b_ in zip(outs[r], ref):
+            assert np.array_equal(a, b_), f"replica {r} differs"
+
+
+def test_batch_lj_only_equals_single_runs(gpu):
+    sizes = [2500, 4100, 2048]
+    _log(f"case: batch of LJ + real-space replicas {sizes}, 30 steps")
+    syss = _systems(sizes, seed0=300)
+    seeds = [0] * len(sizes)
+    outs, _ = _run_batch(syss, seeds, 30, "NONE", 0, True)
+    for r, sysr in enumerate(syss):
+        ref = _run_single(sysr, 0, 30, "NONE", 0, use_graph=False)
+        for a, b_ in zip(outs[r], ref):
+            assert np.array_equal(a, b_), f"replica {r} differs"
+
+
+def test_batch_with_learned_proposals(gpu):
+    from copy_gpu_amd import proposal as PR
+
+    sizes = [3000, 2048, 6000]
+    _log(f"case: batch with per-replica learned proposals {sizes}, 25 steps")
+    syss = _systems(sizes, seed0=500)
+    seeds = [11, 12, 13]
+    rng = np.random.default_rng(0)
+    props = []
+    for r, s in enumerate(syss):
+        if r == 1:
+            props.append(None)  # this replica keeps the analytic sampler
+            continue
+        L = s[4]
+        Mg = 8
+        m = PR.lattice_grid(Mg)
+        d = PR.analytic_weight(m, (L, L, L), 0.5) * rng.uniform(0.5, 2.0, len(m))
+        props.append(PR.Proposal(Mg, d, 0.9, 0.5, (L, L, L)))
+    outs, _ = _run_batch(syss, seeds, 25, "RBE", 100, True, props)
+    for r, sysr in enumerate(syss):
+        ref = _run_single(sysr, seeds[r], 25, "RBE", 100, use_graph=True, proposal=props[r])
+        for a, b_ in zip(outs[r], ref):
+            assert np.array_equal(a, b_), f"replica {r} differs"
