@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -43,11 +44,17 @@ WORKLOADS = {
                steps=200, desc="large charged LJ, plain RBE: 1048576 ions, density 0.3, "
                            "rc_LJ 2.5, rc_coul 6.0, alpha 0.3, P=500, dt 0.002"),
     "c4": dict(n=100000, rho=0.3, rc_lj=2.5, rc_coul=5.0, alpha=0.5, P=50, dt=0.002,
-               steps=200, learned=True, Mg=32, snapshots=50, snap_every=10,
+               steps=200, learned=True, Mg=32, equilibrate=500, snapshots=50, snap_every=10,
                plain_P=(50, 100, 200, 400),
                desc="learned k-space proposal: 100000 ions, density 0.3, rc_LJ 2.5, "
                     "rc_coul 5.0, alpha 0.5; MLP 4-64-64-1 trained on FP64 |rho(k)|^2 of 50 "
                     "MD snapshots on |m_d| <= 32; learned P=50 vs analytic P=50/100/200/400"),
+    "c5": dict(n=0, replicas=64, n_min=2048, n_max=65536, rho=0.3, rc_lj=2.5, rc_coul=5.0,
+               alpha=0.5, P=100, dt=0.002, steps=300, batch=True,
+               desc="64 uneven replicas, N_r = floor(2048 * 32^u_r) seeded (log-uniform on "
+                    "[2048, 65536]), density 0.3, rc_LJ 2.5, rc_coul 5.0, alpha 0.5, P=100 "
+                    "learned-proposal k-samples per replica, dt 0.002; one batched launch "
+                    "per phase, HIP graph per step, vs the same replicas run one after another"),
 }
 
 
@@ -217,11 +224,11 @@ def learned_eval(wl, L, Q, st, step, args):
     f.Init(Q)
     f.sample_kvectors(100, SEED, 0)
     f.Execute(st.X, st.Y, st.Z, *st.F)
-    st.run(f, wl, args.warmup, step)
-    step += args.warmup
+    st.run(f, wl, wl["equilibrate"], step)
+    step += wl["equilibrate"]
     snaps = []
-    log(f"trajectory: {wl['snapshots']} snapshots every {wl['snap_every']} steps "
-        "(analytic RBE, P=100)")
+    log(f"trajectory: {wl['equilibrate']} steps, then {wl['snapshots']} snapshots every "
+        f"{wl['snap_every']} steps (analytic RBE, P=100)")
     for i in range(wl["snapshots"]):
         st.run(f, wl, wl["snap_every"], step)
         step += wl["snap_every"]
@@ -289,6 +296,7 @@ def learned_eval(wl, L, Q, st, step, args):
         else:
             g.close()
     out = {"runs": runs, "train": {
+        "equilibration_steps": wl["equilibrate"],
         "snapshots": wl["snapshots"], "snapshot_spacing_steps": wl["snap_every"],
         "grid": f"|m_d| <= {Mg} ({(2 * Mg + 1) ** 3} lattice points)",
         "samples": int((Mg + 1) ** 3 - 1), "adam_iters": 2400,
@@ -313,6 +321,116 @@ def learned_eval(wl, L, Q, st, step, args):
         log(f"equal eps_F: analytic needs P = {P_eq:.1f}; learned speedup "
             f"{out['speedup_at_equal_eps']:.3f}x")
     return out, learned_state, prop, step
+
+
+def batch_eval(wl, args):
+    """c5: train one proposal network, tabulate it on every replica's own k-lattice,
+    then time the batched step against the same replicas run one after another."""
+    import torch
+
+    from copy_gpu_amd import ForceConfig, systems
+    from copy_gpu_amd import proposal as PR
+    from copy_gpu_amd.batch import ReplicaBatch
+
+    sizes = systems.replica_sizes(wl["replicas"], wl["n_min"], wl["n_max"], seed=SEED)
+    log(f"replicas: {len(sizes)}, total {sum(sizes)} ions, sizes {min(sizes)}..{max(sizes)}")
+    alpha, P = wl["alpha"], wl["P"]
+    t64 = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to("cuda", torch.float64)  # noqa
+    syss = []
+    for r, n in enumerate(sizes):
+        x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED + 1000 + r)
+        vx, vy, vz = systems.velocities(n, 1.0, seed=SEED + 1000 + r)
+        syss.append((n, L, t64(q), State(*(t64(a) for a in (x, y, z, vx, vy, vz)),
+                                          [torch.zeros(n, device="cuda", dtype=torch.float32)
+                                           for _ in range(3)])))
+    # proposal network: trained on FP64 spectra of the largest replica's trajectory
+    big = max(range(len(sizes)), key=lambda r: sizes[r])
+    n, L, Q, st = syss[big]
+    wlb = dict(wl, n=n)
+    f = make_force(wlb, L, P)
+    f.Init(Q)
+    tr = st.clone()
+    f.sample_kvectors(P, SEED, 0)
+    f.Execute(tr.X, tr.Y, tr.Z, *tr.F)
+    tr.run(f, wlb, 100, 1)
+    snaps = []
+    for i in range(20):
+        tr.run(f, wlb, 10, 101 + 10 * i)
+        snaps.append((tr.X.clone(), tr.Y.clone(), tr.Z.clone()))
+    f.close()
+    sig = math.sqrt(alpha / 2.0) * L / math.pi
+    Mg = int(math.ceil(3.0 * sig))
+    t = time.perf_counter()
+    S2 = PR.spectrum(snaps, Q, (L, L, L), alpha, Mg)
+    mlp, hist, k_scale = PR.train_proposal(S2, (L, L, L), alpha, Mg)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    del snaps, tr
+    log(f"proposal: spectra of 20 snapshots of replica {big} (N={n}, Mg={Mg}) and training "
+        f"{t_train:.2f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+
+    cfgs = [ForceConfig(box_len=(L, L, L), cut_off=wl["rc_lj"], coulomb_cut=wl["rc_coul"],
+                        skin=0.3, alpha=alpha, kspace="RBE", coulomb_sample_num=P,
+                        precision=32, n_max=n) for (n, L, _, _) in syss]
+    b = ReplicaBatch(cfgs)
+    props = []
+    for r, ((n, L, Q, st), fr) in enumerate(zip(syss, b.replicas)):
+        fr.Init(Q)
+        Mr = max(8, int(math.ceil(3.0 * math.sqrt(alpha / 2.0) * L / math.pi)))
+        d = PR.grid_density(mlp, (L, L, L), alpha, Mr, k_scale)
+        pr = PR.Proposal(Mr, d, 0.9, alpha, (L, L, L))
+        pr.attach(fr)
+        props.append(pr)
+        fr.sample_kvectors(P, SEED + r, 0)
+        fr.Execute(st.X, st.Y, st.Z, *st.F)
+    seeds = [SEED + r for r in range(len(syss))]
+    states = [(s.X, s.Y, s.Z, s.VX, s.VY, s.VZ, *s.F) for (_, _, _, s) in syss]
+    K = args.steps if args.steps is not None else wl["steps"]
+    W = args.warmup
+    use_graph = not args.no_graph
+    b.md_run(states, dt=wl["dt"], nsteps=W, seeds=seeds, step0=1, use_graph=use_graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.md_run(states, dt=wl["dt"], nsteps=K, seeds=seeds, step0=1 + W, use_graph=use_graph)
+    torch.cuda.synchronize()
+    el_b = time.perf_counter() - t0
+    tot = sum(sizes)
+    rate_b = tot * K / el_b
+    log(f"batched: {K} steps of {len(sizes)} replicas in {el_b * 1e3:.1f} ms -> "
+        f"{rate_b:.4e} atom-steps/s ({el_b * 1e3 / K:.3f} ms per batch step)")
+    # the same replicas, one md_run after another (each with its own graph)
+    for r, fr in enumerate(b.replicas):
+        s_ = syss[r][3]
+        fr.md_run(s_.X, s_.Y, s_.Z, s_.VX, s_.VY, s_.VZ, *s_.F, dt=wl["dt"], nsteps=W,
+                  seed=seeds[r], step0=1 + W + K, use_graph=use_graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r, fr in enumerate(b.replicas):
+        s_ = syss[r][3]
+        fr.md_run(s_.X, s_.Y, s_.Z, s_.VX, s_.VY, s_.VZ, *s_.F, dt=wl["dt"], nsteps=K,
+                  seed=seeds[r], step0=1 + 2 * W + K, use_graph=use_graph)
+    torch.cuda.synchronize()
+    el_s = time.perf_counter() - t0
+    rate_s = tot * K / el_s
+    log(f"sequential: {rate_s:.4e} atom-steps/s ({el_s * 1e3 / K:.3f} ms per step of all); "
+        f"batched / sequential = {rate_b / rate_s:.2f}x")
+    # force error vs FP64 Ewald on four replicas (smallest, two middle, largest)
+    order = sorted(range(len(sizes)), key=lambda r: sizes[r])
+    pick = [order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]]
+    errs = []
+    if not args.skip_ewald:
+        for r in pick:
+            n, L, Q, s_ = syss[r]
+            wlr = dict(wl, n=n)
+            refs, _ = ewald_refs(wlr, L, Q, [(s_.X, s_.Y, s_.Z)])
+            e_l, ek_l = force_error(b.replicas[r], P, [(s_.X, s_.Y, s_.Z)], refs, 10)
+            errs.append({"replica": r, "n": n, "eps_F": e_l, "eps_F_kspace_part": ek_l})
+            log(f"replica {r} (N={n}): eps_F {e_l:.4e} (k-space part {ek_l:.4e})")
+    b.close()
+    return {"value": rate_b, "seconds": el_b, "steps": K, "total_atoms": tot,
+            "sequential_atom_steps_per_s": rate_s, "batched_over_sequential": rate_b / rate_s,
+            "ms_per_batch_step": el_b * 1e3 / K, "ms_per_sequential_sweep": el_s * 1e3 / K,
+            "replica_sizes": sizes, "eps_F_samples": errs, "proposal_train_s": t_train}
 
 
 def main():
@@ -356,6 +474,31 @@ def main():
     wl = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else wl["steps"]
     W = args.warmup
+    if wl.get("batch"):
+        res = batch_eval(wl, args)
+        out = {
+            "metric": METRIC,
+            "value": res["value"],
+            "unit": "atom-steps/s",
+            "n_gpus": 1,
+            "steps": K,
+            "warmup": W,
+            "ms_per_step": res["ms_per_batch_step"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "fp32",
+            "data": "synthetic",
+            "config": {"workload": wl["desc"], "n_atoms_total": res["total_atoms"],
+                       "replicas": len(res["replica_sizes"]), "P": wl["P"],
+                       "kspace": "learned MLP proposal per replica (mixture 0.9/0.1)",
+                       "hip_graph": not args.no_graph, "parallelism": "single GPU"},
+            "batch": {k: v for k, v in res.items() if k not in ("value",)},
+            "roofline": None,
+            "cpu_baseline": None,
+        }
+        print(json.dumps(out), flush=True)
+        return
     n, P = wl["n"], wl["P"]
     dev = torch.device("cuda:0")
     log(f"workload {args.workload}: generating {n} ions")

@@ -84,6 +84,8 @@ _SIGS = {
     "rbmd_force_get_neighbors": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "rbmd_force_get_kspace": (C.c_int, [_P, C.POINTER(C.c_int32), _P, _P, _P, _P]),
     "rbmd_force_get_stats": (C.c_int, [_P, _P]),
+    "rbmd_force_autotune": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, C.c_int32, _P]),
+    "rbmd_batch_autotune": (C.c_int, [_P, _P, C.c_int32, _P]),
     "rbmd_force_set_timing": (C.c_int, [_P, C.c_int32]),
     "rbmd_force_get_timing": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]),

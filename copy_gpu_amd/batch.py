@@ -55,15 +55,32 @@ class ReplicaBatch:
         except Exception:
             pass
 
+    @staticmethod
+    def _states(states):
+        R = len(states)
+        arr = (L.ReplicaState * R)()
+        for r, stt in enumerate(states):
+            t = list(stt) + [None] * (11 - len(stt))
+            arr[r] = L.ReplicaState(*[None if v is None else v.data_ptr() for v in t])
+        return arr
+
+    def autotune(self, states, reps=3, stream=None):
+        """One tile-shape policy for all replicas, timed as batched launches."""
+        assert len(states) == len(self.replicas)
+        rc = self.lib.rbmd_batch_autotune(self.h, self._states(states), reps, _stream(stream))
+        if rc != 0:
+            raise RBMDError(f"rbmd_batch_autotune -> {rc}: "
+                            f"{self.lib.rbmd_batch_last_error(self.h).decode()}")
+        st = self.replicas[0].stats()
+        return {k: st[k] for k in ("split", "tile_stage", "rho_atoms_per_workgroup",
+                                   "tuned_cost_us")}
+
     def md_run(self, states, dt, nsteps, seeds, step0=0, use_graph=True, stream=None):
         """states[r] = (x, y, z, vx, vy, vz, fx, fy, fz[, img[, mass]]) device tensors;
         seeds[r] = replica r's k-sampler seed."""
         R = len(self.replicas)
         assert len(states) == R and len(seeds) == R
-        arr = (L.ReplicaState * R)()
-        for r, stt in enumerate(states):
-            t = list(stt) + [None] * (11 - len(stt))
-            arr[r] = L.ReplicaState(*[None if v is None else v.data_ptr() for v in t])
+        arr = self._states(states)
         sd = (C.c_uint64 * R)(*[int(s) for s in seeds])
         rc = self.lib.rbmd_batch_md_run(self.h, arr, dt, nsteps, sd, step0,
                                         1 if use_graph else 0, _stream(stream))

@@ -181,7 +181,7 @@ static void tile_geometry(Engine& E, int64_t n)
     const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
     const double cellv = E.cell[0] * E.cell[1] * E.cell[2];
     const double mean = 27.0 * (double)n / V * cellv;
-    int cap = ((int)(1.25 * mean + 64.0) + 63) / 64 * 64;
+    int cap = ((int)(E.tune_stage_scale * mean + 64.0) + 63) / 64 * 64;
     const int per = (int)sizeof(double4) + (E.pp.ntypes > 1 ? 4 : 0);
     const int max_entries = (E.lds_max - 4096) / per;
     if (E.stage_override > 0) cap = E.stage_override;
@@ -194,6 +194,7 @@ static void tile_geometry(Engine& E, int64_t n)
     E.split = 1;
     if (4 * E.ncell < 3 * resident)
         E.split = std::min(4, (resident + E.ncell - 1) / E.ncell);
+    if (E.tune_split > 0) E.split = std::min(4, E.tune_split);
 }
 
 extern "C" {
@@ -771,6 +772,12 @@ int rbmd_force_get_stats(rbmd_force_t h, int64_t* out)
     }
     out[10] = big;
     out[11] = E.split;
+    {
+        int apb = 0, nbx = 0, nkt = 0;
+        if (E.P > 0) rho_geometry(E, &apb, &nbx, &nkt);
+        out[12] = apb;
+    }
+    out[13] = (int64_t)(E.tuned_ms * 1e3);
     return RBMD_OK;
 }
 
@@ -799,6 +806,140 @@ int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int
             return RBMD_OK;
         }
     return fail(E, RBMD_ERR_INVALID, "unknown pass name");
+}
+
+// ---------------------------------------------------------------- autotuner
+
+}  // extern "C"
+
+// Tile shapes searched by the autotuner.
+struct TilePolicy {
+    int split;          // workgroups per cell tile (0: heuristic)
+    double stage_scale; // LDS stage = scale * mean stencil population + 64
+    int rho_apb;        // atoms per rho(k) workgroup (0: heuristic)
+};
+
+static void apply_policy(Engine& E, const TilePolicy& p)
+{
+    E.tune_split = p.split;
+    E.tune_stage_scale = p.stage_scale;
+    E.rho_apb = p.rho_apb;
+    tile_geometry(E, E.n);
+}
+
+// Expected share of steps that rebuild the list (weights the build's cost).
+static constexpr double kRebuildShare = 0.15;
+
+// Search order: (split, stage) with the rho(k) heuristic, then the rho(k)
+// workgroup size with the best tile shape.  measure(policy) returns the cost
+// in ms (negative: failure).
+template <class Measure>
+static TilePolicy autotune_search(const TilePolicy& start, Measure measure, double* best_ms)
+{
+    TilePolicy best = start;
+    double bc = measure(best);
+    for (int split : {1, 2, 3, 4})
+        for (double sc : {1.1, 1.25, 1.5}) {
+            const TilePolicy p{split, sc, start.rho_apb};
+            const double c = measure(p);
+            if (c >= 0 && c < bc) {
+                bc = c;
+                best = p;
+            }
+        }
+    for (int apb : {64, 128, 256, 512, 1024, 2048, 4096}) {
+        TilePolicy p = best;
+        p.rho_apb = apb;
+        const double c = measure(p);
+        if (c >= 0 && c < bc) {
+            bc = c;
+            best = p;
+        }
+    }
+    *best_ms = bc;
+    return best;
+}
+
+struct TuneEvents {
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ok = true;
+    TuneEvents()
+    {
+        for (auto& e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    }
+    ~TuneEvents()
+    {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    // build time + force time of the last recorded pass, ms (negative on error)
+    double read(double* tb, double* tf)
+    {
+        float a = 0, b = 0;
+        if (hipEventSynchronize(ev[3]) != hipSuccess ||
+            hipEventElapsedTime(&a, ev[0], ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[2], ev[3]) != hipSuccess)
+            return -1.0;
+        *tb = a;
+        *tf = b;
+        return 0.0;
+    }
+};
+
+extern "C" {
+
+int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const double* z,
+                        void* fx, void* fy, void* fz, int32_t reps, void* stream)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (!E.atoms_set) return fail(E, RBMD_ERR_STATE, "rbmd_force_set_atoms not called");
+    if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
+        return fail(E, RBMD_ERR_STATE, "no k-vectors: call set_kvectors or sample_kvectors");
+    if (E.n == 0) return RBMD_OK;
+    if (!x || !y || !z || !fx || !fy || !fz) return fail(E, RBMD_ERR_INVALID, "null pointer");
+    if (reps < 1 || reps > 100) return fail(E, RBMD_ERR_INVALID, "reps out of [1, 100]");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = maintain(E, s);
+    if (rc != RBMD_OK) return rc;
+    TuneEvents te;
+    if (!te.ok) return fail(E, RBMD_ERR_HIP, "event creation failed");
+    const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
+    auto measure = [&](const TilePolicy& p) -> double {
+        apply_policy(E, p);
+        double sb = 0, sf = 0;
+        for (int r = 0; r <= reps; ++r) {  // the first pass warms up
+            // a rebuild bins into a zeroed histogram (the per-step check
+            // kernel clears it in ordinary passes)
+            (void)hipMemsetAsync(E.cell_count, 0, (size_t)E.ncell * sizeof(int32_t), s);
+            launch_force_rebuild(E, s);
+            (void)hipEventRecord(te.ev[0], s);
+            launch_build(E, x, y, z, s);
+            (void)hipEventRecord(te.ev[1], s);
+            launch_gather(E, x, y, z, s);
+            (void)hipEventRecord(te.ev[2], s);
+            if (ks) launch_rho(E, x, y, z, s);
+            launch_pair(E, fx, fy, fz, false, s);
+            if (ks) launch_kforce(E, x, y, z, fx, fy, fz, nullptr, nullptr, nullptr, nullptr, 0.0, s);
+            (void)hipEventRecord(te.ev[3], s);
+            double tb, tf;
+            if (te.read(&tb, &tf) < 0) return -1.0;
+            if (r > 0) {
+                sb += tb;
+                sf += tf;
+            }
+        }
+        return (sf + kRebuildShare * sb) / reps;
+    };
+    const TilePolicy start{E.tune_split, E.tune_stage_scale, E.rho_apb};
+    double best_ms = 0;
+    const TilePolicy best = autotune_search(start, measure, &best_ms);
+    apply_policy(E, best);
+    launch_force_rebuild(E, s);  // the next pass builds with the chosen shapes
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    E.tuned_ms = best_ms;
+    return RBMD_OK;
 }
 
 // ---------------------------------------------------------------- replica batches
@@ -1000,6 +1141,93 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
             return bfail(b, RBMD_ERR_INVALID,
                          "non-finite positions in replica " + std::to_string(r));
     }
+    return RBMD_OK;
+}
+
+int rbmd_batch_autotune(rbmd_batch_t b, const rbmd_replica_state* st, int32_t reps,
+                        void* stream)
+{
+    if (!b || !st) return RBMD_ERR_INVALID;
+    if (reps < 1 || reps > 100) return bfail(b, RBMD_ERR_INVALID, "reps out of [1, 100]");
+    Batch& B = b->b;
+    const size_t R = b->reps.size();
+    for (size_t r = 0; r < R; ++r) {
+        const Engine& E = *B.es[r];
+        if (!E.atoms_set) return bfail(b, RBMD_ERR_STATE, "replica without set_atoms");
+        if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
+            return bfail(b, RBMD_ERR_STATE, "replica without k-vectors: sample once to fix P");
+        if (!st[r].x || !st[r].vx || !st[r].fx) return bfail(b, RBMD_ERR_INVALID, "null state");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t r = 0; r < R; ++r) {
+        int rc = maintain(*B.es[r], s);
+        if (rc != RBMD_OK) return bfail(b, rc, B.es[r]->err);
+    }
+    BHIP_TRY(hipStreamSynchronize(s));
+    TuneEvents te;
+    if (!te.ok) return bfail(b, RBMD_ERR_HIP, "event creation failed");
+    // the tuning passes run the force phases with dt = 0: the fused closing kick
+    // then leaves the velocities unchanged
+    B.st.resize(R);
+    B.sa.assign(R, SampleArgs{});
+    B.dt = 0.0;
+    for (size_t r = 0; r < R; ++r) {
+        const rbmd_replica_state& q = st[r];
+        RepState& o = B.st[r];
+        o.x = q.x; o.y = q.y; o.z = q.z;
+        o.vx = q.vx; o.vy = q.vy; o.vz = q.vz;
+        o.fx = q.fx; o.fy = q.fy; o.fz = q.fz;
+        o.img = q.img;
+        o.mass = q.mass;
+    }
+    const bool ks = B.kspace == RBMD_KSPACE_RBE || B.kspace == RBMD_KSPACE_EWALD;
+    auto measure = [&](const TilePolicy& p) -> double {
+        for (Engine* E : B.es) apply_policy(*E, p);
+        if (nlist_batch_prepare(B) != hipSuccess || pair_batch_prepare(B) != hipSuccess ||
+            (ks && kspace_batch_prepare(B) != hipSuccess))
+            return -1.0;
+        double sb = 0, sf = 0;
+        for (int r = 0; r <= reps; ++r) {
+            for (Engine* E : B.es) {
+                // a rebuild bins into a zeroed histogram (the kick/check kernel
+                // clears it in ordinary steps)
+                (void)hipMemsetAsync(E->cell_count, 0, (size_t)E->ncell * sizeof(int32_t), s);
+                launch_force_rebuild(*E, s);
+            }
+            (void)hipEventRecord(te.ev[0], s);
+            for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT, BP_LIST})
+                nlist_batch_launch(B, ph, s);
+            (void)hipEventRecord(te.ev[1], s);
+            nlist_batch_launch(B, BP_GATHER, s);
+            (void)hipEventRecord(te.ev[2], s);
+            if (ks) {
+                kspace_batch_launch(B, BP_RHO, s);
+                kspace_batch_launch(B, BP_RHO_FINAL, s);
+            }
+            pair_batch_launch(B, BP_PAIR, s);
+            pair_batch_launch(B, BP_PAIR_BIG, s);
+            if (ks) kspace_batch_launch(B, BP_KFORCE, s);
+            (void)hipEventRecord(te.ev[3], s);
+            double tb, tf;
+            if (te.read(&tb, &tf) < 0) return -1.0;
+            if (r > 0) {
+                sb += tb;
+                sf += tf;
+            }
+        }
+        return (sf + kRebuildShare * sb) / reps;
+    };
+    const Engine& E0 = *B.es[0];
+    const TilePolicy start{E0.tune_split, E0.tune_stage_scale, E0.rho_apb};
+    double best_ms = 0;
+    const TilePolicy best = autotune_search(start, measure, &best_ms);
+    for (Engine* E : B.es) {
+        apply_policy(*E, best);
+        E->tuned_ms = best_ms;
+        launch_force_rebuild(*E, s);
+    }
+    BHIP_TRY(hipGetLastError());
+    BHIP_TRY(hipStreamSynchronize(s));
     return RBMD_OK;
 }
 

@@ -17,6 +17,29 @@
 
 namespace rbmd {
 
+void rho_geometry(const Engine& e, int* atoms_per_block, int* nbx, int* nkt)
+{
+    const int n = (int)e.n;
+    *nkt = (e.P + 63) / 64;
+    int apb;
+    if (e.rho_apb > 0) {
+        apb = e.rho_apb;
+    } else {
+        int want = 2048 / *nkt;
+        if (want < 1) want = 1;
+        apb = (n + want - 1) / want;
+    }
+    // the partials [nbx][P][2] must fit the kpart buffer
+    const int64_t max_nbx = e.P > 0 ? e.kpart_cap / (2 * (int64_t)e.P) : 1;
+    if (max_nbx >= 1 && (int64_t)apb * max_nbx < (int64_t)n)
+        apb = (int)(((int64_t)n + max_nbx - 1) / max_nbx);
+    apb = ((apb + 63) / 64) * 64;
+    if (apb < 64) apb = 64;
+    *atoms_per_block = apb;
+    *nbx = (n + apb - 1) / apb;
+    if (*nbx < 1) *nbx = 1;
+}
+
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -328,19 +351,6 @@ struct KForce {
     }
 };
 
-void rho_geometry(const Engine& e, int* atoms_per_block, int* nbx, int* nkt)
-{
-    const int n = (int)e.n;
-    *nkt = (e.P + 63) / 64;
-    int want = 2048 / *nkt;
-    if (want < 1) want = 1;
-    int apb = (n + want - 1) / want;
-    apb = ((apb + 63) / 64) * 64;
-    if (apb < 64) apb = 64;
-    *atoms_per_block = apb;
-    *nbx = (n + apb - 1) / apb;
-    if (*nbx < 1) *nbx = 1;
-}
 
 template <typename T>
 typename RhoPartial<T>::Args rho_args(Engine& e, const double* x, const double* y,

@@ -52,6 +52,11 @@ struct Engine {
     int split = 1;      // workgroups per cell tile
     int stage_cap = 0;  // LDS tile capacity (candidates)
     int stage_override = 0;
+    // tile shapes chosen by the autotuner (0 / default: heuristics)
+    int tune_split = 0;              // workgroups per cell tile
+    double tune_stage_scale = 1.25;  // LDS stage = scale * mean stencil population + 64
+    int rho_apb = 0;                 // atoms per rho(k) workgroup (multiple of 64)
+    double tuned_ms = 0;             // cost of the chosen shapes at the last autotune
     int lds_max = 0;    // device LDS bytes per workgroup
     int lds_cu = 0;     // device LDS bytes per CU
     int num_cu = 0;
@@ -196,6 +201,7 @@ void launch_ev_final(Engine& e, bool with_pair, hipStream_t s);
 
 // ---- launchers (kspace.hip)
 SampleArgs sample_args(const Engine& e, int P, uint64_t seed, const uint64_t* step_host);
+void rho_geometry(const Engine& e, int* atoms_per_block, int* nbx, int* nkt);
 void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, hipStream_t s);
 void launch_kweights_default(Engine& e, hipStream_t s);
 void launch_rho(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);

@@ -206,12 +206,23 @@ class LJCutCoulKspaceForce:
         return offs, idx[:cnt.value]
 
     def stats(self) -> dict:
-        out = np.zeros(12, np.int64)
+        out = np.zeros(14, np.int64)
         self._check(self.lib.rbmd_force_get_stats(self.h, out.ctypes.data_as(C.c_void_p)),
                     "get_stats")
         keys = ["builds", "entries", "cap", "overflow_rows", "cells", "inner_entries",
-                "ewald_k", "n", "big_tiles", "tile_stage", "largest_tile", "split"]
+                "ewald_k", "n", "big_tiles", "tile_stage", "largest_tile", "split",
+                "rho_atoms_per_workgroup", "tuned_cost_us"]
         return dict(zip(keys, (int(v) for v in out)))
+
+    def autotune(self, x, y, z, fx, fy, fz, reps=5, stream=None):
+        """Pick tile shapes (workgroups per tile, LDS stage, rho(k) workgroup size) by
+        timing candidates on this configuration; returns the chosen shapes."""
+        self._check(self.lib.rbmd_force_autotune(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(fx),
+                                                 _ptr(fy), _ptr(fz), reps, _stream(stream)),
+                    "autotune")
+        st = self.stats()
+        return {k: st[k] for k in ("split", "tile_stage", "rho_atoms_per_workgroup",
+                                   "tuned_cost_us")}
 
     def set_timing(self, enable: bool):
         self._check(self.lib.rbmd_force_set_timing(self.h, 1 if enable else 0), "set_timing")

@@ -4,6 +4,7 @@ Configurations (reduced LJ units):
   fcc_lj      N = 4 nc^3 atoms on an FCC lattice at density rho (no charges)
   electrolyte N ions uniform in a cubic box at density rho, minimum separation 0.8,
               half +1 half -1 by a seeded shuffle
+  replicas    sizes N_r = floor(n_min (n_max / n_min)^u_r), u_r uniform from Philox
 """
 from __future__ import annotations
 
@@ -64,3 +65,16 @@ def sample_kvectors_host(alpha: float, box_len, P: int, seed: int, step: int):
     rc = L.lib().rbmd_sample_kvectors_host(alpha, _p(Lb), P, seed, step, _p(m), C.byref(S))
     assert rc == 0
     return m, S.value
+
+
+def replica_sizes(count: int, n_min: int = 2048, n_max: int = 65536,
+                  seed: int = DEFAULT_SEED):
+    """Log-uniform replica sizes: N_r = floor(n_min (n_max/n_min)^u_r), u_r in [0, 1)
+    the 53-bit uniform of Philox4x32-10 at counter (r, 0x52455053, 0, 0), key = seed."""
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    out = []
+    for r in range(count):
+        w = philox4x32((r, 0x52455053, 0, 0), key)
+        u = ((int(w[0]) >> 5) * 67108864.0 + (int(w[1]) >> 6)) / 9007199254740992.0
+        out.append(int(np.floor(n_min * (n_max / n_min) ** u)))
+    return out

@@ -149,8 +149,20 @@ int rbmd_force_get_kspace(rbmd_force_t h, int32_t* P, int32_t* m, double* w, dou
  * that overflowed at the last build, [4] cells, [5] inner (LJ-range) entries,
  * [6] Ewald k-vectors (EWALD mode), [7] n, [8] cell tiles larger than the LDS
  * stage at the last build, [9] LDS stage (atoms), [10] largest tile (atoms),
- * [11] workgroups per tile. */
+ * [11] workgroups per tile, [12] atoms per rho(k) workgroup, [13] cost of the tile
+ * shapes at the last autotune (microseconds, 0 if never tuned).  out has 14 entries. */
 int rbmd_force_get_stats(rbmd_force_t h, int64_t* out);
+
+/* Tile-shape autotuner (no counterpart in the reference, whose launch shapes are
+ * fixed by VTK-m / the rbmd-gpu kernels).  Times the list build and the force pass
+ * (gather, rho(k), pair pass, k-space force) with HIP events on the given
+ * configuration for candidate workgroups per cell tile (1-4), LDS stage sizes and
+ * rho(k) workgroup sizes, reps passes each, and keeps the fastest for
+ * cost = force pass + 0.15 x list build.  Forces in fx, fy, fz are those of x, y, z
+ * afterwards.  Synchronising; forces a list rebuild at the next pass.  Results: get_stats
+ * [9], [11], [12], [13]. */
+int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const double* z,
+                        void* fx, void* fy, void* fz, int32_t reps, void* stream);
 
 /* Per-kernel device time (HIP events on the launch stream).  enable != 0
  * starts recording around every launch of the named passes; get returns the
@@ -200,6 +212,11 @@ rbmd_force_t rbmd_batch_replica(rbmd_batch_t b, int32_t r);
  * Synchronising. */
 int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, int64_t nsteps,
                       const uint64_t* seeds, uint64_t step0, int32_t use_graph, void* stream);
+/* The tile-shape autotuner of rbmd_force_autotune over the batched phases: one shape
+ * policy for all replicas, timed as batched launches.  Velocities are left unchanged
+ * (the tuning passes kick with dt = 0); forces are recomputed.  Synchronising. */
+int rbmd_batch_autotune(rbmd_batch_t b, const rbmd_replica_state* st, int32_t reps,
+                        void* stream);
 
 /* ---- learned k-space proposal ----
  *

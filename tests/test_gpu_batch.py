@@ -150,3 +150,39 @@ def test_batch_with_learned_proposals(gpu):
         ref = _run_single(sysr, seeds[r], 25, "RBE", 100, use_graph=True, proposal=props[r])
         for a, b_ in zip(outs[r], ref):
             assert np.array_equal(a, b_), f"replica {r} differs"
+
+
+def test_batch_autotune_then_run(gpu):
+    """The batch autotuner times one shape policy for all replicas as batched launches;
+    it leaves velocities unchanged, and the tuned batch still advances every replica
+    with forces equal to a fresh evaluation at the final positions."""
+    from copy_gpu_amd.batch import ReplicaBatch
+
+    sizes = [2048, 7000, 3500, 12000]
+    _log(f"case: batch autotune on replicas {sizes}, then 20 steps")
+    syss = _systems(sizes, seed0=700)
+    b = ReplicaBatch([_cfg(s[4], s[0].size) for s in syss])
+    states = []
+    for r, (sysr, f) in enumerate(zip(syss, b.replicas)):
+        f.Init(_dev(sysr[3]))
+        st = _state(sysr, f.fdtype)
+        _prime(f, st, 100, 40 + r, "RBE")
+        states.append(st)
+    v0 = [[t.clone() for t in st[3:6]] for st in states]
+    shapes = b.autotune([tuple(st) for st in states], reps=2)
+    _log(f"  tuned shapes {shapes}")
+    assert shapes["tuned_cost_us"] > 0
+    for st, v in zip(states, v0):
+        for a, c in zip(st[3:6], v):
+            assert torch.equal(a, c)
+    b.md_run([tuple(st) for st in states], dt=0.002, nsteps=20, seeds=[40 + r for r in
+             range(len(sizes))], step0=1)
+    torch.cuda.synchronize()
+    for st, f in zip(states, b.replicas):
+        F = torch.stack(st[6:9], 1).double()
+        assert torch.isfinite(F).all()
+        Fc = [torch.zeros_like(st[6]) for _ in range(3)]
+        f.Execute(*st[:3], *Fc)
+        Fc = torch.stack(Fc, 1).double()
+        assert float(torch.linalg.norm(F - Fc) / torch.linalg.norm(Fc)) < 1e-5
+    b.close()

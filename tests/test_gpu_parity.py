@@ -246,6 +246,35 @@ def test_rbe_forces_given_klist(orc, gpu, precision, tol):
     assert np.linalg.norm(W - Wo) < 10 * tol * np.linalg.norm(Wo)
 
 
+def test_autotuned_shapes_keep_parity(orc, gpu):
+    """The autotuner only changes launch shapes (workgroups per tile, LDS stage, rho(k)
+    workgroup size): forces with the tuned shapes still meet the FP32 bar."""
+    from copy_gpu_amd import systems
+
+    _log("case: autotune, 12000 ions, RBE FP32 P=100, then parity against the oracle")
+    x, y, z, q, L = _ions(12000, seed=77)
+    Lb = np.array([L] * 3)
+    alpha, P = 0.5, 100
+    m, S = systems.sample_kvectors_host(alpha, Lb, P, seed=SEED, step=4)
+    f = _make(Lb, x.size, kspace="RBE", P=P)
+    f.Init(_dev(q))
+    f.set_kvectors(_dev(m.reshape(-1), torch.int32))
+    X, Y, Z = _dev(x), _dev(y), _dev(z)
+    F = [torch.zeros(x.size, device="cuda", dtype=torch.float32) for _ in range(3)]
+    shapes = f.autotune(X, Y, Z, *F, reps=2)
+    _log(f"  tuned shapes {shapes}")
+    assert shapes["tuned_cost_us"] > 0
+    assert 1 <= shapes["split"] <= 4 and shapes["rho_atoms_per_workgroup"] % 64 == 0
+    Ft = torch.stack(F, 1).double().cpu().numpy()
+    Fc = _compute(f, x, y, z)
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    reo, imo = orc.rho(xw, yw, zw, q, Lb, m)
+    Fp, _ = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, alpha)
+    Fk, _ = orc.kspace_forces(xw, yw, zw, q, Lb, alpha, m, np.full(P, S / P), reo, imo)
+    assert _rel(Fc, Fp + Fk) < 1e-5
+    assert _rel(Ft, Fp + Fk) < 1e-5
+
+
 def test_device_sampler_matches_host(gpu):
     from copy_gpu_amd import systems
 
