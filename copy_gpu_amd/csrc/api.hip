@@ -404,6 +404,9 @@ int rbmd_force_set_atoms(rbmd_force_t h, int64_t n, const double* q, const int32
         for (double v : hq) sq += v * v;
         E.e_self = E.kspace ? -sqrt(E.alpha / M_PI) * sq * E.qqr2e : 0.0;
     }
+    // the inverse permutation is read by the fused gather of the MD step before
+    // the first build writes it: start it in range
+    if (n > 0) HIP_TRY(hipMemsetAsync(E.atom_rank, 0, (size_t)n * 4, s));
     launch_force_rebuild(E, s);
     if (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) {
         const int rc = alloc_kspace(E, E.kspace == RBMD_KSPACE_RBE ? std::max(E.p_max, E.P) : E.P);
@@ -491,16 +494,20 @@ int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_
 }
 
 // One force evaluation; with kick pointers (MD mode) the k-space kernel also
-// applies the closing half kick, otherwise the caller kicks separately.
+// applies the closing half kick, otherwise the caller kicks separately.  In MD
+// mode (checked) the kick/drift kernel has already gathered the positions.
+// One stream: running the structure factor on a second stream concurrently with
+// the list build and pair pass measured slower (cross-queue joins cost more than
+// the overlap gains at these kernel sizes).
 static int force_pass(Engine& E, const double* x, const double* y, const double* z, void* fx,
                       void* fy, void* fz, bool ev, hipStream_t s, bool checked,
                       double* vx = nullptr, double* vy = nullptr, double* vz = nullptr,
                       const double* mass = nullptr, double dt = 0.0)
 {
     if (!checked) launch_check(E, x, y, z, s);
-    launch_build(E, x, y, z, s);
-    launch_gather(E, x, y, z, s);
     const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
+    launch_build(E, x, y, z, s);
+    if (!checked) launch_gather(E, x, y, z, s);
     if (ks) launch_rho(E, x, y, z, s);
     launch_pair(E, fx, fy, fz, ev, s);
     if (ks) launch_kforce(E, x, y, z, fx, fy, fz, vx, vy, vz, mass, dt, s);
@@ -971,11 +978,13 @@ static int bfail(rbmd_batch_t b, int code, const std::string& msg)
     } while (0)
 
 // One batched velocity-Verlet step: every phase is one launch over all replicas.
-static void batch_step(Batch& B, hipStream_t s)
+// The kick/drift kernel gathers the positions.
+static void batch_step(rbmd_batch_t b, hipStream_t s)
 {
+    Batch& B = b->b;
     const bool ks = B.kspace == RBMD_KSPACE_RBE || B.kspace == RBMD_KSPACE_EWALD;
     nlist_batch_launch(B, BP_KICKDRIFT, s);
-    for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT, BP_LIST, BP_GATHER})
+    for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT, BP_LIST})
         nlist_batch_launch(B, ph, s);
     if (ks) {
         kspace_batch_launch(B, BP_RHO, s);
@@ -983,10 +992,11 @@ static void batch_step(Batch& B, hipStream_t s)
     }
     pair_batch_launch(B, BP_PAIR, s);
     pair_batch_launch(B, BP_PAIR_BIG, s);
-    if (ks)
+    if (ks) {
         kspace_batch_launch(B, BP_KFORCE, s);
-    else
+    } else {
         nlist_batch_launch(B, BP_KICK, s);
+    }
 }
 
 extern "C" {
@@ -1012,8 +1022,10 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
             return rc;
         }
         // tiles beyond the LDS stage are rare at these densities: a few
-        // workgroups per replica serve them
+        // workgroups per replica serve them.  The batch fills the GPU with
+        // whole tiles, so a tile is never split over several workgroups.
         h->e.big_grid = 4;
+        h->e.tune_split = 1;
         b->reps.push_back(h);
         b->b.es.push_back(&h->e);
     }
@@ -1105,7 +1117,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     BHIP_TRY(pair_batch_prepare(B));
     if (B.kspace != RBMD_KSPACE_NONE) BHIP_TRY(kspace_batch_prepare(B));
     if (!use_graph) {
-        for (int64_t k = 0; k < nsteps; ++k) batch_step(B, s);
+        for (int64_t k = 0; k < nsteps; ++k) batch_step(b, s);
         BHIP_TRY(hipGetLastError());
     } else {
         std::vector<const void*> key;
@@ -1121,7 +1133,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
             b->graph_exec = nullptr;
             b->graph = nullptr;
             BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            batch_step(B, s);
+            batch_step(b, s);
             hipError_t e = hipStreamEndCapture(s, &b->graph);
             if (e != hipSuccess)
                 return bfail(b, RBMD_ERR_HIP, std::string("graph capture failed: ") +

@@ -168,8 +168,9 @@ struct Scatter {
 
 // ---------------------------------------------------------------- per-cell sort
 // One wave per cell: order the cell's atoms by original id (rank sort in LDS)
-// so the layout is deterministic, then write the sorted reference positions,
-// charges, types, cells and the inverse permutation.
+// so the layout is deterministic, then write the sorted reference positions
+// (also the current sorted positions: a build happens at the current
+// positions), charges, types, cells and the inverse permutation.
 constexpr int kCellSortMax = 2048;
 
 struct CellSort {
@@ -179,7 +180,7 @@ struct CellSort {
         const int32_t* start;
         const double *xr, *yr, *zr, *q;
         const int32_t* type;
-        double4* xs_ref;
+        double4 *xs_ref, *xs;
         int32_t *scell, *ts;
         const uint32_t* flags;
     };
@@ -208,7 +209,9 @@ struct CellSort {
             const int s = s0 + r;
             if (sortable) a.perm[s] = v;
             a.inv_perm[v] = s;
-            a.xs_ref[s] = make_double4(a.xr[v], a.yr[v], a.zr[v], a.q[v]);
+            const double4 p = make_double4(a.xr[v], a.yr[v], a.zr[v], a.q[v]);
+            a.xs_ref[s] = p;
+            a.xs[s] = p;  // current positions = the reference just built
             a.scell[s] = c;
             a.ts[s] = a.type ? a.type[v] : 0;
         }
@@ -408,8 +411,9 @@ constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sam
 
 // First half of the velocity-Verlet step: v += dt/2 F/m, x += v dt, wrap with
 // image flags, displacement check against the list build (and zeroing of the
-// cell histogram for a possible rebuild).  The first blocks also draw this
-// step's k-vectors (the sampler of k_sample) so that no extra launch is needed.
+// cell histogram for a possible rebuild), and the gather of the new positions
+// into the cell-sorted array.  The first blocks also draw this step's
+// k-vectors (the sampler of k_sample) so that no extra launch is needed.
 template <typename T>
 struct KickDrift {
     static constexpr int kThreads = kBlock;
@@ -426,6 +430,9 @@ struct KickDrift {
         uint32_t* flags;
         int32_t* cell_count;
         int ncell;
+        const double* q;
+        const int32_t* inv_perm;
+        double4* xs;
         SampleArgs sa;
     };
     __device__ static void run(const Args& a, int blk, int)
@@ -485,6 +492,9 @@ struct KickDrift {
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 > a.thr2) a.flags[0] = 1u;
         if (d2 != d2) a.flags[1] = 1u;
+        // the gather of the current positions, unwrapped against the reference
+        // (a rebuild later in the step rewrites them in the new order)
+        a.xs[a.inv_perm[i]] = make_double4(a.xr[i] + dx, a.yr[i] + dy, a.zr[i] + dz, a.q[i]);
     }
 };
 
@@ -544,8 +554,8 @@ Scatter::Args scatter_args(Engine& e)
 
 CellSort::Args cellsort_args(Engine& e)
 {
-    return CellSort::Args{e.perm,   e.atom_rank /* reused as inverse perm */, e.cell_start, e.xr,
-                          e.yr,     e.zr,  e.q,     e.type, e.xs_ref, e.scell, e.ts, e.flags};
+    return CellSort::Args{e.perm, e.atom_rank /* reused as inverse perm */, e.cell_start, e.xr,
+                          e.yr,   e.zr, e.q, e.type, e.xs_ref, e.xs, e.scell, e.ts, e.flags};
 }
 
 ListTile::Args list_args(Engine& e)
@@ -570,7 +580,8 @@ typename KickDrift<T>::Args kickdrift_args(Engine& e, double* x, double* y, doub
     const double h = 0.5 * e.skin;
     return typename KickDrift<T>::Args{x,  y,    z,    vx,   vy,  vz, (const T*)fx, (const T*)fy,
                                        (const T*)fz, img, mass, dt, (int)e.n, e.box, e.xr,
-                                       e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell, sa};
+                                       e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell,
+                                       e.q, e.atom_rank, e.xs, sa};
 }
 
 int kickdrift_blocks(const Engine& e, const SampleArgs& sa)

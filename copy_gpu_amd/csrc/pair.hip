@@ -329,10 +329,13 @@ struct PairTile {
         const PairParams* pp;
         T *fx, *fy, *fz;
         double* ev_part;
+        uint32_t* flags;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
         const PairArgs& a = A.a;
+        // the list of this step is built: consume the rebuild request
+        if (blk == 0 && threadIdx.x == 0) A.flags[0] = 0u;
         extern __shared__ double4 lds[];
         int32_t* ltype = (int32_t*)(lds + A.stage_cap);
         __shared__ TileHdr h;
@@ -456,7 +459,7 @@ typename PairTile<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void*
     return typename PairTile<T, EV, M>::Args{
         PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
         e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box, e.stage_cap,
-        pair_scalars(e), e.d_pp, (T*)fx, (T*)fy, (T*)fz, e.ev_part};
+        pair_scalars(e), e.d_pp, (T*)fx, (T*)fy, (T*)fz, e.ev_part, e.flags};
 }
 
 template <typename T, bool EV, bool M>
