@@ -110,8 +110,10 @@ static void free_all(Engine& E)
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (E.h_stats) hipHostFree(E.h_stats);
-    if (E.graph_exec) hipGraphExecDestroy(E.graph_exec);
-    if (E.graph) hipGraphDestroy(E.graph);
+    for (auto& G : E.graphs) {
+        if (G.exec) hipGraphExecDestroy(G.exec);
+        if (G.graph) hipGraphDestroy(G.graph);
+    }
     if (E.own_stream) hipStreamDestroy(E.own_stream);
     if (E.order_event) hipEventDestroy(E.order_event);
 }
@@ -627,27 +629,37 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         key.push_back((const void*)(uintptr_t)seed);
         // the proposal's table pointers, lambda and P-dependent weights are baked too
         key.push_back((const void*)(uintptr_t)E.gp_gen);
-        if (!E.graph_exec || key != E.graph_key) {
-            if (E.graph_exec) hipGraphExecDestroy(E.graph_exec);
-            if (E.graph) hipGraphDestroy(E.graph);
-            E.graph_exec = nullptr;
-            E.graph = nullptr;
+        // two cached graphs: kGraphSteps steps (amortises the launch of a graph)
+        // and one step (the remainder)
+        for (int g = 0; g < 2; ++g) {
+            Engine::StepGraph& G = E.graphs[g];
+            const int steps = g == 0 ? kGraphSteps : 1;
+            if (g == 0 && nsteps < kGraphSteps) continue;
+            if (G.exec && key == G.key) continue;
+            if (G.exec) hipGraphExecDestroy(G.exec);
+            if (G.graph) hipGraphDestroy(G.graph);
+            G.exec = nullptr;
+            G.graph = nullptr;
             E.capturing = true;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            md_step(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, seed, nullptr, s);
-            hipError_t st = hipStreamEndCapture(s, &E.graph);
+            for (int k = 0; k < steps; ++k)
+                md_step(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, seed, nullptr, s);
+            hipError_t st = hipStreamEndCapture(s, &G.graph);
             E.capturing = false;
             if (st != hipSuccess) {
                 E.err = std::string("graph capture failed: ") + hipGetErrorString(st);
                 return RBMD_ERR_HIP;
             }
-            HIP_TRY(hipGraphInstantiate(&E.graph_exec, E.graph, nullptr, nullptr, 0));
-            E.graph_key = key;
+            HIP_TRY(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+            G.key = key;
         }
         // device step counter = step0; the graph's rho_final kernel advances it
         unsigned long long s0 = step0;
         HIP_TRY(hipMemcpyAsync(E.d_step, &s0, 8, hipMemcpyHostToDevice, s));
-        for (int64_t k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(E.graph_exec, s));
+        for (int64_t k = 0; k + kGraphSteps <= nsteps; k += kGraphSteps)
+            HIP_TRY(hipGraphLaunch(E.graphs[0].exec, s));
+        for (int64_t k = 0; k < nsteps % kGraphSteps; ++k)
+            HIP_TRY(hipGraphLaunch(E.graphs[1].exec, s));
     }
     HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -37,6 +37,9 @@ struct SampleArgs {
     double* kw = nullptr;
 };
 
+// MD steps captured in one graph (amortises the cost of launching a graph).
+constexpr int kGraphSteps = 8;
+
 struct Engine {
     // ---- parameters
     Box box{};
@@ -125,10 +128,13 @@ struct Engine {
     int64_t t_cnt[P_NUM] = {};
     std::vector<hipEvent_t> event_pool;
 
-    // MD graph cache
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    std::vector<const void*> graph_key;
+    // MD graph cache: [0] kGraphSteps steps, [1] one step
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<const void*> key;
+    };
+    StepGraph graphs[2];
     bool capturing = false;
 
     std::string err;
