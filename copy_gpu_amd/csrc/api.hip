@@ -106,7 +106,7 @@ static void free_all(Engine& E)
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
-                    E.ev_part,  E.ev,    E.big_tiles, E.gp_cdf, E.gp_ratio, E.gp_guide};
+                    E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (E.h_stats) hipHostFree(E.h_stats);
@@ -326,7 +326,6 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.atom_rank, N * 4));
     CR(dalloc(&E.cell_count, (size_t)E.ncell * 4));
     CR(dalloc(&E.cell_start, ((size_t)E.ncell + 1) * 4));
-    CR(dalloc(&E.big_tiles, (size_t)E.ncell * 4));
     CR(dalloc(&E.ncount, N * sizeof(int2)));
     CR(dalloc(&E.flags, 64));
     CR(dalloc(&E.stats, 128));
@@ -335,7 +334,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.d_pp, sizeof(PairParams)));
     CR(dalloc(&E.d_step, 64));
     CR(dalloc(&E.ev, 16 * 8));
-    E.ev_part_blocks = E.ncell * 4 + 64;  // staged pass (split <= 4) + big-tile pass
+    E.ev_part_blocks = E.ncell * 4;  // tile workgroups (split <= 4)
     CR(dalloc(&E.ev_part, (size_t)E.ev_part_blocks * 8 * 8));
     CR(hipHostMalloc((void**)&E.h_stats, 64, hipHostMallocDefault));
     CR(hipStreamCreateWithFlags(&E.own_stream, hipStreamNonBlocking));
@@ -891,7 +890,8 @@ struct TuneEvents {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
-    // build time + force time of the last recorded pass, ms (negative on error)
+    // times of the last rebuild pass (ev0 -> ev1) and plain pass (ev2 -> ev3), ms
+    // (negative on error)
     double read(double* tb, double* tf)
     {
         float a = 0, b = 0;
@@ -924,22 +924,26 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
     TuneEvents te;
     if (!te.ok) return fail(E, RBMD_ERR_HIP, "event creation failed");
     const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
+    auto pass = [&]() {
+        launch_build(E, x, y, z, s);
+        launch_gather(E, x, y, z, s);
+        if (ks) launch_rho(E, x, y, z, s);
+        launch_pair(E, fx, fy, fz, false, s);
+        if (ks) launch_kforce(E, x, y, z, fx, fy, fz, nullptr, nullptr, nullptr, nullptr, 0.0, s);
+    };
     auto measure = [&](const TilePolicy& p) -> double {
         apply_policy(E, p);
         double sb = 0, sf = 0;
-        for (int r = 0; r <= reps; ++r) {  // the first pass warms up
-            // a rebuild bins into a zeroed histogram (the per-step check
-            // kernel clears it in ordinary passes)
+        for (int r = 0; r <= reps; ++r) {  // the first round warms up
+            // a pass that rebuilds (into a zeroed histogram: the per-step check
+            // kernel clears it in ordinary passes), then one that does not
             (void)hipMemsetAsync(E.cell_count, 0, (size_t)E.ncell * sizeof(int32_t), s);
             launch_force_rebuild(E, s);
             (void)hipEventRecord(te.ev[0], s);
-            launch_build(E, x, y, z, s);
+            pass();
             (void)hipEventRecord(te.ev[1], s);
-            launch_gather(E, x, y, z, s);
             (void)hipEventRecord(te.ev[2], s);
-            if (ks) launch_rho(E, x, y, z, s);
-            launch_pair(E, fx, fy, fz, false, s);
-            if (ks) launch_kforce(E, x, y, z, fx, fy, fz, nullptr, nullptr, nullptr, nullptr, 0.0, s);
+            pass();
             (void)hipEventRecord(te.ev[3], s);
             double tb, tf;
             if (te.read(&tb, &tf) < 0) return -1.0;
@@ -948,7 +952,7 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
                 sf += tf;
             }
         }
-        return (sf + kRebuildShare * sb) / reps;
+        return (sf + kRebuildShare * (sb - sf)) / reps;
     };
     const TilePolicy start{E.tune_split, E.tune_stage_scale, E.rho_apb};
     double best_ms = 0;
@@ -996,14 +1000,12 @@ static void batch_step(rbmd_batch_t b, hipStream_t s)
     Batch& B = b->b;
     const bool ks = B.kspace == RBMD_KSPACE_RBE || B.kspace == RBMD_KSPACE_EWALD;
     nlist_batch_launch(B, BP_KICKDRIFT, s);
-    for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT, BP_LIST})
-        nlist_batch_launch(B, ph, s);
+    for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT}) nlist_batch_launch(B, ph, s);
     if (ks) {
         kspace_batch_launch(B, BP_RHO, s);
         kspace_batch_launch(B, BP_RHO_FINAL, s);
     }
     pair_batch_launch(B, BP_PAIR, s);
-    pair_batch_launch(B, BP_PAIR_BIG, s);
     if (ks) {
         kspace_batch_launch(B, BP_KFORCE, s);
     } else {
@@ -1033,10 +1035,8 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
             rbmd_batch_destroy(b);
             return rc;
         }
-        // tiles beyond the LDS stage are rare at these densities: a few
-        // workgroups per replica serve them.  The batch fills the GPU with
-        // whole tiles, so a tile is never split over several workgroups.
-        h->e.big_grid = 4;
+        // the batch fills the GPU with whole tiles, so a tile is never split
+        // over several workgroups
         h->e.tune_split = 1;
         b->reps.push_back(h);
         b->b.es.push_back(&h->e);
@@ -1210,6 +1210,16 @@ int rbmd_batch_autotune(rbmd_batch_t b, const rbmd_replica_state* st, int32_t re
         if (nlist_batch_prepare(B) != hipSuccess || pair_batch_prepare(B) != hipSuccess ||
             (ks && kspace_batch_prepare(B) != hipSuccess))
             return -1.0;
+        auto pass = [&]() {
+            for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT}) nlist_batch_launch(B, ph, s);
+            nlist_batch_launch(B, BP_GATHER, s);
+            if (ks) {
+                kspace_batch_launch(B, BP_RHO, s);
+                kspace_batch_launch(B, BP_RHO_FINAL, s);
+            }
+            pair_batch_launch(B, BP_PAIR, s);
+            if (ks) kspace_batch_launch(B, BP_KFORCE, s);
+        };
         double sb = 0, sf = 0;
         for (int r = 0; r <= reps; ++r) {
             for (Engine* E : B.es) {
@@ -1219,18 +1229,10 @@ int rbmd_batch_autotune(rbmd_batch_t b, const rbmd_replica_state* st, int32_t re
                 launch_force_rebuild(*E, s);
             }
             (void)hipEventRecord(te.ev[0], s);
-            for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT, BP_LIST})
-                nlist_batch_launch(B, ph, s);
+            pass();
             (void)hipEventRecord(te.ev[1], s);
-            nlist_batch_launch(B, BP_GATHER, s);
             (void)hipEventRecord(te.ev[2], s);
-            if (ks) {
-                kspace_batch_launch(B, BP_RHO, s);
-                kspace_batch_launch(B, BP_RHO_FINAL, s);
-            }
-            pair_batch_launch(B, BP_PAIR, s);
-            pair_batch_launch(B, BP_PAIR_BIG, s);
-            if (ks) kspace_batch_launch(B, BP_KFORCE, s);
+            pass();
             (void)hipEventRecord(te.ev[3], s);
             double tb, tf;
             if (te.read(&tb, &tf) < 0) return -1.0;
@@ -1239,7 +1241,7 @@ int rbmd_batch_autotune(rbmd_batch_t b, const rbmd_replica_state* st, int32_t re
                 sf += tf;
             }
         }
-        return (sf + kRebuildShare * sb) / reps;
+        return (sf + kRebuildShare * (sb - sf)) / reps;
     };
     const Engine& E0 = *B.es[0];
     const TilePolicy start{E0.tune_split, E0.tune_stage_scale, E0.rho_apb};

@@ -107,12 +107,21 @@ struct Scan {
         const int32_t* count;
         int32_t* start;
         int ncell;
-        const uint32_t* flags;
+        uint32_t* flags;
         unsigned long long* stats;
     };
     __device__ static void run(const Args& a, int, int)
     {
-        if (a.flags[0] == 0u) return;
+        // consume the rebuild request: flags[2] tells the rest of this step's
+        // build kernels and the tile pass whether they rebuild
+        __shared__ int go;
+        if (threadIdx.x == 0) {
+            go = a.flags[0] != 0u;
+            a.flags[2] = (uint32_t)go;
+            if (go) a.flags[0] = 0u;
+        }
+        __syncthreads();
+        if (!go) return;
         __shared__ int32_t wsum[16];
         __shared__ int32_t carry;
         const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -159,7 +168,7 @@ struct Scatter {
     };
     __device__ static void run(const Args& a, int blk, int)
     {
-        if (a.flags[0] == 0u) return;
+        if (a.flags[2] == 0u) return;
         const int i = blk * kBlock + threadIdx.x;
         if (i >= a.n) return;
         a.perm[a.start[a.atom_cell[i]] + a.atom_rank[i]] = i;
@@ -186,7 +195,7 @@ struct CellSort {
     };
     __device__ static void run(const Args& a, int blk, int)
     {
-        if (a.flags[0] == 0u) return;
+        if (a.flags[2] == 0u) return;
         __shared__ int32_t buf[kCellSortMax];
         const int c = blk;
         const int lane = threadIdx.x;
@@ -219,134 +228,8 @@ struct CellSort {
 };
 
 // ---------------------------------------------------------------- Verlet list
-// One workgroup per (home cell, part): the tile's reference positions are
-// staged in LDS already moved to the home cell's image, x_j - s (s the image
-// shift of j's stencil cell), then one wave per home atom i sweeps the whole
-// tile 64 candidates at a time.  A candidate is kept iff
-// (r_list^2 - d2 > 1e-4) && (d2 > 1e-4) with dx = (x_j - s) - x_i and
-// d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the reference's operation order
-// (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  Hits are
-// compacted with ballot + mbcnt into the row as uint16 tile-local indices:
-// LJ-range hits (d2 < r_in^2) from the row start, the others from the row end.
-// Tiles larger than the LDS stage read candidates from global memory; tiles
-// beyond the uint16 range mark their rows overflowed.
-__device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict__ xs_ref, int u)
-{
-#pragma clang fp contract(off)
-    const int st = tile_seg(h, u);
-    const double4 p = xs_ref[h.gstart[st] + (u - h.off[st])];
-    return make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
-}
-
-template <bool STAGED>
-__device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ lds,
-                                 const double4* __restrict__ xs_ref, int first, int a1, int h0,
-                                 int lane, double rl2, double rin2, int cap,
-                                 uint16_t* __restrict__ nlist, int2* __restrict__ ncount,
-                                 unsigned long long* __restrict__ stats)
-{
-#pragma clang fp contract(off)
-    const int total = h.off[27];
-    for (int s = first; s < a1; s += kTileWaves) {
-        if (total > 65535) {
-            if (lane == 0) {
-                ncount[s] = make_int2(cap + 1, 0);
-                atomicAdd(&stats[2], 1ull);
-                atomicMax(&stats[4], (unsigned long long)(cap + 1));
-            }
-            continue;
-        }
-        const int ui = h.off[kHomeSeg] + (s - h0);
-        const double4 pi = STAGED ? lds[ui] : xs_ref[s];
-        int n_in = 0, n_out = 0;
-        uint16_t* __restrict__ row = nlist + (size_t)s * cap;
-        for (int ub = 0; ub < total; ub += 128) {
-            double4 pj[2];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int u = ub + 64 * k + lane;
-                const int uu = u < total ? u : 0;
-                pj[k] = STAGED ? lds[uu] : stage_ref(h, xs_ref, uu);
-            }
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int u = ub + 64 * k + lane;
-                const double dx = pj[k].x - pi.x;
-                const double dy = pj[k].y - pi.y;
-                const double dz = pj[k].z - pi.z;
-                const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                const bool hit = (u < total) && ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
-                const bool inner = d2 < rin2;
-                const uint64_t mi = __ballot(hit && inner);
-                const uint64_t mo = __ballot(hit && !inner);
-                if (hit) {
-                    if (inner) {
-                        const int p = n_in + (int)mbcnt64(mi);
-                        if (p < cap) row[p] = (uint16_t)u;
-                    } else {
-                        const int p = n_out + (int)mbcnt64(mo);
-                        if (p < cap) row[cap - 1 - p] = (uint16_t)u;
-                    }
-                }
-                n_in += __popcll(mi);
-                n_out += __popcll(mo);
-            }
-        }
-        if (lane == 0) {
-            ncount[s] = make_int2(n_in, n_out);
-            if (n_in + n_out > cap) {
-                atomicAdd(&stats[2], 1ull);
-                atomicMax(&stats[4], (unsigned long long)(n_in + n_out));
-            }
-        }
-    }
-}
-
-struct ListTile {
-    static constexpr int kThreads = kTileThreads;
-    struct Args {
-        const double4* xs_ref;
-        const int32_t* cell_start;
-        int split;
-        int3 dims;
-        Box box;
-        double rl2, rin2;
-        int cap, stage_cap;
-        uint16_t* nlist;
-        int2* ncount;
-        const uint32_t* flags;
-        unsigned long long* stats;
-        int32_t* big_tiles;
-    };
-    __device__ static void run(const Args& a, int blk, int)
-    {
-        if (a.flags[0] == 0u) return;
-        extern __shared__ double4 lds[];
-        __shared__ TileHdr h;
-        const int c = blk / a.split, part = blk % a.split;
-        tile_setup(c, a.dims, a.box, a.cell_start, h);
-        const int total = h.off[27];
-        const bool staged = total <= a.stage_cap;
-        if (!staged && part == 0 && threadIdx.x == 0)
-            a.big_tiles[atomicAdd(&a.stats[6], 1ull)] = c;  // served by the global-memory pair pass
-        if (staged) {
-            for (int u = threadIdx.x; u < total; u += kTileThreads)
-                lds[u] = stage_ref(h, a.xs_ref, u);
-        }
-        __syncthreads();
-        const int lane = threadIdx.x & 63;
-        const int wv = uniform(threadIdx.x >> 6);
-        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
-        const int a0 = h0 + (hc * part) / a.split, a1 = h0 + (hc * (part + 1)) / a.split;
-        if (staged)
-            list_wave<true>(h, lds, a.xs_ref, a0 + wv, a1, h0, lane, a.rl2, a.rin2, a.cap,
-                            a.nlist, a.ncount, a.stats);
-        else
-            list_wave<false>(h, lds, a.xs_ref, a0 + wv, a1, h0, lane, a.rl2, a.rin2, a.cap,
-                             a.nlist, a.ncount, a.stats);
-    }
-};
-
+// (row builder list_wave and stage_ref live in tile.h, shared with the fused
+// list + pair pass)
 // List statistics on demand: entries, LJ-range entries, longest row.
 __global__ __launch_bounds__(256) void k_list_stats(const int2* __restrict__ ncount, int n,
                                                     unsigned long long* __restrict__ out)
@@ -382,7 +265,7 @@ __global__ __launch_bounds__(256) void k_list_stats(const int2* __restrict__ nco
 // ---------------------------------------------------------------- gather
 // Current positions into the sorted packed array, unwrapped against the
 // reference positions (x_ref + min_image(x - x_ref)), so that the image of
-// every list pair stays the one seen at the build.  Consumes the rebuild flag.
+// every list pair stays the one seen at the build.
 struct Gather {
     static constexpr int kThreads = kBlock;
     struct Args {
@@ -391,12 +274,10 @@ struct Gather {
         int n;
         Box box;
         double4* xs;
-        uint32_t* flags;
     };
     __device__ static void run(const Args& a, int blk, int)
     {
         const int i = blk * kBlock + threadIdx.x;
-        if (i == 0) a.flags[0] = 0u;
         if (i >= a.n) return;
         const double p = a.xr[i], b = a.yr[i], c = a.zr[i];
         const double dx = min_image(a.x[i] - p, a.box.L[0], a.box.invL[0]);
@@ -558,17 +439,9 @@ CellSort::Args cellsort_args(Engine& e)
                           e.yr,   e.zr, e.q, e.type, e.xs_ref, e.xs, e.scell, e.ts, e.flags};
 }
 
-ListTile::Args list_args(Engine& e)
-{
-    return ListTile::Args{e.xs_ref,   e.cell_start, e.split,   make_int3(e.dims[0], e.dims[1], e.dims[2]),
-                          e.box,      e.r_list * e.r_list, e.r_in * e.r_in, e.cap, e.stage_cap,
-                          e.nlist,    e.ncount, e.flags, e.stats, e.big_tiles};
-}
-
 Gather::Args gather_args(Engine& e, const double* x, const double* y, const double* z)
 {
-    return Gather::Args{x, y, z, e.xr, e.yr, e.zr, e.q, e.atom_rank, (int)e.n, e.box, e.xs,
-                        e.flags};
+    return Gather::Args{x, y, z, e.xr, e.yr, e.zr, e.q, e.atom_rank, (int)e.n, e.box, e.xs};
 }
 
 template <typename T>
@@ -632,11 +505,7 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
         ScopedTimer t(e, P_CELLSORT, s);
         k_one<CellSort><<<e.ncell, 64, 0, s>>>(cellsort_args(e));
     }
-    {
-        ScopedTimer t(e, P_LIST, s);
-        k_one<ListTile><<<e.ncell * e.split, kTileThreads, (size_t)e.stage_cap * sizeof(double4),
-                          s>>>(list_args(e));
-    }
+    // the Verlet rows are built inside the tile pass (pair.hip)
 }
 
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s)
@@ -698,11 +567,9 @@ static hipError_t nlist_prepare_t(Batch& b)
     std::vector<Scan::Args> sc(R);
     std::vector<Scatter::Args> sp(R);
     std::vector<CellSort::Args> cs(R);
-    std::vector<ListTile::Args> li(R);
     std::vector<Gather::Args> ga(R);
     std::vector<typename Kick<T>::Args> ki(R);
-    std::vector<int> nkd(R), nat(R), none(R, 1), ncell(R), nlist(R);
-    size_t lds = 0;
+    std::vector<int> nkd(R), nat(R), none(R, 1), ncell(R);
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
         const RepState& st = b.st[r];
@@ -712,15 +579,11 @@ static hipError_t nlist_prepare_t(Batch& b)
         sc[r] = scan_args(e);
         sp[r] = scatter_args(e);
         cs[r] = cellsort_args(e);
-        li[r] = list_args(e);
         ga[r] = gather_args(e, st.x, st.y, st.z);
         ki[r] = kick_args<T>(e, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz, st.mass, b.dt);
         nkd[r] = kickdrift_blocks(e, b.sa[r]);
         nat[r] = nblk(e.n > 0 ? e.n : 1);
         ncell[r] = e.ncell;
-        nlist[r] = e.ncell * e.split;
-        const size_t l = (size_t)e.stage_cap * sizeof(double4);
-        lds = l > lds ? l : lds;
     }
     hipError_t err;
     if ((err = phase_upload(b.ph[BP_KICKDRIFT], kd, nkd, 0)) != hipSuccess) return err;
@@ -728,7 +591,6 @@ static hipError_t nlist_prepare_t(Batch& b)
     if ((err = phase_upload(b.ph[BP_SCAN], sc, none, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_SCATTER], sp, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_CELLSORT], cs, ncell, 0)) != hipSuccess) return err;
-    if ((err = phase_upload(b.ph[BP_LIST], li, nlist, lds)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_GATHER], ga, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_KICK], ki, nat, 0)) != hipSuccess) return err;
     return hipSuccess;
@@ -751,7 +613,6 @@ void nlist_batch_launch(Batch& b, int phase, hipStream_t s)
     case BP_SCAN: phase_launch<Scan>(b.ph[phase], s); break;
     case BP_SCATTER: phase_launch<Scatter>(b.ph[phase], s); break;
     case BP_CELLSORT: phase_launch<CellSort>(b.ph[phase], s); break;
-    case BP_LIST: phase_launch<ListTile>(b.ph[phase], s); break;
     case BP_GATHER: phase_launch<Gather>(b.ph[phase], s); break;
     case BP_KICK:
         if (f32) phase_launch<Kick<float>>(b.ph[phase], s);

@@ -313,37 +313,67 @@ __device__ inline void block_ev(const double* acc, int lane, int wv, double* __r
     }
 }
 
-// Staged pass: one workgroup per (tile, part); tiles larger than the LDS stage
-// are skipped here and served by PairBig.
+// Tile pass: one workgroup per (cell tile, part).  On a rebuild step it first
+// builds the Verlet rows of its home atoms (list_wave over the tile's reference
+// positions), then the pair pass: the tile's current positions are staged in
+// LDS shifted to the home cell's image (and types when there are several);
+// each 16-lane quarter-wave walks one home atom's uint16 row -- LJ-range
+// entries first, so later rounds skip the LJ code -- then reduces across its 16
+// lanes.  Tiles larger than the LDS stage take the same steps from global
+// memory.  The rows of a home atom are written and read by the same workgroup,
+// so the list build and the pair pass need no launch in between.
 template <typename T, bool EV, bool MULTI>
-struct PairTile {
+struct TilePass {
     static constexpr int kThreads = kTileThreads;
     struct Args {
         PairArgs a;
+        const double4* xs_ref;
         const int32_t* cell_start;
         int split;
         int3 dims;
         Box box;
         int stage_cap;
+        double rl2, rin2;
         PairScalars ps;
         const PairParams* pp;
         T *fx, *fy, *fz;
         double* ev_part;
-        uint32_t* flags;
+        const uint32_t* flags;
+        unsigned long long* stats;
+        uint16_t* nlist;
+        int2* ncount;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
         const PairArgs& a = A.a;
-        // the list of this step is built: consume the rebuild request
-        if (blk == 0 && threadIdx.x == 0) A.flags[0] = 0u;
         extern __shared__ double4 lds[];
         int32_t* ltype = (int32_t*)(lds + A.stage_cap);
         __shared__ TileHdr h;
         const int c = blk / A.split, part = blk % A.split;
+        const bool rebuild = A.flags[2] != 0u;
         tile_setup(c, A.dims, A.box, A.cell_start, h);
         const int total = h.off[27];
-        const bool mine = total <= A.stage_cap;
-        if (mine) {
+        const bool staged = total <= A.stage_cap;
+        const int lane = threadIdx.x & 63;
+        const int wv = uniform(threadIdx.x >> 6);
+        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
+        const int a0 = h0 + (hc * part) / A.split, a1 = h0 + (hc * (part + 1)) / A.split;
+        if (rebuild) {
+            if (!staged && part == 0 && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
+            if (staged) {
+                for (int u = threadIdx.x; u < total; u += kTileThreads)
+                    lds[u] = stage_ref(h, A.xs_ref, u);
+            }
+            __syncthreads();
+            if (staged)
+                list_wave<true>(h, lds, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                                A.nlist, A.ncount, A.stats);
+            else
+                list_wave<false>(h, lds, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                                 A.nlist, A.ncount, A.stats);
+            __syncthreads();  // rows and LDS are reused below
+        }
+        if (staged) {
             for (int u = threadIdx.x; u < total; u += kTileThreads) {
                 const int st = tile_seg(h, u);
                 const int j = h.gstart[st] + (u - h.off[st]);
@@ -353,50 +383,13 @@ struct PairTile {
             }
         }
         __syncthreads();
-        const int lane = threadIdx.x & 63;
-        const int wv = uniform(threadIdx.x >> 6);
-        const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
-        const int a0 = h0 + (hc * part) / A.split, a1 = h0 + (hc * (part + 1)) / A.split;
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (mine)
+        if (staged)
             pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
                                           A.fx, A.fy, A.fz, acc);
-        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
-    }
-};
-
-// Global-memory pass over the tiles the list build found larger than the LDS
-// stage (big_tiles[0 .. stats[6])), a fixed small grid looping over them.
-template <typename T, bool EV, bool MULTI>
-struct PairBig {
-    static constexpr int kThreads = kTileThreads;
-    struct Args {
-        PairArgs a;
-        const int32_t* cell_start;
-        const int32_t* big_tiles;
-        const unsigned long long* stats;
-        int3 dims;
-        Box box;
-        PairScalars ps;
-        const PairParams* pp;
-        T *fx, *fy, *fz;
-        double* ev_part;
-    };
-    __device__ static void run(const Args& A, int blk, int nblk)
-    {
-        __shared__ TileHdr h;
-        const int nbig = (int)A.stats[6];
-        const int lane = threadIdx.x & 63;
-        const int wv = uniform(threadIdx.x >> 6);
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int b = blk; b < nbig; b += nblk) {
-            const int c = A.big_tiles[b];
-            __syncthreads();
-            tile_setup(c, A.dims, A.box, A.cell_start, h);
-            const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
-            pair_wave<T, EV, MULTI, false>(A.a, h, nullptr, nullptr, h0, h0 + hc, h0, wv, lane,
-                                           A.ps, A.pp, A.fx, A.fy, A.fz, acc);
-        }
+        else
+            pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
+                                           A.pp, A.fx, A.fy, A.fz, acc);
         if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
     }
 };
@@ -454,22 +447,13 @@ PairScalars pair_scalars(const Engine& e)
 }
 
 template <typename T, bool EV, bool M>
-typename PairTile<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void* fz)
+typename TilePass<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void* fz)
 {
-    return typename PairTile<T, EV, M>::Args{
+    return typename TilePass<T, EV, M>::Args{
         PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
-        e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box, e.stage_cap,
-        pair_scalars(e), e.d_pp, (T*)fx, (T*)fy, (T*)fz, e.ev_part, e.flags};
-}
-
-template <typename T, bool EV, bool M>
-typename PairBig<T, EV, M>::Args big_args(Engine& e, void* fx, void* fy, void* fz)
-{
-    return typename PairBig<T, EV, M>::Args{
-        PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
-        e.cell_start, e.big_tiles, e.stats, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box,
-        pair_scalars(e), e.d_pp, (T*)fx, (T*)fy, (T*)fz,
-        e.ev_part + (size_t)e.ncell * e.split * 8};
+        e.xs_ref, e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box,
+        e.stage_cap, e.r_list * e.r_list, e.r_in * e.r_in, pair_scalars(e), e.d_pp, (T*)fx,
+        (T*)fy, (T*)fz, e.ev_part, e.flags, e.stats, e.nlist, e.ncount};
 }
 
 size_t pair_lds(const Engine& e)
@@ -480,31 +464,25 @@ size_t pair_lds(const Engine& e)
 template <typename T, bool EV, bool M>
 void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
 {
-    k_one<PairTile<T, EV, M>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
+    k_one<TilePass<T, EV, M>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
         tile_args<T, EV, M>(e, fx, fy, fz));
-    k_one<PairBig<T, EV, M>><<<e.big_grid, kTileThreads, 0, s>>>(big_args<T, EV, M>(e, fx, fy, fz));
 }
 
 template <typename T, bool M>
 hipError_t pair_prepare_t(Batch& b)
 {
     const size_t R = b.es.size();
-    std::vector<typename PairTile<T, false, M>::Args> ta(R);
-    std::vector<typename PairBig<T, false, M>::Args> ba(R);
-    std::vector<int> nt(R), nb(R);
+    std::vector<typename TilePass<T, false, M>::Args> ta(R);
+    std::vector<int> nt(R);
     size_t lds = 0;
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
         ta[r] = tile_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz);
-        ba[r] = big_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz);
         nt[r] = e.ncell * e.split;
-        nb[r] = e.big_grid;
         const size_t l = pair_lds(e);
         lds = l > lds ? l : lds;
     }
-    hipError_t err;
-    if ((err = phase_upload(b.ph[BP_PAIR], ta, nt, lds)) != hipSuccess) return err;
-    return phase_upload(b.ph[BP_PAIR_BIG], ba, nb, 0);
+    return phase_upload(b.ph[BP_PAIR], ta, nt, lds);
 }
 
 }  // namespace
@@ -532,7 +510,7 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s)
 {
     ScopedTimer t(e, P_EV_FINAL, s);
-    const int nb = with_pair ? e.ncell * e.split + e.big_grid : 0;  // staged + big-tile kernels
+    const int nb = with_pair ? e.ncell * e.split : 0;  // one partial per tile workgroup
     const int P = (e.kspace != 0) ? e.P : 0;
     k_ev_final<<<1, 256, 0, s>>>(e.ev_part, nb, e.kev, P, e.e_self, e.ev);
 }
@@ -547,8 +525,7 @@ void pair_batch_launch(Batch& b, int phase, hipStream_t s)
 {
 #define RBMD_PB(T, M)                                                                         \
     do {                                                                                      \
-        if (phase == BP_PAIR) phase_launch<PairTile<T, false, M>>(b.ph[phase], s);           \
-        else if (phase == BP_PAIR_BIG) phase_launch<PairBig<T, false, M>>(b.ph[phase], s);   \
+        if (phase == BP_PAIR) phase_launch<TilePass<T, false, M>>(b.ph[phase], s);           \
     } while (0)
     if (b.precision == 32) {
         if (b.multi) RBMD_PB(float, true);

@@ -73,9 +73,8 @@ struct Engine {
     int32_t *cell_count = nullptr, *cell_start = nullptr, *atom_cell = nullptr,
             *atom_rank = nullptr;
     uint16_t* nlist = nullptr;  // tile-local indices
-    int32_t* big_tiles = nullptr;  // tiles larger than the LDS stage (count in stats[6])
     int2* ncount = nullptr;
-    uint32_t* flags = nullptr;              // [0] rebuild request, [1] NaN seen
+    uint32_t* flags = nullptr;  // [0] rebuild request, [1] NaN seen, [2] rebuilding this step
     unsigned long long* stats = nullptr;    // [0] builds [1] entries [2] overflow rows [3] inner [4] longest overflowing row [6] big tiles [8..10] list stats
     double* q = nullptr;
     int32_t* type = nullptr;
@@ -108,7 +107,6 @@ struct Engine {
 
     // energy / virial
     double* ev_part = nullptr;  // [blocks][8]
-    int big_grid = 64;          // workgroups of the big-tile pair pass
     int ev_part_blocks = 0;
     double* ev = nullptr;       // [10]
 
@@ -156,12 +154,10 @@ enum BatchPh {
     BP_SCAN,
     BP_SCATTER,
     BP_CELLSORT,
-    BP_LIST,
     BP_GATHER,
     BP_RHO,
     BP_RHO_FINAL,
     BP_PAIR,
-    BP_PAIR_BIG,
     BP_KFORCE,
     BP_KICK,
     BP_NUM

@@ -80,6 +80,90 @@ __device__ inline int tile_seg(const TileHdr& h, int u)
     }
     return lo;
 }
+// ---------------------------------------------------------------- Verlet list
+// One workgroup per (home cell, part): the tile's reference positions are
+// staged in LDS already moved to the home cell's image, x_j - s (s the image
+// shift of j's stencil cell), then one wave per home atom i sweeps the whole
+// tile 64 candidates at a time.  A candidate is kept iff
+// (r_list^2 - d2 > 1e-4) && (d2 > 1e-4) with dx = (x_j - s) - x_i and
+// d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the reference's operation order
+// (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  Hits are
+// compacted with ballot + mbcnt into the row as uint16 tile-local indices:
+// LJ-range hits (d2 < r_in^2) from the row start, the others from the row end.
+// Tiles larger than the LDS stage read candidates from global memory; tiles
+// beyond the uint16 range mark their rows overflowed.
+__device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict__ xs_ref, int u)
+{
+#pragma clang fp contract(off)
+    const int st = tile_seg(h, u);
+    const double4 p = xs_ref[h.gstart[st] + (u - h.off[st])];
+    return make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+}
+
+template <bool STAGED>
+__device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ lds,
+                                 const double4* __restrict__ xs_ref, int first, int a1, int h0,
+                                 int lane, double rl2, double rin2, int cap,
+                                 uint16_t* __restrict__ nlist, int2* __restrict__ ncount,
+                                 unsigned long long* __restrict__ stats)
+{
+#pragma clang fp contract(off)
+    const int total = h.off[27];
+    for (int s = first; s < a1; s += kTileWaves) {
+        if (total > 65535) {
+            if (lane == 0) {
+                ncount[s] = make_int2(cap + 1, 0);
+                atomicAdd(&stats[2], 1ull);
+                atomicMax(&stats[4], (unsigned long long)(cap + 1));
+            }
+            continue;
+        }
+        const int ui = h.off[kHomeSeg] + (s - h0);
+        const double4 pi = STAGED ? lds[ui] : xs_ref[s];
+        int n_in = 0, n_out = 0;
+        uint16_t* __restrict__ row = nlist + (size_t)s * cap;
+        for (int ub = 0; ub < total; ub += 128) {
+            double4 pj[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = ub + 64 * k + lane;
+                const int uu = u < total ? u : 0;
+                pj[k] = STAGED ? lds[uu] : stage_ref(h, xs_ref, uu);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = ub + 64 * k + lane;
+                const double dx = pj[k].x - pi.x;
+                const double dy = pj[k].y - pi.y;
+                const double dz = pj[k].z - pi.z;
+                const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const bool hit = (u < total) && ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
+                const bool inner = d2 < rin2;
+                const uint64_t mi = __ballot(hit && inner);
+                const uint64_t mo = __ballot(hit && !inner);
+                if (hit) {
+                    if (inner) {
+                        const int p = n_in + (int)mbcnt64(mi);
+                        if (p < cap) row[p] = (uint16_t)u;
+                    } else {
+                        const int p = n_out + (int)mbcnt64(mo);
+                        if (p < cap) row[cap - 1 - p] = (uint16_t)u;
+                    }
+                }
+                n_in += __popcll(mi);
+                n_out += __popcll(mo);
+            }
+        }
+        if (lane == 0) {
+            ncount[s] = make_int2(n_in, n_out);
+            if (n_in + n_out > cap) {
+                atomicAdd(&stats[2], 1ull);
+                atomicMax(&stats[4], (unsigned long long)(n_in + n_out));
+            }
+        }
+    }
+}
+
 #endif
 
 }  // namespace rbmd

@@ -167,8 +167,9 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
 /* Per-kernel device time (HIP events on the launch stream).  enable != 0
  * starts recording around every launch of the named passes; get returns the
  * mean milliseconds per launch and the launch count for one pass name:
- * "check", "bin", "scan", "scatter", "cellsort", "list", "gather", "rho",
- * "rho_final", "pair", "kforce", "kick", "ev_final". */
+ * "check" (also the MD kick/drift kernel), "bin", "scan", "scatter", "cellsort",
+ * "gather", "sample", "rho", "rho_final", "pair" (the tile pass: Verlet rows on rebuild
+ * steps, then the pair forces), "kforce", "kick", "ev_final". */
 int rbmd_force_set_timing(rbmd_force_t h, int32_t enable);
 int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int64_t* launches);
 
