@@ -104,6 +104,7 @@ static void free_all(Engine& E)
 {
     void* ptrs[] = {E.xr,       E.yr,    E.zr,     E.xs_ref, E.xs,        E.perm,     E.scell,
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
+                    E.md_count, E.md_bucket, E.md_cell, E.md_w,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
                     E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide};
@@ -325,6 +326,16 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.atom_cell, N * 4));
     CR(dalloc(&E.atom_rank, N * 4));
     CR(dalloc(&E.cell_count, (size_t)E.ncell * 4));
+    {
+        // per-cell buckets of the MD step's binning: 4x the mean population
+        // (a cell beyond it is gathered by a scan instead)
+        const double mean = (double)N / (double)E.ncell;
+        E.bucket_cap = std::min(2048, std::max(64, ((int)(4.0 * mean) + 32 + 63) / 64 * 64));
+        CR(dalloc(&E.md_count, (size_t)2 * E.ncell * 4));
+        CR(dalloc(&E.md_bucket, (size_t)2 * E.ncell * E.bucket_cap * 4));
+        CR(dalloc(&E.md_cell, N * 4));
+        CR(dalloc(&E.md_w, N * 3 * 8));
+    }
     CR(dalloc(&E.cell_start, ((size_t)E.ncell + 1) * 4));
     CR(dalloc(&E.ncount, N * sizeof(int2)));
     CR(dalloc(&E.flags, 64));
@@ -344,6 +355,8 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(hipMemset(E.d_step, 0, 64));
     CR(hipMemset(E.ev, 0, 16 * 8));
     CR(hipMemset(E.cell_start, 0, ((size_t)E.ncell + 1) * 4));
+    CR(hipMemset(E.cell_count, 0, (size_t)E.ncell * 4));
+    CR(hipMemset(E.md_count, 0, (size_t)2 * E.ncell * 4));
     CR(hipMemset(E.xr, 0, N * 8));
     CR(hipMemset(E.yr, 0, N * 8));
     CR(hipMemset(E.zr, 0, N * 8));
@@ -496,7 +509,8 @@ int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_
 
 // One force evaluation; with kick pointers (MD mode) the k-space kernel also
 // applies the closing half kick, otherwise the caller kicks separately.  In MD
-// mode (checked) the kick/drift kernel has already gathered the positions.
+// mode (checked) the kick/drift kernel has already checked, gathered and binned
+// the positions, so one cell-sort kernel does a rebuild.
 // One stream: running the structure factor on a second stream concurrently with
 // the list build and pair pass measured slower (cross-queue joins cost more than
 // the overlap gains at these kernel sizes).
@@ -507,10 +521,14 @@ static int force_pass(Engine& E, const double* x, const double* y, const double*
 {
     if (!checked) launch_check(E, x, y, z, s);
     const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
-    launch_build(E, x, y, z, s);
-    if (!checked) launch_gather(E, x, y, z, s);
+    if (checked) {
+        launch_cellsort_md(E, x, y, z, s);  // the kick kernel binned and gathered
+    } else {
+        launch_build(E, x, y, z, s);
+        launch_gather(E, x, y, z, s);
+    }
     if (ks) launch_rho(E, x, y, z, s);
-    launch_pair(E, fx, fy, fz, ev, s);
+    launch_pair(E, fx, fy, fz, ev, s, /*md=*/checked);
     if (ks) launch_kforce(E, x, y, z, fx, fy, fz, vx, vy, vz, mass, dt, s);
     if (ev) launch_ev_final(E, true, s);
     return RBMD_OK;
@@ -611,6 +629,9 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     }
     int rc = maintain(E, s);
     if (rc != RBMD_OK) return rc;
+    // bin the current positions for a rebuild in the first step (later steps'
+    // bins come from the tile pass of the step before)
+    launch_md_bin(E, x, y, z, s);
     if (!use_graph) {
         for (int64_t k = 0; k < nsteps; ++k) {
             const uint64_t st = step0 + (uint64_t)k;
@@ -994,13 +1015,13 @@ static int bfail(rbmd_batch_t b, int code, const std::string& msg)
     } while (0)
 
 // One batched velocity-Verlet step: every phase is one launch over all replicas.
-// The kick/drift kernel gathers the positions.
+// The kick/drift kernel gathers and bins the positions.
 static void batch_step(rbmd_batch_t b, hipStream_t s)
 {
     Batch& B = b->b;
     const bool ks = B.kspace == RBMD_KSPACE_RBE || B.kspace == RBMD_KSPACE_EWALD;
     nlist_batch_launch(B, BP_KICKDRIFT, s);
-    for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT}) nlist_batch_launch(B, ph, s);
+    nlist_batch_launch(B, BP_CELLSORT_MD, s);
     if (ks) {
         kspace_batch_launch(B, BP_RHO, s);
         kspace_batch_launch(B, BP_RHO_FINAL, s);
@@ -1128,6 +1149,10 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     BHIP_TRY(nlist_batch_prepare(B));
     BHIP_TRY(pair_batch_prepare(B));
     if (B.kspace != RBMD_KSPACE_NONE) BHIP_TRY(kspace_batch_prepare(B));
+    // bins of the current positions for a rebuild in the first step
+    for (Engine* E : B.es)
+        BHIP_TRY(hipMemsetAsync(E->md_count, 0, (size_t)2 * E->ncell * sizeof(int32_t), s));
+    nlist_batch_launch(B, BP_MDBIN, s);
     if (!use_graph) {
         for (int64_t k = 0; k < nsteps; ++k) batch_step(b, s);
         BHIP_TRY(hipGetLastError());

@@ -41,6 +41,19 @@ struct PairParams {
     double sig2[kMaxTypes * kMaxTypes];   // ((sig_i + sig_j)/2)^2
 };
 
+// Bins of the MD step's next rebuild, double-buffered by the step epoch
+// (flags[3]): the tile pass of a step bins its home atoms' current positions into
+// buffer (epoch + 1) & 1 and clears buffer epoch & 1, which the cell sort of the
+// same step has consumed; the k-space force (or kick) kernel then advances the
+// epoch.
+struct MdBins {
+    int32_t* count;        // [2][ncell]
+    int32_t* bucket;       // [2][ncell][cap] atom ids
+    int32_t* cell;         // [n] cell of each atom at its binning
+    double *xw, *yw, *zw;  // [n] wrapped positions at the binning
+    int ncell, cap;
+};
+
 // ---------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
 // SC'11.  Round: (hi0,lo0) = M0*c0, (hi1,lo1) = M1*c2,

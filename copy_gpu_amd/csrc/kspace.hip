@@ -291,10 +291,12 @@ struct KForce {
         double *vx, *vy, *vz;
         const double* mass;
         double dt;
+        uint32_t* flags;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
 #pragma clang fp contract(off)
+        if (KICK && blk == 0 && threadIdx.x == 0) A.flags[3] += 1u;  // MD step epoch
         __shared__ KCoef<T> st[kKfStage];
         __shared__ T red[kKfSlices][3][kKfAtoms];
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -380,7 +382,7 @@ typename KForce<T, K>::Args kforce_args(Engine& e, const double* x, const double
 {
     return typename KForce<T, K>::Args{x,        y,      z,        e.q,    (int)e.n, e.box,
                                        (const KCoef<T>*)e.kcoef, e.P, (T*)fx, (T*)fy, (T*)fz,
-                                       vx,       vy,     vz,       mass,   dt};
+                                       vx,       vy,     vz,       mass,   dt,       e.flags};
 }
 
 template <typename T>

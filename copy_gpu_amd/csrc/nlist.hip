@@ -54,9 +54,7 @@ struct Check {
 __global__ void k_set_flag(uint32_t* flags) { flags[0] = 1u; }
 
 // ---------------------------------------------------------------- binning
-// Wrap into the box (x - L floor((x-lo)/L), no contraction: the CPU oracle
-// computes the same bits), cell = trunc((x-lo)/cell) clamped, rank inside the
-// cell from an atomic histogram.
+// Rank inside the cell from an atomic histogram.
 struct Bin {
     static constexpr int kThreads = kBlock;
     struct Args {
@@ -71,28 +69,15 @@ struct Bin {
     };
     __device__ static void run(const Args& a, int blk, int)
     {
-#pragma clang fp contract(off)
         if (a.flags[0] == 0u) return;
         const int i = blk * kBlock + threadIdx.x;
         if (i >= a.n) return;
         const double p[3] = {a.x[i], a.y[i], a.z[i]};
-        const int dm[3] = {a.dims.x, a.dims.y, a.dims.z};
-        const double cs[3] = {a.cell.x, a.cell.y, a.cell.z};
         double w[3];
-        int c[3];
-        for (int d = 0; d < 3; ++d) {
-            double f = floor((p[d] - a.box.lo[d]) / a.box.L[d]);
-            double fl = f * a.box.L[d];
-            w[d] = p[d] - fl;
-            int v = (int)((w[d] - a.box.lo[d]) / cs[d]);
-            v = v < 0 ? 0 : v;
-            v = v > dm[d] - 1 ? dm[d] - 1 : v;
-            c[d] = v;
-        }
+        const int cid = wrap_cell(p, a.box, a.dims, a.cell, w);
         a.xr[i] = w[0];
         a.yr[i] = w[1];
         a.zr[i] = w[2];
-        const int cid = c[0] + c[1] * a.dims.x + c[2] * a.dims.x * a.dims.y;
         a.atom_cell[i] = cid;
         a.atom_rank[i] = atomicAdd(&a.cell_count[cid], 1);
     }
@@ -227,6 +212,125 @@ struct CellSort {
     }
 };
 
+// MD-step binning before the first step of a run: bins the current positions
+// into buffer (epoch & 1) of the MD bins (which the caller cleared), as the tile
+// pass of a previous step would have.
+struct MdBin {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        const double *x, *y, *z;
+        int n;
+        Box box;
+        int3 dims;
+        double3 cell;
+        MdBins bins;
+        const uint32_t* flags;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i >= a.n) return;
+        const double p[3] = {a.x[i], a.y[i], a.z[i]};
+        md_bin_atom(a.bins, (int)(a.flags[3] & 1u), i, p, a.box, a.dims, a.cell);
+    }
+};
+
+// MD-step cell sort.  The tile pass of the previous step (or the binning before
+// the run) binned every atom into per-cell buckets of buffer epoch & 1, with its
+// wrapped position at that time.  On a rebuild step one wave takes one cell: its
+// start is the prefix sum of the counts of the cells before it (fixed order),
+// its atoms are the bucket's ids ordered by id -- or, when the bucket
+// overflowed, every atom binned to the cell, found by a scan in id order -- and
+// it writes the cell bounds, both permutations, the reference positions (the
+// binned ones: every atom is within skin/2 of them, which is what the list
+// needs), the sorted reference positions, the current positions unwrapped
+// against the new reference, charges, types and cells.  This replaces the
+// binning, scan and scatter kernels in the MD step.  Workgroup 0 records
+// whether this step rebuilds (flags[2]) for the tile pass, which then clears the
+// request.
+struct CellSortMD {
+    static constexpr int kThreads = 64;
+    struct Args {
+        const double *x, *y, *z, *q;
+        const int32_t* type;
+        int n, ncell;
+        Box box;
+        MdBins bins;
+        int32_t *cell_start, *perm, *inv_perm, *scell, *ts;
+        double *xr, *yr, *zr;
+        double4 *xs_ref, *xs;
+        uint32_t* flags;
+        unsigned long long* stats;
+    };
+    __device__ static void emit(const Args& a, int c, int v, int s)
+    {
+        a.perm[s] = v;
+        a.inv_perm[v] = s;
+        const double w0 = a.bins.xw[v], w1 = a.bins.yw[v], w2 = a.bins.zw[v];
+        a.xr[v] = w0;
+        a.yr[v] = w1;
+        a.zr[v] = w2;
+        const double qv = a.q[v];
+        a.xs_ref[s] = make_double4(w0, w1, w2, qv);
+        a.xs[s] = make_double4(w0 + min_image(a.x[v] - w0, a.box.L[0], a.box.invL[0]),
+                               w1 + min_image(a.y[v] - w1, a.box.L[1], a.box.invL[1]),
+                               w2 + min_image(a.z[v] - w2, a.box.L[2], a.box.invL[2]), qv);
+        a.scell[s] = c;
+        a.ts[s] = a.type ? a.type[v] : 0;
+    }
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const uint32_t req = a.flags[0];
+        if (blk == 0 && threadIdx.x == 0) {
+            a.flags[2] = req;
+            if (req) {
+                a.stats[0] += 1ull;
+                a.stats[1] = 0ull;
+                a.stats[2] = 0ull;
+                a.stats[3] = 0ull;
+                a.stats[4] = 0ull;
+                a.stats[6] = 0ull;
+            }
+        }
+        if (req == 0u) return;
+        __shared__ int32_t buf[kCellSortMax];
+        const int c = blk;
+        const int lane = threadIdx.x;
+        const int b = (int)(a.flags[3] & 1u);
+        const int32_t* count = a.bins.count + (size_t)b * a.ncell;
+        int part = 0;
+        for (int k = lane; k < c; k += 64) part += count[k];
+        const int s0 = wave_sum(part);
+        const int cnt = count[c];
+        if (lane == 0) {
+            a.cell_start[c] = s0;
+            if (c == a.ncell - 1) a.cell_start[a.ncell] = s0 + cnt;
+        }
+        if (cnt <= a.bins.cap) {
+            // bucket ids (arrival order) ranked by id in LDS; cap <= kCellSortMax
+            const int32_t* bk = a.bins.bucket + ((size_t)b * a.ncell + c) * a.bins.cap;
+            for (int k = lane; k < cnt; k += 64) buf[k] = bk[k];
+            __syncthreads();
+            for (int k = lane; k < cnt; k += 64) {
+                const int v = buf[k];
+                int r = 0;
+                for (int t = 0; t < cnt; ++t) r += (buf[t] < v) ? 1 : 0;
+                emit(a, c, v, s0 + r);
+            }
+        } else {
+            // overflowed bucket: every atom binned to the cell, in id order
+            int found = 0;
+            for (int base = 0; base < a.n; base += 64) {
+                const int i = base + lane;
+                const bool hit = i < a.n && a.bins.cell[i] == c;
+                const uint64_t m = __ballot(hit);
+                if (hit) emit(a, c, i, s0 + found + (int)mbcnt64(m));
+                found += __popcll(m);
+            }
+        }
+    }
+};
+
 // ---------------------------------------------------------------- Verlet list
 // (row builder list_wave and stage_ref live in tile.h, shared with the fused
 // list + pair pass)
@@ -291,10 +395,9 @@ struct Gather {
 constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sampler
 
 // First half of the velocity-Verlet step: v += dt/2 F/m, x += v dt, wrap with
-// image flags, displacement check against the list build (and zeroing of the
-// cell histogram for a possible rebuild), and the gather of the new positions
-// into the cell-sorted array.  The first blocks also draw this step's
-// k-vectors (the sampler of k_sample) so that no extra launch is needed.
+// image flags, displacement check against the list build, and the gather of the
+// new positions into the cell-sorted array.  The first blocks also draw this
+// step's k-vectors (the sampler of k_sample) so that no extra launch is needed.
 template <typename T>
 struct KickDrift {
     static constexpr int kThreads = kBlock;
@@ -309,8 +412,6 @@ struct KickDrift {
         const double *xr, *yr, *zr;
         double thr2;
         uint32_t* flags;
-        int32_t* cell_count;
-        int ncell;
         const double* q;
         const int32_t* inv_perm;
         double4* xs;
@@ -320,7 +421,6 @@ struct KickDrift {
     {
         const SampleArgs& sa = a.sa;
         const int i = blk * kBlock + threadIdx.x;
-        if (i < a.ncell) a.cell_count[i] = 0;
         if (blk < (sa.P + 63) / 64) {
             // the first ceil(P/64) blocks also draw this step's k-vectors (64 per
             // block, one wave), with the CDF tables staged in LDS
@@ -388,10 +488,12 @@ struct Kick {
         const double* mass;
         double dt;
         int n;
+        uint32_t* flags;
     };
     __device__ static void run(const Args& a, int blk, int)
     {
         const int i = blk * kBlock + threadIdx.x;
+        if (i == 0) a.flags[3] += 1u;  // step epoch of the MD bins
         if (i >= a.n) return;
         const double h = 0.5 * a.dt / (a.mass ? a.mass[i] : 1.0);
         a.vx[i] += h * (double)a.fx[i];
@@ -453,13 +555,27 @@ typename KickDrift<T>::Args kickdrift_args(Engine& e, double* x, double* y, doub
     const double h = 0.5 * e.skin;
     return typename KickDrift<T>::Args{x,  y,    z,    vx,   vy,  vz, (const T*)fx, (const T*)fy,
                                        (const T*)fz, img, mass, dt, (int)e.n, e.box, e.xr,
-                                       e.yr, e.zr, h * h, e.flags, e.cell_count, e.ncell,
-                                       e.q, e.atom_rank, e.xs, sa};
+                                       e.yr, e.zr, h * h, e.flags, e.q, e.atom_rank, e.xs, sa};
+}
+
+CellSortMD::Args cellsort_md_args(Engine& e, const double* x, const double* y, const double* z)
+{
+    return CellSortMD::Args{x,      y,       z,          e.q,           e.type,
+                            (int)e.n, e.ncell, e.box,    md_bins(e),    e.cell_start,
+                            e.perm, e.atom_rank /* inverse perm */,     e.scell,
+                            e.ts,   e.xr,    e.yr,       e.zr,          e.xs_ref,
+                            e.xs,   e.flags, e.stats};
+}
+
+MdBin::Args mdbin_args(Engine& e, const double* x, const double* y, const double* z)
+{
+    return MdBin::Args{x, y, z, (int)e.n, e.box, make_int3(e.dims[0], e.dims[1], e.dims[2]),
+                       make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e), e.flags};
 }
 
 int kickdrift_blocks(const Engine& e, const SampleArgs& sa)
 {
-    int64_t cover = e.n > e.ncell ? e.n : e.ncell;
+    int64_t cover = e.n;
     if (cover < (int64_t)sa.P * kBlock / 64) cover = (int64_t)sa.P * kBlock / 64;
     if (cover < 1) cover = 1;
     return nblk(cover);
@@ -470,7 +586,7 @@ typename Kick<T>::Args kick_args(Engine& e, double* vx, double* vy, double* vz, 
                                  const void* fy, const void* fz, const double* mass, double dt)
 {
     return typename Kick<T>::Args{vx, vy, vz, (const T*)fx, (const T*)fy, (const T*)fz, mass, dt,
-                                  (int)e.n};
+                                  (int)e.n, e.flags};
 }
 
 }  // namespace
@@ -506,6 +622,19 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
         k_one<CellSort><<<e.ncell, 64, 0, s>>>(cellsort_args(e));
     }
     // the Verlet rows are built inside the tile pass (pair.hip)
+}
+
+void launch_md_bin(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
+{
+    (void)hipMemsetAsync(e.md_count, 0, (size_t)2 * e.ncell * sizeof(int32_t), s);
+    k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z));
+}
+
+void launch_cellsort_md(Engine& e, const double* x, const double* y, const double* z,
+                        hipStream_t s)
+{
+    ScopedTimer t(e, P_CELLSORT, s);
+    k_one<CellSortMD><<<e.ncell, 64, 0, s>>>(cellsort_md_args(e, x, y, z));
 }
 
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s)
@@ -567,6 +696,8 @@ static hipError_t nlist_prepare_t(Batch& b)
     std::vector<Scan::Args> sc(R);
     std::vector<Scatter::Args> sp(R);
     std::vector<CellSort::Args> cs(R);
+    std::vector<CellSortMD::Args> cm(R);
+    std::vector<MdBin::Args> mb(R);
     std::vector<Gather::Args> ga(R);
     std::vector<typename Kick<T>::Args> ki(R);
     std::vector<int> nkd(R), nat(R), none(R, 1), ncell(R);
@@ -579,6 +710,8 @@ static hipError_t nlist_prepare_t(Batch& b)
         sc[r] = scan_args(e);
         sp[r] = scatter_args(e);
         cs[r] = cellsort_args(e);
+        cm[r] = cellsort_md_args(e, st.x, st.y, st.z);
+        mb[r] = mdbin_args(e, st.x, st.y, st.z);
         ga[r] = gather_args(e, st.x, st.y, st.z);
         ki[r] = kick_args<T>(e, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz, st.mass, b.dt);
         nkd[r] = kickdrift_blocks(e, b.sa[r]);
@@ -591,6 +724,8 @@ static hipError_t nlist_prepare_t(Batch& b)
     if ((err = phase_upload(b.ph[BP_SCAN], sc, none, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_SCATTER], sp, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_CELLSORT], cs, ncell, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_CELLSORT_MD], cm, ncell, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_MDBIN], mb, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_GATHER], ga, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_KICK], ki, nat, 0)) != hipSuccess) return err;
     return hipSuccess;
@@ -613,6 +748,8 @@ void nlist_batch_launch(Batch& b, int phase, hipStream_t s)
     case BP_SCAN: phase_launch<Scan>(b.ph[phase], s); break;
     case BP_SCATTER: phase_launch<Scatter>(b.ph[phase], s); break;
     case BP_CELLSORT: phase_launch<CellSort>(b.ph[phase], s); break;
+    case BP_CELLSORT_MD: phase_launch<CellSortMD>(b.ph[phase], s); break;
+    case BP_MDBIN: phase_launch<MdBin>(b.ph[phase], s); break;
     case BP_GATHER: phase_launch<Gather>(b.ph[phase], s); break;
     case BP_KICK:
         if (f32) phase_launch<Kick<float>>(b.ph[phase], s);

@@ -338,10 +338,14 @@ struct TilePass {
         const PairParams* pp;
         T *fx, *fy, *fz;
         double* ev_part;
-        const uint32_t* flags;
+        uint32_t* flags;
         unsigned long long* stats;
         uint16_t* nlist;
         int2* ncount;
+        int32_t* cell_count;
+        int md;        // MD step: bin the home atoms for the next rebuild
+        double3 cell;  // cell edge lengths (binning)
+        MdBins bins;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
@@ -351,6 +355,12 @@ struct TilePass {
         __shared__ TileHdr h;
         const int c = blk / A.split, part = blk % A.split;
         const bool rebuild = A.flags[2] != 0u;
+        if (threadIdx.x == 0) {
+            // the build kernels of this step have read the request and the cell
+            // histogram: clear both for the next step's binning
+            if (blk == 0) A.flags[0] = 0u;
+            if (part == 0) A.cell_count[c] = 0;
+        }
         tile_setup(c, A.dims, A.box, A.cell_start, h);
         const int total = h.off[27];
         const bool staged = total <= A.stage_cap;
@@ -358,6 +368,26 @@ struct TilePass {
         const int wv = uniform(threadIdx.x >> 6);
         const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
         const int a0 = h0 + (hc * part) / A.split, a1 = h0 + (hc * (part + 1)) / A.split;
+        if (A.md) {
+            // bins of the next rebuild: clear this step's buffer (consumed by the
+            // cell sort) for this cell, bin the current home atoms into the other
+            const int b = (int)(A.flags[3] & 1u);
+            if (part == 0 && threadIdx.x == 0) A.bins.count[b * A.bins.ncell + c] = 0;
+            for (int base = a0; base < a1; base += kTileThreads) {
+                const int s = base + (int)threadIdx.x;
+                const bool act = s < a1;
+                double p[3] = {0.0, 0.0, 0.0};
+                int v = 0;
+                if (act) {
+                    const double4 p4 = a.xs[s];
+                    p[0] = p4.x;
+                    p[1] = p4.y;
+                    p[2] = p4.z;
+                    v = a.perm[s];
+                }
+                md_bin_wave(A.bins, 1 - b, act, v, p, A.box, A.dims, A.cell);
+            }
+        }
         if (rebuild) {
             if (!staged && part == 0 && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
             if (staged) {
@@ -447,13 +477,14 @@ PairScalars pair_scalars(const Engine& e)
 }
 
 template <typename T, bool EV, bool M>
-typename TilePass<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void* fz)
+typename TilePass<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void* fz, bool md)
 {
     return typename TilePass<T, EV, M>::Args{
         PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
         e.xs_ref, e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box,
         e.stage_cap, e.r_list * e.r_list, e.r_in * e.r_in, pair_scalars(e), e.d_pp, (T*)fx,
-        (T*)fy, (T*)fz, e.ev_part, e.flags, e.stats, e.nlist, e.ncount};
+        (T*)fy, (T*)fz, e.ev_part, e.flags, e.stats, e.nlist, e.ncount, e.cell_count,
+        md ? 1 : 0, make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e)};
 }
 
 size_t pair_lds(const Engine& e)
@@ -462,10 +493,10 @@ size_t pair_lds(const Engine& e)
 }
 
 template <typename T, bool EV, bool M>
-void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, bool md, hipStream_t s)
 {
     k_one<TilePass<T, EV, M>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
-        tile_args<T, EV, M>(e, fx, fy, fz));
+        tile_args<T, EV, M>(e, fx, fy, fz, md));
 }
 
 template <typename T, bool M>
@@ -477,7 +508,7 @@ hipError_t pair_prepare_t(Batch& b)
     size_t lds = 0;
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
-        ta[r] = tile_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz);
+        ta[r] = tile_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz, true);
         nt[r] = e.ncell * e.split;
         const size_t l = pair_lds(e);
         lds = l > lds ? l : lds;
@@ -487,15 +518,15 @@ hipError_t pair_prepare_t(Batch& b)
 
 }  // namespace
 
-void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s)
+void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s, bool md)
 {
     if (e.n == 0) return;
     ScopedTimer t(e, P_PAIR, s);
     const bool multi = e.pp.ntypes > 1;
 #define RBMD_PAIR_M(T, EV)                                                                    \
     do {                                                                                      \
-        if (multi) pair_launch_t<T, EV, true>(e, fx, fy, fz, s);                              \
-        else pair_launch_t<T, EV, false>(e, fx, fy, fz, s);                                   \
+        if (multi) pair_launch_t<T, EV, true>(e, fx, fy, fz, md, s);                          \
+        else pair_launch_t<T, EV, false>(e, fx, fy, fz, md, s);                               \
     } while (0)
     if (e.precision == 32) {
         if (ev) RBMD_PAIR_M(float, true);

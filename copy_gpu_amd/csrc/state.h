@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -72,9 +73,16 @@ struct Engine {
     int32_t *perm = nullptr, *scell = nullptr, *ts = nullptr;
     int32_t *cell_count = nullptr, *cell_start = nullptr, *atom_cell = nullptr,
             *atom_rank = nullptr;
+    // bins of the MD step's next rebuild (double-buffered by the epoch flags[3])
+    int32_t* md_count = nullptr;   // [2][ncell]
+    int32_t* md_bucket = nullptr;  // [2][ncell][bucket_cap]
+    int32_t* md_cell = nullptr;    // [n]
+    double* md_w = nullptr;        // [3][n] wrapped positions at binning
+    int bucket_cap = 0;
     uint16_t* nlist = nullptr;  // tile-local indices
     int2* ncount = nullptr;
-    uint32_t* flags = nullptr;  // [0] rebuild request, [1] NaN seen, [2] rebuilding this step
+    uint32_t* flags = nullptr;  // [0] rebuild request, [1] NaN seen, [2] rebuilding this step,
+                                // [3] step epoch of the MD bins
     unsigned long long* stats = nullptr;    // [0] builds [1] entries [2] overflow rows [3] inner [4] longest overflowing row [6] big tiles [8..10] list stats
     double* q = nullptr;
     int32_t* type = nullptr;
@@ -154,6 +162,8 @@ enum BatchPh {
     BP_SCAN,
     BP_SCATTER,
     BP_CELLSORT,
+    BP_CELLSORT_MD,
+    BP_MDBIN,
     BP_GATHER,
     BP_RHO,
     BP_RHO_FINAL,
@@ -183,6 +193,13 @@ void pair_batch_launch(Batch& b, int phase, hipStream_t s);
 hipError_t kspace_batch_prepare(Batch& b);
 void kspace_batch_launch(Batch& b, int phase, hipStream_t s);
 
+inline MdBins md_bins(const Engine& e)
+{
+    const size_t n = (size_t)std::max<int64_t>(e.n_max, 1);
+    return MdBins{e.md_count, e.md_bucket, e.md_cell,  e.md_w,
+                  e.md_w + n, e.md_w + 2 * n, e.ncell, e.bucket_cap};
+}
+
 // ---- launchers (nlist.hip)
 void launch_check(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
 void launch_force_rebuild(Engine& e, hipStream_t s);
@@ -192,13 +209,17 @@ void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double*
                              double* vz, const void* fx, const void* fy, const void* fz,
                              int32_t* img, const double* mass, double dt, const SampleArgs& sa,
                              hipStream_t s);
+void launch_md_bin(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
+void launch_cellsort_md(Engine& e, const double* x, const double* y, const double* z,
+                        hipStream_t s);
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s);
 void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,
                  const void* fz, const double* mass, double dt, hipStream_t s);
 void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s);
 
 // ---- launchers (pair.hip)
-void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s);
+void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s,
+                 bool md = false);
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s);
 
 // ---- launchers (kspace.hip)

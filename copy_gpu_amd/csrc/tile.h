@@ -66,6 +66,74 @@ __device__ inline void tile_setup(int c, int3 dims, const Box& box,
     __syncthreads();
 }
 
+// ---------------------------------------------------------------- binning
+// Wrap into the box (x - L floor((x-lo)/L), no contraction: the CPU oracle
+// computes the same bits) and the cell trunc((x-lo)/cell), clamped.
+__device__ inline int wrap_cell(const double p[3], const Box& box, int3 dims, double3 cell,
+                                double w[3])
+{
+#pragma clang fp contract(off)
+    const int dm[3] = {dims.x, dims.y, dims.z};
+    const double cs[3] = {cell.x, cell.y, cell.z};
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        double f = floor((p[d] - box.lo[d]) / box.L[d]);
+        double fl = f * box.L[d];
+        w[d] = p[d] - fl;
+        int v = (int)((w[d] - box.lo[d]) / cs[d]);
+        v = v < 0 ? 0 : v;
+        v = v > dm[d] - 1 ? dm[d] - 1 : v;
+        c[d] = v;
+    }
+    return c[0] + c[1] * dims.x + c[2] * dims.x * dims.y;
+}
+
+// Wave-aggregated binning for lanes that mostly share one cell (the home atoms
+// of a tile): one atomic per distinct cell of the wave, ranks by ballot order.
+// Every lane of the wave calls it; inactive lanes bin nothing.
+__device__ inline void md_bin_wave(const MdBins& b, int buf, bool active, int v,
+                                   const double p[3], const Box& box, int3 dims, double3 cell)
+{
+    int c = -1;
+    if (active) {
+        double w[3];
+        c = wrap_cell(p, box, dims, cell, w);
+        b.cell[v] = c;
+        b.xw[v] = w[0];
+        b.yw[v] = w[1];
+        b.zw[v] = w[2];
+    }
+    const int lane = (int)__lane_id();
+    uint64_t pending = __ballot(active);
+    while (pending != 0ull) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int cl = __shfl(c, leader, 64);
+        const uint64_t grp = __ballot(c == cl) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&b.count[buf * b.ncell + cl], __popcll(grp));
+        base = __shfl(base, leader, 64);
+        if ((grp >> lane) & 1ull) {
+            const int rank = base + (int)mbcnt64(grp);
+            if (rank < b.cap) b.bucket[((size_t)buf * b.ncell + cl) * b.cap + rank] = v;
+        }
+        pending &= ~grp;
+    }
+}
+
+__device__ inline void md_bin_atom(const MdBins& b, int buf, int v, const double p[3],
+                                   const Box& box, int3 dims, double3 cell)
+{
+    double w[3];
+    const int c = wrap_cell(p, box, dims, cell, w);
+    b.cell[v] = c;
+    b.xw[v] = w[0];
+    b.yw[v] = w[1];
+    b.zw[v] = w[2];
+    const int slot = buf * b.ncell + c;
+    const int rank = atomicAdd(&b.count[slot], 1);
+    if (rank < b.cap) b.bucket[(size_t)slot * b.cap + rank] = v;
+}
+
 // Segment holding tile-local index u (largest st with off[st] <= u; empty
 // segments before it have smaller st, empty ones after it start above u).
 __device__ inline int tile_seg(const TileHdr& h, int u)
