@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "librbmd_force.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RBMD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "nlist.hip", "pair.hip", "kspace.hip", "mlp.hip", "systems.cpp",
+SOURCES = ["api.hip", "nlist.hip", "pair.hip", "kspace.hip", "md.hip", "mlp.hip", "systems.cpp",
            "proposal.cpp"]
 
 FLAGS = [

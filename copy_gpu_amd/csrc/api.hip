@@ -507,29 +507,21 @@ int rbmd_force_sample_kvectors(rbmd_force_t h, int32_t P, uint64_t seed, uint64_
     return RBMD_OK;
 }
 
-// One force evaluation; with kick pointers (MD mode) the k-space kernel also
-// applies the closing half kick, otherwise the caller kicks separately.  In MD
-// mode (checked) the kick/drift kernel has already checked, gathered and binned
-// the positions, so one cell-sort kernel does a rebuild.
-// One stream: running the structure factor on a second stream concurrently with
-// the list build and pair pass measured slower (cross-queue joins cost more than
-// the overlap gains at these kernel sizes).
+// One force evaluation at fixed positions: displacement check, list build (on
+// request), gather, rho(k), tile pass, k-space force.  One stream: running the
+// structure factor on a second stream concurrently with the list build and pair
+// pass measured slower (cross-queue joins cost more than the overlap gains at
+// these kernel sizes).
 static int force_pass(Engine& E, const double* x, const double* y, const double* z, void* fx,
-                      void* fy, void* fz, bool ev, hipStream_t s, bool checked,
-                      double* vx = nullptr, double* vy = nullptr, double* vz = nullptr,
-                      const double* mass = nullptr, double dt = 0.0)
+                      void* fy, void* fz, bool ev, hipStream_t s)
 {
-    if (!checked) launch_check(E, x, y, z, s);
-    const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
-    if (checked) {
-        launch_cellsort_md(E, x, y, z, s);  // the kick kernel binned and gathered
-    } else {
-        launch_build(E, x, y, z, s);
-        launch_gather(E, x, y, z, s);
-    }
+    launch_check(E, x, y, z, s);
+    const bool ks = has_kspace(E);
+    launch_build(E, x, y, z, s);
+    launch_gather(E, x, y, z, s);
     if (ks) launch_rho(E, x, y, z, s);
-    launch_pair(E, fx, fy, fz, ev, s, /*md=*/checked);
-    if (ks) launch_kforce(E, x, y, z, fx, fy, fz, vx, vy, vz, mass, dt, s);
+    launch_pair(E, fx, fy, fz, ev, s);
+    if (ks) launch_kforce(E, x, y, z, fx, fy, fz, s);
     if (ev) launch_ev_final(E, true, s);
     return RBMD_OK;
 }
@@ -569,7 +561,7 @@ int rbmd_force_compute(rbmd_force_t h, const double* x, const double* y, const d
     hipStream_t s = (hipStream_t)stream;
     int rc = maintain(E, s);
     if (rc != RBMD_OK) return rc;
-    force_pass(E, x, y, z, fx, fy, fz, (flags & RBMD_EV) != 0, s, false);
+    force_pass(E, x, y, z, fx, fy, fz, (flags & RBMD_EV) != 0, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
     return RBMD_OK;
@@ -590,19 +582,28 @@ int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* 
     return RBMD_OK;
 }
 
-static void md_step(Engine& E, double* x, double* y, double* z, double* vx, double* vy,
-                    double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
-                    double dt, uint64_t seed, const uint64_t* step_host, hipStream_t s)
+// One run segment of `steps` velocity-Verlet steps: the opening kick/drift, then
+// per step the rho partials + cell sort, the rho final + tile pass, and the
+// k-space force with the closing half kick, fused with the next step's opening
+// kick/drift except after the last step.  step_host: the first step's number
+// (eager launches), or null under graph capture, where the sampler reads the
+// device step counter that the rho final reduction advances.
+static void md_segment(Engine& E, const RepState& st, double dt, uint64_t seed,
+                       const uint64_t* step_host, int steps, hipStream_t s)
 {
-    SampleArgs sa;
-    if (E.kspace == RBMD_KSPACE_RBE) sa = sample_args(E, E.P, seed, step_host);
-    launch_kick_drift_check(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa, s);
-    const bool ks = (E.kspace == RBMD_KSPACE_RBE || E.kspace == RBMD_KSPACE_EWALD) && E.P > 0;
-    if (ks) {
-        force_pass(E, x, y, z, fx, fy, fz, false, s, true, vx, vy, vz, mass, dt);
-    } else {
-        force_pass(E, x, y, z, fx, fy, fz, false, s, true);
-        launch_kick(E, vx, vy, vz, fx, fy, fz, mass, dt, s);
+    auto sa_of = [&](int k) {
+        SampleArgs sa;
+        if (E.kspace != RBMD_KSPACE_RBE) return sa;
+        if (!step_host) return sample_args(E, E.P, seed, nullptr);
+        const uint64_t v = *step_host + (uint64_t)k;
+        return sample_args(E, E.P, seed, &v);
+    };
+    launch_md_open(E, st, dt, sa_of(0), s);
+    for (int k = 0; k < steps; ++k) {
+        launch_md_sort_rho(E, st, s);
+        launch_tile_rho(E, st.fx, st.fy, st.fz, s);
+        if (k + 1 < steps) launch_md_close_open(E, st, dt, sa_of(k + 1), s);
+        else launch_md_close(E, st, dt, s);
     }
 }
 
@@ -632,10 +633,19 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     // bin the current positions for a rebuild in the first step (later steps'
     // bins come from the tile pass of the step before)
     launch_md_bin(E, x, y, z, s);
+    RepState rs;
+    rs.x = x; rs.y = y; rs.z = z;
+    rs.vx = vx; rs.vy = vy; rs.vz = vz;
+    rs.fx = fx; rs.fy = fy; rs.fz = fz;
+    rs.img = img;
+    rs.mass = mass;
     if (!use_graph) {
-        for (int64_t k = 0; k < nsteps; ++k) {
+        // segments of at most 2^30 steps; the kernels give the same bits however
+        // a run is cut into segments
+        constexpr int64_t kEagerSegment = int64_t(1) << 30;
+        for (int64_t k = 0; k < nsteps; k += kEagerSegment) {
             const uint64_t st = step0 + (uint64_t)k;
-            md_step(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, seed, &st, s);
+            md_segment(E, rs, dt, seed, &st, (int)std::min<int64_t>(kEagerSegment, nsteps - k), s);
         }
         HIP_TRY(hipGetLastError());
     } else {
@@ -649,6 +659,10 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         key.push_back((const void*)(uintptr_t)seed);
         // the proposal's table pointers, lambda and P-dependent weights are baked too
         key.push_back((const void*)(uintptr_t)E.gp_gen);
+        // and the launch shapes the autotuner may change
+        for (int64_t v : {(int64_t)E.split, (int64_t)E.stage_cap, (int64_t)E.ncell,
+                          (int64_t)E.rho_apb, (int64_t)E.kpart_cap})
+            key.push_back((const void*)(uintptr_t)v);
         // two cached graphs: kGraphSteps steps (amortises the launch of a graph)
         // and one step (the remainder)
         for (int g = 0; g < 2; ++g) {
@@ -662,8 +676,7 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
             G.graph = nullptr;
             E.capturing = true;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < steps; ++k)
-                md_step(E, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, seed, nullptr, s);
+            md_segment(E, rs, dt, seed, nullptr, steps, s);
             hipError_t st = hipStreamEndCapture(s, &G.graph);
             E.capturing = false;
             if (st != hipSuccess) {
@@ -950,7 +963,7 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
         launch_gather(E, x, y, z, s);
         if (ks) launch_rho(E, x, y, z, s);
         launch_pair(E, fx, fy, fz, false, s);
-        if (ks) launch_kforce(E, x, y, z, fx, fy, fz, nullptr, nullptr, nullptr, nullptr, 0.0, s);
+        if (ks) launch_kforce(E, x, y, z, fx, fy, fz, s);
     };
     auto measure = [&](const TilePolicy& p) -> double {
         apply_policy(E, p);
@@ -995,9 +1008,7 @@ struct rbmd_batch_s {
     Batch b;
     hipStream_t own_stream = nullptr;
     hipEvent_t order_event = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    std::vector<const void*> graph_key;
+    Engine::StepGraph graphs[2];  // [0] kGraphSteps steps, [1] one step
     std::string err;
 };
 
@@ -1014,23 +1025,17 @@ static int bfail(rbmd_batch_t b, int code, const std::string& msg)
                                                                  hipGetErrorString(_st)); \
     } while (0)
 
-// One batched velocity-Verlet step: every phase is one launch over all replicas.
-// The kick/drift kernel gathers and bins the positions.
-static void batch_step(rbmd_batch_t b, hipStream_t s)
+// A batched run segment of `steps` velocity-Verlet steps: the kernel sequence of
+// md_segment, every phase one launch over all replicas.  The replicas' samplers
+// read their device step counters (advanced by the rho final reduction).
+static void batch_segment(rbmd_batch_t b, int steps, hipStream_t s)
 {
     Batch& B = b->b;
-    const bool ks = B.kspace == RBMD_KSPACE_RBE || B.kspace == RBMD_KSPACE_EWALD;
-    nlist_batch_launch(B, BP_KICKDRIFT, s);
-    nlist_batch_launch(B, BP_CELLSORT_MD, s);
-    if (ks) {
-        kspace_batch_launch(B, BP_RHO, s);
-        kspace_batch_launch(B, BP_RHO_FINAL, s);
-    }
-    pair_batch_launch(B, BP_PAIR, s);
-    if (ks) {
-        kspace_batch_launch(B, BP_KFORCE, s);
-    } else {
-        nlist_batch_launch(B, BP_KICK, s);
+    md_batch_launch(B, BP_KICKDRIFT, s);
+    for (int k = 0; k < steps; ++k) {
+        md_batch_launch(B, BP_SORT_RHO, s);
+        pair_batch_launch(B, BP_PAIR, s);
+        md_batch_launch(B, k + 1 < steps ? BP_KFORCE_DRIFT : BP_KFORCE, s);
     }
 }
 
@@ -1078,8 +1083,10 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
 void rbmd_batch_destroy(rbmd_batch_t b)
 {
     if (!b) return;
-    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto& G : b->graphs) {
+        if (G.exec) hipGraphExecDestroy(G.exec);
+        if (G.graph) hipGraphDestroy(G.graph);
+    }
     for (auto& ph : b->b.ph) ph.release();
     for (rbmd_force_t h : b->reps) rbmd_force_destroy(h);
     if (b->own_stream) hipStreamDestroy(b->own_stream);
@@ -1148,13 +1155,15 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     }
     BHIP_TRY(nlist_batch_prepare(B));
     BHIP_TRY(pair_batch_prepare(B));
-    if (B.kspace != RBMD_KSPACE_NONE) BHIP_TRY(kspace_batch_prepare(B));
+    BHIP_TRY(md_batch_prepare(B));
     // bins of the current positions for a rebuild in the first step
     for (Engine* E : B.es)
         BHIP_TRY(hipMemsetAsync(E->md_count, 0, (size_t)2 * E->ncell * sizeof(int32_t), s));
     nlist_batch_launch(B, BP_MDBIN, s);
     if (!use_graph) {
-        for (int64_t k = 0; k < nsteps; ++k) batch_step(b, s);
+        constexpr int64_t kEagerSegment = int64_t(1) << 30;
+        for (int64_t k = 0; k < nsteps; k += kEagerSegment)
+            batch_segment(b, (int)std::min<int64_t>(kEagerSegment, nsteps - k), s);
         BHIP_TRY(hipGetLastError());
     } else {
         std::vector<const void*> key;
@@ -1164,21 +1173,28 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
             key.push_back((const void*)(uintptr_t)ph.items);
             key.push_back((const void*)(uintptr_t)ph.lds);
         }
-        if (!b->graph_exec || key != b->graph_key) {
-            if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
-            if (b->graph) hipGraphDestroy(b->graph);
-            b->graph_exec = nullptr;
-            b->graph = nullptr;
+        for (int g = 0; g < 2; ++g) {
+            Engine::StepGraph& G = b->graphs[g];
+            const int steps = g == 0 ? kGraphSteps : 1;
+            if (g == 0 && nsteps < kGraphSteps) continue;
+            if (G.exec && key == G.key) continue;
+            if (G.exec) hipGraphExecDestroy(G.exec);
+            if (G.graph) hipGraphDestroy(G.graph);
+            G.exec = nullptr;
+            G.graph = nullptr;
             BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            batch_step(b, s);
-            hipError_t e = hipStreamEndCapture(s, &b->graph);
+            batch_segment(b, steps, s);
+            hipError_t e = hipStreamEndCapture(s, &G.graph);
             if (e != hipSuccess)
                 return bfail(b, RBMD_ERR_HIP, std::string("graph capture failed: ") +
                                                   hipGetErrorString(e));
-            BHIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
-            b->graph_key = key;
+            BHIP_TRY(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+            G.key = key;
         }
-        for (int64_t k = 0; k < nsteps; ++k) BHIP_TRY(hipGraphLaunch(b->graph_exec, s));
+        for (int64_t k = 0; k + kGraphSteps <= nsteps; k += kGraphSteps)
+            BHIP_TRY(hipGraphLaunch(b->graphs[0].exec, s));
+        for (int64_t k = 0; k < nsteps % kGraphSteps; ++k)
+            BHIP_TRY(hipGraphLaunch(b->graphs[1].exec, s));
     }
     BHIP_TRY(hipStreamSynchronize(s));
     for (size_t r = 0; r < R; ++r) {
@@ -1233,17 +1249,14 @@ int rbmd_batch_autotune(rbmd_batch_t b, const rbmd_replica_state* st, int32_t re
     auto measure = [&](const TilePolicy& p) -> double {
         for (Engine* E : B.es) apply_policy(*E, p);
         if (nlist_batch_prepare(B) != hipSuccess || pair_batch_prepare(B) != hipSuccess ||
-            (ks && kspace_batch_prepare(B) != hipSuccess))
+            md_batch_prepare(B) != hipSuccess || (ks && kspace_batch_prepare(B) != hipSuccess))
             return -1.0;
         auto pass = [&]() {
             for (int ph : {BP_BIN, BP_SCAN, BP_SCATTER, BP_CELLSORT}) nlist_batch_launch(B, ph, s);
             nlist_batch_launch(B, BP_GATHER, s);
-            if (ks) {
-                kspace_batch_launch(B, BP_RHO, s);
-                kspace_batch_launch(B, BP_RHO_FINAL, s);
-            }
-            pair_batch_launch(B, BP_PAIR, s);
-            if (ks) kspace_batch_launch(B, BP_KFORCE, s);
+            if (ks) kspace_batch_launch(B, BP_RHO, s);
+            pair_batch_launch(B, BP_PAIR, s);  // rho final + tile pass
+            md_batch_launch(B, BP_KFORCE, s);
         };
         double sb = 0, sf = 0;
         for (int r = 0; r <= reps; ++r) {

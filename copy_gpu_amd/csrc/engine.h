@@ -190,6 +190,12 @@ RBMD_HD inline double draw_kvector(uint64_t seed, uint64_t step, uint32_t l, con
 
 #ifdef __HIPCC__
 // ---------------------------------------------------------------- wave64 helpers
+// Minimum-image displacement along one axis of a periodic box.
+__device__ inline double min_image(double t, double L, double invL)
+{
+    return t - L * rint(t * invL);
+}
+
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
 // Number of set bits of mask below this lane (ballot compaction offset).

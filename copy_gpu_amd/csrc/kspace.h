@@ -1,0 +1,381 @@
+// kspace.h -- k-space kernel bodies shared by the k-space launches and the
+// fused MD-step kernels: the structure factor rho(k) = sum_j q_j exp(i k.r_j)
+// (per-workgroup partial sums, then a fixed-order final reduction with the
+// force coefficients and per-k energy / virial terms) and the per-atom k-space
+// force.
+//
+// Reference (paths relative to the reference checkout):
+//   rho(k)                 rbmd/framework/run/worklet/RunWorklet.cpp:1616-1677,
+//                          rbmd/framework/run/ExecutionMD.cpp:412-448
+//   RBE force              RunWorklet.cpp:1508-1548, ExecForceFunction.h:247-269
+//   Ewald force / energy   ExecForceFunction.h:226-245, output/ThermoOutput.cpp:209-255
+//
+// Phases are formed from FP64 fractional coordinates, phi = frac(m . s) with
+// s = (x - lo)/L, so FP32 sines keep full accuracy at any |k.r|.  FP32 mode
+// accumulates rho with Kahan compensation per lane and reduces the per-block
+// partials in FP64, deterministically.
+#pragma once
+#include "state.h"
+
+namespace rbmd {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- trig helpers
+template <typename T>
+__device__ inline void sincos2pi(double phi, T* s, T* c);
+
+// FP32: the hardware sine/cosine take their argument in turns (sin(2 pi x));
+// on [-0.5, 0.5] turns their error is 1.3e-7 absolute (measured against FP64),
+// the size of the FP32 rounding of the turn fraction itself.
+template <>
+__device__ inline void sincos2pi<float>(double phi, float* s, float* c)
+{
+    const float f = (float)(phi - rint(phi));  // [-0.5, 0.5] turns
+    *s = __builtin_amdgcn_sinf(f);
+    *c = __builtin_amdgcn_cosf(f);
+}
+
+template <>
+__device__ inline void sincos2pi<double>(double phi, double* s, double* c)
+{
+    const double f = phi - rint(phi);
+    sincospi(2.0 * f, s, c);
+}
+
+// Kahan accumulator in T (used for float; a no-op compensation in double).
+template <typename T>
+struct Kahan {
+    T s = 0, c = 0;
+    __device__ inline void add(T v)
+    {
+        const T y = v - c;
+        const T t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ inline double value() const { return (double)s - (double)c; }
+};
+
+template <>
+struct Kahan<double> {
+    double s = 0;
+    __device__ inline void add(double v) { s += v; }
+    __device__ inline double value() const { return s; }
+};
+
+// ---------------------------------------------------------------- rho partials
+// Workgroups = (atom tiles) x (k tiles of 64), atom tile fastest.  Lane <-> k
+// within the tile, the 4 waves split the tile's atoms; atoms are staged
+// through LDS 256 at a time.
+constexpr int kRhoBlock = 256;
+constexpr int kRhoStage = 256;
+
+template <typename T>
+struct RhoPartial {
+    static constexpr int kThreads = kRhoBlock;
+    struct Args {
+        const double *x, *y, *z, *q;
+        int n, atoms_per_block, nbx;
+        Box box;
+        const int32_t* km;
+        int P;
+        double* kpart;
+    };
+    // LDS of one workgroup: kRhoStage staged atoms + the per-wave sums
+    static constexpr size_t kLdsBytes =
+        kRhoStage * sizeof(double4) + (kRhoBlock / 64) * 64 * 2 * sizeof(double);
+    __device__ static void run(const Args& A, int blk, int)
+    {
+        __shared__ double4 lds[kLdsBytes / sizeof(double4)];
+        run_lds(A, blk, lds);
+    }
+    __device__ static void run_lds(const Args& A, int blk, double4* st)
+    {
+#pragma clang fp contract(off)
+        double(*red)[64][2] = (double(*)[64][2])(st + kRhoStage);
+        const int bx = blk % A.nbx, ky = blk / A.nbx;
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int kidx = ky * 64 + lane;
+        const int P = A.P;
+        const bool kok = kidx < P;
+        double mx = 0, my = 0, mz = 0;
+        if (kok) {
+            mx = (double)A.km[3 * kidx];
+            my = (double)A.km[3 * kidx + 1];
+            mz = (double)A.km[3 * kidx + 2];
+        }
+        Kahan<T> re, im;
+        const int a0 = bx * A.atoms_per_block;
+        int a1 = a0 + A.atoms_per_block;
+        a1 = a1 < A.n ? a1 : A.n;
+        const Box& box = A.box;
+        for (int base = a0; base < a1; base += kRhoStage) {
+            __syncthreads();
+            const int i = base + threadIdx.x;
+            if (threadIdx.x < kRhoStage) {
+                if (i < a1)
+                    st[threadIdx.x] = make_double4((A.x[i] - box.lo[0]) * box.invL[0],
+                                                   (A.y[i] - box.lo[1]) * box.invL[1],
+                                                   (A.z[i] - box.lo[2]) * box.invL[2], A.q[i]);
+                else
+                    st[threadIdx.x] = make_double4(0, 0, 0, 0);
+            }
+            __syncthreads();
+            const int cnt = (a1 - base) < kRhoStage ? (a1 - base) : kRhoStage;
+            if (kok) {
+                for (int a = wv; a < cnt; a += kRhoBlock / 64) {
+                    const double4 p = st[a];
+                    const double phi = fma(mz, p.z, fma(my, p.y, mx * p.x));
+                    T sn, cs;
+                    sincos2pi<T>(phi, &sn, &cs);
+                    const T qa = (T)p.w;
+                    re.add(qa * cs);
+                    im.add(qa * sn);
+                }
+            }
+        }
+        red[wv][lane][0] = re.value();
+        red[wv][lane][1] = im.value();
+        __syncthreads();
+        if (wv == 0 && kok) {
+            double r = 0.0, m = 0.0;
+            for (int w = 0; w < kRhoBlock / 64; ++w) {
+                r += red[w][lane][0];
+                m += red[w][lane][1];
+            }
+            const size_t o = ((size_t)bx * P + kidx) * 2;
+            A.kpart[o] = r;
+            A.kpart[o + 1] = m;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- rho final
+// One workgroup of NT threads per k: FP64 sums over the atom-tile partials
+// (strided, then DPP wave sums, then the waves in order: a fixed order), then
+// the force coefficients and the energy / virial terms of this k.
+template <typename T, int NT>
+struct RhoFinalT {
+    static constexpr int kThreads = NT;
+    struct Args {
+        const double* kpart;
+        int nbx, P;
+        const int32_t* km;
+        const double* kw;
+        Box box;
+        double alpha, qqr2e;
+        double* rho;
+        KCoef<T>* coef;
+        double* kev;
+        unsigned long long* step_dev;
+    };
+    __device__ static void run(const Args& A, int blk, int)
+    {
+        __shared__ double wr[NT / 64][2];
+        const int k = blk, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        const int P = A.P;
+        double r = 0.0, m = 0.0;
+        for (int b = t; b < A.nbx; b += NT) {
+            r += A.kpart[((size_t)b * P + k) * 2];
+            m += A.kpart[((size_t)b * P + k) * 2 + 1];
+        }
+        r = wave_sum_dpp(r);
+        m = wave_sum_dpp(m);
+        if (lane == 0) {
+            wr[wv][0] = r;
+            wr[wv][1] = m;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double re = 0.0, im = 0.0;
+            for (int w = 0; w < NT / 64; ++w) {
+                re += wr[w][0];
+                im += wr[w][1];
+            }
+            const Box& box = A.box;
+            A.rho[2 * k] = re;
+            A.rho[2 * k + 1] = im;
+            const double mm[3] = {(double)A.km[3 * k], (double)A.km[3 * k + 1],
+                                  (double)A.km[3 * k + 2]};
+            double kv[3];
+            for (int d = 0; d < 3; ++d) kv[d] = 2.0 * kPi * mm[d] / box.L[d];
+            const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+            const double V = box.L[0] * box.L[1] * box.L[2];
+            const double w = A.kw[k];
+            const double c = w * (-4.0 * kPi) / (V * k2) * A.qqr2e;
+            KCoef<T> kc;
+            for (int d = 0; d < 3; ++d) {
+                kc.m[d] = mm[d];
+                kc.k[d] = (T)kv[d];
+            }
+            kc.A = (T)(c * im);
+            kc.B = (T)(c * re);
+            A.coef[k] = kc;
+            const double e = w * (2.0 * kPi / V) * (re * re + im * im) / k2 * A.qqr2e;
+            const double b = 2.0 * (1.0 / k2 + 1.0 / (4.0 * A.alpha));
+            double* o = A.kev + (size_t)k * 7;
+            o[0] = e;
+            o[1] = e * (1.0 - b * kv[0] * kv[0]);
+            o[2] = e * (1.0 - b * kv[1] * kv[1]);
+            o[3] = e * (1.0 - b * kv[2] * kv[2]);
+            o[4] = -e * b * kv[0] * kv[1];
+            o[5] = -e * b * kv[0] * kv[2];
+            o[6] = -e * b * kv[1] * kv[2];
+            if (k == 0 && A.step_dev) *A.step_dev += 1ull;
+        }
+    }
+};
+
+template <typename T>
+using RhoFinal = RhoFinalT<T, 256>;
+
+// ---------------------------------------------------------------- k-space force
+// Workgroup = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes
+// k = w, w+8, ... of the coefficients staged in LDS (every lane reads the same
+// k: broadcast), then the 8 partial forces of each atom are summed in a fixed
+// order.  F_i += q_i sum_k k (cos(phi) A_k - sin(phi) B_k).
+constexpr int kKfAtoms = 64;
+constexpr int kKfSlices = 8;
+constexpr int kKfThreads = kKfAtoms * kKfSlices;
+constexpr int kKfStage = 512;
+
+template <typename T>
+struct KForceArgs {
+    const double *x, *y, *z, *q;
+    int n;
+    Box box;
+    const KCoef<T>* coef;
+    int P;  // 0: no k-space term (the kernel only completes / kicks)
+    T *fx, *fy, *fz;
+    double *vx, *vy, *vz;
+    const double* mass;
+    double dt;
+    uint32_t* flags;
+};
+
+template <typename T>
+struct KForceLds {
+    KCoef<T> st[kKfStage];
+    T red[kKfSlices][3][kKfAtoms];
+};
+
+// The k-space force of the 64 atoms of workgroup blk; each lane of wave 0 then
+// calls fin(i, f0, f1, f2) with the complete force (pair force + k-space) of
+// its atom i < n.  fin runs with FP contraction off in the caller's body.
+template <typename T, class Fin>
+__device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T>& L, Fin fin)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blk * kKfAtoms + lane;
+    const int n = A.n, P = A.P;
+    const Box& box = A.box;
+    double sx = 0, sy = 0, sz = 0;
+    if (i < n && P > 0) {
+        sx = (A.x[i] - box.lo[0]) * box.invL[0];
+        sy = (A.y[i] - box.lo[1]) * box.invL[1];
+        sz = (A.z[i] - box.lo[2]) * box.invL[2];
+    }
+    T ax = 0, ay = 0, az = 0;
+    for (int base = 0; base < P; base += kKfStage) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < kKfStage && base + k < P; k += kKfThreads)
+            L.st[k] = A.coef[base + k];
+        __syncthreads();
+        const int cnt = (P - base) < kKfStage ? (P - base) : kKfStage;
+        for (int k = wv; k < cnt; k += kKfSlices) {
+            const KCoef<T>& c = L.st[k];
+            const double phi = fma(c.m[2], sz, fma(c.m[1], sy, c.m[0] * sx));
+            T sn, cs;
+            sincos2pi<T>(phi, &sn, &cs);
+            const T g = cs * c.A - sn * c.B;
+            ax += g * c.k[0];
+            ay += g * c.k[1];
+            az += g * c.k[2];
+        }
+    }
+    if (P > 0) {
+        L.red[wv][0][lane] = ax;
+        L.red[wv][1][lane] = ay;
+        L.red[wv][2][lane] = az;
+        __syncthreads();
+    }
+    if (wv == 0 && i < n) {
+        T f0 = A.fx[i], f1 = A.fy[i], f2 = A.fz[i];
+        if (P > 0) {
+            T s0 = 0, s1 = 0, s2 = 0;
+            for (int w = 0; w < kKfSlices; ++w) {
+                s0 += L.red[w][0][lane];
+                s1 += L.red[w][1][lane];
+                s2 += L.red[w][2][lane];
+            }
+            const T qi = (T)A.q[i];
+            f0 = f0 + qi * s0;
+            f1 = f1 + qi * s1;
+            f2 = f2 + qi * s2;
+        }
+        fin(i, f0, f1, f2);
+    }
+}
+
+// F += k-space force; with KICK the closing half kick v += dt/2 F/m of the
+// velocity-Verlet step follows (and the MD step epoch advances).
+template <typename T, bool KICK>
+struct KForce {
+    static constexpr int kThreads = kKfThreads;
+    using Args = KForceArgs<T>;
+    __device__ static void run(const Args& A, int blk, int)
+    {
+#pragma clang fp contract(off)
+        if (KICK && blk == 0 && threadIdx.x == 0) A.flags[3] += 1u;  // MD step epoch
+        __shared__ KForceLds<T> L;
+        kforce_block(A, blk, L, [&](int i, T f0, T f1, T f2) {
+#pragma clang fp contract(off)
+            A.fx[i] = f0;
+            A.fy[i] = f1;
+            A.fz[i] = f2;
+            if (KICK) {
+                const double h = 0.5 * A.dt / (A.mass ? A.mass[i] : 1.0);
+                A.vx[i] = A.vx[i] + h * (double)f0;
+                A.vy[i] = A.vy[i] + h * (double)f1;
+                A.vz[i] = A.vz[i] + h * (double)f2;
+            }
+        });
+    }
+};
+
+template <typename T>
+inline typename RhoPartial<T>::Args rho_args(Engine& e, const double* x, const double* y,
+                                             const double* z, int* blocks)
+{
+    int apb, nbx, nkt;
+    rho_geometry(e, &apb, &nbx, &nkt);
+    *blocks = e.P > 0 ? nbx * nkt : 0;
+    return typename RhoPartial<T>::Args{x, y, z, e.q, (int)e.n, apb, nbx, e.box, e.km, e.P,
+                                        e.kpart};
+}
+
+template <typename T, int NT>
+inline typename RhoFinalT<T, NT>::Args rho_final_args(Engine& e, unsigned long long* step_dev)
+{
+    int apb, nbx, nkt;
+    rho_geometry(e, &apb, &nbx, &nkt);
+    return typename RhoFinalT<T, NT>::Args{e.kpart, nbx,     e.P,   e.km,
+                                           e.kw,    e.box,   e.alpha, e.qqr2e,
+                                           e.rho,   (KCoef<T>*)e.kcoef, e.kev, step_dev};
+}
+
+template <typename T>
+inline KForceArgs<T> kforce_args(Engine& e, const double* x, const double* y, const double* z,
+                                 void* fx, void* fy, void* fz, double* vx, double* vy,
+                                 double* vz, const double* mass, double dt, bool kspace = true)
+{
+    return KForceArgs<T>{x,        y,  z,  e.q,  (int)e.n, e.box, (const KCoef<T>*)e.kcoef,
+                         kspace ? e.P : 0, (T*)fx, (T*)fy, (T*)fz, vx, vy, vz, mass, dt,
+                         e.flags};
+}
+
+inline int kforce_blocks(const Engine& e) { return (int)((e.n + kKfAtoms - 1) / kKfAtoms); }
+
+}  // namespace rbmd

@@ -1,0 +1,193 @@
+// md.hip -- launches of the velocity-Verlet MD-step kernels (bodies in md.h and,
+// for the tile pass, pair.hip) for one handle and for replica batches.
+//
+// Reference behaviour followed (paths relative to the reference checkout):
+//   velocity Verlet step          rbmd/framework/run/ExecutionNVT.cpp:133-161, 224-250,
+//                                 rbmd/framework/run/worklet/RunWorklet.cpp:1764-1782, 1854-1877
+#include "md.h"
+#include "state.h"
+
+namespace rbmd {
+
+namespace {
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+template <typename T>
+KickDriftArgs<T> kickdrift_args(Engine& e, const RepState& st, double dt, const SampleArgs& sa)
+{
+    const double h = 0.5 * e.skin;
+    return KickDriftArgs<T>{st.x,   st.y,   st.z,   st.vx,         st.vy,
+                            st.vz,  (const T*)st.fx, (const T*)st.fy, (const T*)st.fz,
+                            st.img, st.mass, dt,    (int)e.n,      e.box,
+                            e.xr,   e.yr,   e.zr,   h * h,         e.flags,
+                            e.q,    e.atom_rank /* inverse perm */, e.xs, sa};
+}
+
+CellSortMDArgs cellsort_md_args(Engine& e, const RepState& st)
+{
+    return CellSortMDArgs{st.x,    st.y,    st.z,     e.q,         e.type,
+                          (int)e.n, e.ncell, e.box,   md_bins(e),  e.cell_start,
+                          e.perm,  e.atom_rank /* inverse perm */, e.scell,
+                          e.ts,    e.xr,    e.yr,     e.zr,        e.xs_ref,
+                          e.xs,    e.flags, e.stats};
+}
+
+template <typename T>
+typename SortRho<T>::Args sortrho_args(Engine& e, const RepState& st, int* blocks)
+{
+    typename SortRho<T>::Args a;
+    int nr = 0;
+    if (has_kspace(e)) a.rho = rho_args<T>(e, st.x, st.y, st.z, &nr);
+    else memset(&a.rho, 0, sizeof a.rho);
+    a.nrho = nr;
+    a.cs = cellsort_md_args(e, st);
+    *blocks = nr + sortrho_cell_blocks(e.ncell);
+    return a;
+}
+
+template <typename T>
+KForceArgs<T> close_args(Engine& e, const RepState& st, double dt)
+{
+    return kforce_args<T>(e, st.x, st.y, st.z, st.fx, st.fy, st.fz, st.vx, st.vy, st.vz,
+                          st.mass, dt, has_kspace(e));
+}
+
+template <typename T>
+typename KForceDrift<T>::Args close_open_args(Engine& e, const RepState& st, double dt,
+                                              const SampleArgs& sa_next)
+{
+    return typename KForceDrift<T>::Args{close_args<T>(e, st, dt),
+                                         kickdrift_args<T>(e, st, dt, sa_next)};
+}
+
+int kickdrift_blocks(const Engine& e, const SampleArgs& sa)
+{
+    const int b = std::max(ceil_div(e.n, kMdBlock), ceil_div(sa.P, 64));
+    return b > 0 ? b : 1;
+}
+
+int close_open_blocks(const Engine& e, const SampleArgs& sa)
+{
+    const int b = std::max(kforce_blocks(e), ceil_div(sa.P, 64));
+    return b > 0 ? b : 1;
+}
+
+template <typename T>
+void md_open_t(Engine& e, const RepState& st, double dt, const SampleArgs& sa, hipStream_t s)
+{
+    k_one<KickDrift<T>><<<kickdrift_blocks(e, sa), kMdBlock, 0, s>>>(
+        kickdrift_args<T>(e, st, dt, sa));
+}
+
+template <typename T>
+void md_sort_rho_t(Engine& e, const RepState& st, hipStream_t s)
+{
+    int blocks;
+    const auto a = sortrho_args<T>(e, st, &blocks);
+    k_one<SortRho<T>><<<blocks, kRhoBlock, 0, s>>>(a);
+}
+
+template <typename T>
+void md_close_t(Engine& e, const RepState& st, double dt, hipStream_t s)
+{
+    k_one<KForce<T, true>><<<std::max(kforce_blocks(e), 1), kKfThreads, 0, s>>>(
+        close_args<T>(e, st, dt));
+}
+
+template <typename T>
+void md_close_open_t(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
+                     hipStream_t s)
+{
+    k_one<KForceDrift<T>><<<close_open_blocks(e, sa), kKfThreads, 0, s>>>(
+        close_open_args<T>(e, st, dt, sa));
+}
+
+template <typename T>
+hipError_t md_prepare_t(Batch& b)
+{
+    const size_t R = b.es.size();
+    std::vector<typename KickDrift<T>::Args> kd(R);
+    std::vector<typename SortRho<T>::Args> sr(R);
+    std::vector<KForceArgs<T>> kc(R);
+    std::vector<typename KForceDrift<T>::Args> ko(R);
+    std::vector<int> nkd(R), nsr(R), nkc(R), nko(R);
+    for (size_t r = 0; r < R; ++r) {
+        Engine& e = *b.es[r];
+        const RepState& st = b.st[r];
+        kd[r] = kickdrift_args<T>(e, st, b.dt, b.sa[r]);
+        sr[r] = sortrho_args<T>(e, st, &nsr[r]);
+        kc[r] = close_args<T>(e, st, b.dt);
+        ko[r] = close_open_args<T>(e, st, b.dt, b.sa[r]);
+        nkd[r] = kickdrift_blocks(e, b.sa[r]);
+        nkc[r] = std::max(kforce_blocks(e), 1);
+        nko[r] = close_open_blocks(e, b.sa[r]);
+    }
+    hipError_t err;
+    if ((err = phase_upload(b.ph[BP_KICKDRIFT], kd, nkd, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_SORT_RHO], sr, nsr, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_KFORCE], kc, nkc, 0)) != hipSuccess) return err;
+    return phase_upload(b.ph[BP_KFORCE_DRIFT], ko, nko, 0);
+}
+
+}  // namespace
+
+// ================================================================= launchers
+
+void launch_md_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
+                    hipStream_t s)
+{
+    ScopedTimer t(e, P_CHECK, s);
+    if (e.precision == 32) md_open_t<float>(e, st, dt, sa, s);
+    else md_open_t<double>(e, st, dt, sa, s);
+}
+
+void launch_md_sort_rho(Engine& e, const RepState& st, hipStream_t s)
+{
+    ScopedTimer t(e, P_RHO, s);
+    if (e.precision == 32) md_sort_rho_t<float>(e, st, s);
+    else md_sort_rho_t<double>(e, st, s);
+}
+
+void launch_md_close(Engine& e, const RepState& st, double dt, hipStream_t s)
+{
+    ScopedTimer t(e, P_KFORCE, s);
+    if (e.precision == 32) md_close_t<float>(e, st, dt, s);
+    else md_close_t<double>(e, st, dt, s);
+}
+
+void launch_md_close_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
+                          hipStream_t s)
+{
+    ScopedTimer t(e, P_KFORCE, s);
+    if (e.precision == 32) md_close_open_t<float>(e, st, dt, sa, s);
+    else md_close_open_t<double>(e, st, dt, sa, s);
+}
+
+hipError_t md_batch_prepare(Batch& b)
+{
+    return b.precision == 32 ? md_prepare_t<float>(b) : md_prepare_t<double>(b);
+}
+
+void md_batch_launch(Batch& b, int phase, hipStream_t s)
+{
+    const bool f32 = b.precision == 32;
+#define RBMD_MB(B)                                                                            \
+    do {                                                                                      \
+        if (f32) phase_launch<B<float>>(b.ph[phase], s);                                      \
+        else phase_launch<B<double>>(b.ph[phase], s);                                         \
+    } while (0)
+    switch (phase) {
+    case BP_KICKDRIFT: RBMD_MB(KickDrift); break;
+    case BP_SORT_RHO: RBMD_MB(SortRho); break;
+    case BP_KFORCE_DRIFT: RBMD_MB(KForceDrift); break;
+    case BP_KFORCE:
+        if (f32) phase_launch<KForce<float, true>>(b.ph[phase], s);
+        else phase_launch<KForce<double, true>>(b.ph[phase], s);
+        break;
+    default: break;
+    }
+#undef RBMD_MB
+}
+
+}  // namespace rbmd

@@ -1,11 +1,12 @@
-// nlist.hip -- cell list (counting sort) and Verlet list (wave64 ballot compaction),
-// per-step displacement check and position gather, FP64 velocity-Verlet kernels.
+// nlist.hip -- cell list (counting sort) of the force pass, per-step displacement
+// check and position gather, the MD-run binning prologue and list statistics.
+// (The Verlet rows are built inside the tile pass, pair.hip; the MD-step kernels
+// are in md.h / md.hip.)
 //
 // Reference behaviour followed (paths relative to the reference checkout):
 //   cell grid / binning / sort    rbmd/framework/locator/ContPointLocator.{h,cxx} (h:44-53, cxx:38-92)
 //   27-cell stencil + periodic    rbmd/framework/locator/ExecPointLocator.h:303-319
 //   list criterion                rbmd/framework/run/worklet/RunWorklet.cpp:40-96 (:83)
-//   kick / drift + image flags    rbmd/framework/run/worklet/RunWorklet.cpp:1764-1782, 1854-1877
 // The list radius adds a Verlet skin; the list is rebuilt on the device when an
 // atom has moved more than half the skin since the last build.
 #include "state.h"
@@ -16,11 +17,6 @@ namespace rbmd {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ inline double min_image(double t, double L, double invL)
-{
-    return t - L * rint(t * invL);
-}
 
 // ---------------------------------------------------------------- per-step check
 // Max-displacement test against the positions of the last build; zeroes the
@@ -235,102 +231,6 @@ struct MdBin {
     }
 };
 
-// MD-step cell sort.  The tile pass of the previous step (or the binning before
-// the run) binned every atom into per-cell buckets of buffer epoch & 1, with its
-// wrapped position at that time.  On a rebuild step one wave takes one cell: its
-// start is the prefix sum of the counts of the cells before it (fixed order),
-// its atoms are the bucket's ids ordered by id -- or, when the bucket
-// overflowed, every atom binned to the cell, found by a scan in id order -- and
-// it writes the cell bounds, both permutations, the reference positions (the
-// binned ones: every atom is within skin/2 of them, which is what the list
-// needs), the sorted reference positions, the current positions unwrapped
-// against the new reference, charges, types and cells.  This replaces the
-// binning, scan and scatter kernels in the MD step.  Workgroup 0 records
-// whether this step rebuilds (flags[2]) for the tile pass, which then clears the
-// request.
-struct CellSortMD {
-    static constexpr int kThreads = 64;
-    struct Args {
-        const double *x, *y, *z, *q;
-        const int32_t* type;
-        int n, ncell;
-        Box box;
-        MdBins bins;
-        int32_t *cell_start, *perm, *inv_perm, *scell, *ts;
-        double *xr, *yr, *zr;
-        double4 *xs_ref, *xs;
-        uint32_t* flags;
-        unsigned long long* stats;
-    };
-    __device__ static void emit(const Args& a, int c, int v, int s)
-    {
-        a.perm[s] = v;
-        a.inv_perm[v] = s;
-        const double w0 = a.bins.xw[v], w1 = a.bins.yw[v], w2 = a.bins.zw[v];
-        a.xr[v] = w0;
-        a.yr[v] = w1;
-        a.zr[v] = w2;
-        const double qv = a.q[v];
-        a.xs_ref[s] = make_double4(w0, w1, w2, qv);
-        a.xs[s] = make_double4(w0 + min_image(a.x[v] - w0, a.box.L[0], a.box.invL[0]),
-                               w1 + min_image(a.y[v] - w1, a.box.L[1], a.box.invL[1]),
-                               w2 + min_image(a.z[v] - w2, a.box.L[2], a.box.invL[2]), qv);
-        a.scell[s] = c;
-        a.ts[s] = a.type ? a.type[v] : 0;
-    }
-    __device__ static void run(const Args& a, int blk, int)
-    {
-        const uint32_t req = a.flags[0];
-        if (blk == 0 && threadIdx.x == 0) {
-            a.flags[2] = req;
-            if (req) {
-                a.stats[0] += 1ull;
-                a.stats[1] = 0ull;
-                a.stats[2] = 0ull;
-                a.stats[3] = 0ull;
-                a.stats[4] = 0ull;
-                a.stats[6] = 0ull;
-            }
-        }
-        if (req == 0u) return;
-        __shared__ int32_t buf[kCellSortMax];
-        const int c = blk;
-        const int lane = threadIdx.x;
-        const int b = (int)(a.flags[3] & 1u);
-        const int32_t* count = a.bins.count + (size_t)b * a.ncell;
-        int part = 0;
-        for (int k = lane; k < c; k += 64) part += count[k];
-        const int s0 = wave_sum(part);
-        const int cnt = count[c];
-        if (lane == 0) {
-            a.cell_start[c] = s0;
-            if (c == a.ncell - 1) a.cell_start[a.ncell] = s0 + cnt;
-        }
-        if (cnt <= a.bins.cap) {
-            // bucket ids (arrival order) ranked by id in LDS; cap <= kCellSortMax
-            const int32_t* bk = a.bins.bucket + ((size_t)b * a.ncell + c) * a.bins.cap;
-            for (int k = lane; k < cnt; k += 64) buf[k] = bk[k];
-            __syncthreads();
-            for (int k = lane; k < cnt; k += 64) {
-                const int v = buf[k];
-                int r = 0;
-                for (int t = 0; t < cnt; ++t) r += (buf[t] < v) ? 1 : 0;
-                emit(a, c, v, s0 + r);
-            }
-        } else {
-            // overflowed bucket: every atom binned to the cell, in id order
-            int found = 0;
-            for (int base = 0; base < a.n; base += 64) {
-                const int i = base + lane;
-                const bool hit = i < a.n && a.bins.cell[i] == c;
-                const uint64_t m = __ballot(hit);
-                if (hit) emit(a, c, i, s0 + found + (int)mbcnt64(m));
-                found += __popcll(m);
-            }
-        }
-    }
-};
-
 // ---------------------------------------------------------------- Verlet list
 // (row builder list_wave and stage_ref live in tile.h, shared with the fused
 // list + pair pass)
@@ -391,117 +291,6 @@ struct Gather {
     }
 };
 
-// ---------------------------------------------------------------- integrator
-constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sampler
-
-// First half of the velocity-Verlet step: v += dt/2 F/m, x += v dt, wrap with
-// image flags, displacement check against the list build, and the gather of the
-// new positions into the cell-sorted array.  The first blocks also draw this
-// step's k-vectors (the sampler of k_sample) so that no extra launch is needed.
-template <typename T>
-struct KickDrift {
-    static constexpr int kThreads = kBlock;
-    struct Args {
-        double *x, *y, *z, *vx, *vy, *vz;
-        const T *fx, *fy, *fz;
-        int32_t* img;
-        const double* mass;
-        double dt;
-        int n;
-        Box box;
-        const double *xr, *yr, *zr;
-        double thr2;
-        uint32_t* flags;
-        const double* q;
-        const int32_t* inv_perm;
-        double4* xs;
-        SampleArgs sa;
-    };
-    __device__ static void run(const Args& a, int blk, int)
-    {
-        const SampleArgs& sa = a.sa;
-        const int i = blk * kBlock + threadIdx.x;
-        if (blk < (sa.P + 63) / 64) {
-            // the first ceil(P/64) blocks also draw this step's k-vectors (64 per
-            // block, one wave), with the CDF tables staged in LDS
-            __shared__ double cdf[kSamplerLds];
-            const int len = 2 * (sa.M.x + sa.M.y + sa.M.z) + 3;
-            const bool in_lds = len <= kSamplerLds;
-            if (in_lds)
-                for (int k = threadIdx.x; k < len; k += kBlock) cdf[k] = sa.cdf[k];
-            __syncthreads();
-            const int l = blk * 64 + threadIdx.x;
-            if (threadIdx.x < 64 && l < sa.P) {
-                const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
-                const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
-                int m[3];
-                const double w = draw_kvector(sa.seed, step, (uint32_t)l, in_lds ? cdf : sa.cdf,
-                                              Ms, sa.gp, sa.P, sa.w, sa.w_out, m);
-                sa.km[3 * l] = m[0];
-                sa.km[3 * l + 1] = m[1];
-                sa.km[3 * l + 2] = m[2];
-                sa.kw[l] = w;
-            }
-        }
-        if (i >= a.n) return;
-        const Box& box = a.box;
-        const double dt = a.dt;
-        const double h = 0.5 * dt / (a.mass ? a.mass[i] : 1.0);
-        double v[3] = {a.vx[i] + h * (double)a.fx[i], a.vy[i] + h * (double)a.fy[i],
-                       a.vz[i] + h * (double)a.fz[i]};
-        double p[3] = {a.x[i] + v[0] * dt, a.y[i] + v[1] * dt, a.z[i] + v[2] * dt};
-        int fl[3];
-        for (int d = 0; d < 3; ++d) {
-            double f = floor((p[d] - box.lo[d]) * box.invL[d]);
-            p[d] -= f * box.L[d];
-            fl[d] = (int)f;
-        }
-        a.vx[i] = v[0];
-        a.vy[i] = v[1];
-        a.vz[i] = v[2];
-        a.x[i] = p[0];
-        a.y[i] = p[1];
-        a.z[i] = p[2];
-        if (a.img) {
-            a.img[3 * i] += fl[0];
-            a.img[3 * i + 1] += fl[1];
-            a.img[3 * i + 2] += fl[2];
-        }
-        const double dx = min_image(p[0] - a.xr[i], box.L[0], box.invL[0]);
-        const double dy = min_image(p[1] - a.yr[i], box.L[1], box.invL[1]);
-        const double dz = min_image(p[2] - a.zr[i], box.L[2], box.invL[2]);
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 > a.thr2) a.flags[0] = 1u;
-        if (d2 != d2) a.flags[1] = 1u;
-        // the gather of the current positions, unwrapped against the reference
-        // (a rebuild later in the step rewrites them in the new order)
-        a.xs[a.inv_perm[i]] = make_double4(a.xr[i] + dx, a.yr[i] + dy, a.zr[i] + dz, a.q[i]);
-    }
-};
-
-template <typename T>
-struct Kick {
-    static constexpr int kThreads = kBlock;
-    struct Args {
-        double *vx, *vy, *vz;
-        const T *fx, *fy, *fz;
-        const double* mass;
-        double dt;
-        int n;
-        uint32_t* flags;
-    };
-    __device__ static void run(const Args& a, int blk, int)
-    {
-        const int i = blk * kBlock + threadIdx.x;
-        if (i == 0) a.flags[3] += 1u;  // step epoch of the MD bins
-        if (i >= a.n) return;
-        const double h = 0.5 * a.dt / (a.mass ? a.mass[i] : 1.0);
-        a.vx[i] += h * (double)a.fx[i];
-        a.vy[i] += h * (double)a.fy[i];
-        a.vz[i] += h * (double)a.fz[i];
-    }
-};
-
 __global__ __launch_bounds__(kBlock) void k_copy(uint4* __restrict__ dst,
                                                  const uint4* __restrict__ src, int64_t n16)
 {
@@ -546,47 +335,10 @@ Gather::Args gather_args(Engine& e, const double* x, const double* y, const doub
     return Gather::Args{x, y, z, e.xr, e.yr, e.zr, e.q, e.atom_rank, (int)e.n, e.box, e.xs};
 }
 
-template <typename T>
-typename KickDrift<T>::Args kickdrift_args(Engine& e, double* x, double* y, double* z,
-                                           double* vx, double* vy, double* vz, const void* fx,
-                                           const void* fy, const void* fz, int32_t* img,
-                                           const double* mass, double dt, const SampleArgs& sa)
-{
-    const double h = 0.5 * e.skin;
-    return typename KickDrift<T>::Args{x,  y,    z,    vx,   vy,  vz, (const T*)fx, (const T*)fy,
-                                       (const T*)fz, img, mass, dt, (int)e.n, e.box, e.xr,
-                                       e.yr, e.zr, h * h, e.flags, e.q, e.atom_rank, e.xs, sa};
-}
-
-CellSortMD::Args cellsort_md_args(Engine& e, const double* x, const double* y, const double* z)
-{
-    return CellSortMD::Args{x,      y,       z,          e.q,           e.type,
-                            (int)e.n, e.ncell, e.box,    md_bins(e),    e.cell_start,
-                            e.perm, e.atom_rank /* inverse perm */,     e.scell,
-                            e.ts,   e.xr,    e.yr,       e.zr,          e.xs_ref,
-                            e.xs,   e.flags, e.stats};
-}
-
 MdBin::Args mdbin_args(Engine& e, const double* x, const double* y, const double* z)
 {
     return MdBin::Args{x, y, z, (int)e.n, e.box, make_int3(e.dims[0], e.dims[1], e.dims[2]),
                        make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e), e.flags};
-}
-
-int kickdrift_blocks(const Engine& e, const SampleArgs& sa)
-{
-    int64_t cover = e.n;
-    if (cover < (int64_t)sa.P * kBlock / 64) cover = (int64_t)sa.P * kBlock / 64;
-    if (cover < 1) cover = 1;
-    return nblk(cover);
-}
-
-template <typename T>
-typename Kick<T>::Args kick_args(Engine& e, double* vx, double* vy, double* vz, const void* fx,
-                                 const void* fy, const void* fz, const double* mass, double dt)
-{
-    return typename Kick<T>::Args{vx, vy, vz, (const T*)fx, (const T*)fy, (const T*)fz, mass, dt,
-                                  (int)e.n, e.flags};
 }
 
 }  // namespace
@@ -630,13 +382,6 @@ void launch_md_bin(Engine& e, const double* x, const double* y, const double* z,
     k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z));
 }
 
-void launch_cellsort_md(Engine& e, const double* x, const double* y, const double* z,
-                        hipStream_t s)
-{
-    ScopedTimer t(e, P_CELLSORT, s);
-    k_one<CellSortMD><<<e.ncell, 64, 0, s>>>(cellsort_md_args(e, x, y, z));
-}
-
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s)
 {
     k_list_stats<<<1, 256, 0, s>>>(e.ncount, (int)e.n, out);
@@ -647,33 +392,6 @@ void launch_gather(Engine& e, const double* x, const double* y, const double* z,
     ScopedTimer t(e, P_GATHER, s);
     const int n = (int)e.n;
     k_one<Gather><<<nblk(n > 0 ? n : 1), kBlock, 0, s>>>(gather_args(e, x, y, z));
-}
-
-void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double* vx, double* vy,
-                             double* vz, const void* fx, const void* fy, const void* fz,
-                             int32_t* img, const double* mass, double dt, const SampleArgs& sa,
-                             hipStream_t s)
-{
-    ScopedTimer t(e, P_CHECK, s);
-    const int blocks = kickdrift_blocks(e, sa);
-    if (e.precision == 32)
-        k_one<KickDrift<float>><<<blocks, kBlock, 0, s>>>(
-            kickdrift_args<float>(e, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa));
-    else
-        k_one<KickDrift<double>><<<blocks, kBlock, 0, s>>>(
-            kickdrift_args<double>(e, x, y, z, vx, vy, vz, fx, fy, fz, img, mass, dt, sa));
-}
-
-void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,
-                 const void* fz, const double* mass, double dt, hipStream_t s)
-{
-    ScopedTimer t(e, P_KICK, s);
-    if (e.precision == 32)
-        k_one<Kick<float>><<<nblk(e.n), kBlock, 0, s>>>(
-            kick_args<float>(e, vx, vy, vz, fx, fy, fz, mass, dt));
-    else
-        k_one<Kick<double>><<<nblk(e.n), kBlock, 0, s>>>(
-            kick_args<double>(e, vx, vy, vz, fx, fy, fz, mass, dt));
 }
 
 void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s)
@@ -687,74 +405,47 @@ void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s)
 
 // ================================================================= batched phases
 
-template <typename T>
-static hipError_t nlist_prepare_t(Batch& b)
+hipError_t nlist_batch_prepare(Batch& b)
 {
     const size_t R = b.es.size();
-    std::vector<typename KickDrift<T>::Args> kd(R);
     std::vector<Bin::Args> bi(R);
     std::vector<Scan::Args> sc(R);
     std::vector<Scatter::Args> sp(R);
     std::vector<CellSort::Args> cs(R);
-    std::vector<CellSortMD::Args> cm(R);
     std::vector<MdBin::Args> mb(R);
     std::vector<Gather::Args> ga(R);
-    std::vector<typename Kick<T>::Args> ki(R);
-    std::vector<int> nkd(R), nat(R), none(R, 1), ncell(R);
+    std::vector<int> nat(R), none(R, 1), ncell(R);
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
         const RepState& st = b.st[r];
-        kd[r] = kickdrift_args<T>(e, st.x, st.y, st.z, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz,
-                                  st.img, st.mass, b.dt, b.sa[r]);
         bi[r] = bin_args(e, st.x, st.y, st.z);
         sc[r] = scan_args(e);
         sp[r] = scatter_args(e);
         cs[r] = cellsort_args(e);
-        cm[r] = cellsort_md_args(e, st.x, st.y, st.z);
         mb[r] = mdbin_args(e, st.x, st.y, st.z);
         ga[r] = gather_args(e, st.x, st.y, st.z);
-        ki[r] = kick_args<T>(e, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz, st.mass, b.dt);
-        nkd[r] = kickdrift_blocks(e, b.sa[r]);
         nat[r] = nblk(e.n > 0 ? e.n : 1);
         ncell[r] = e.ncell;
     }
     hipError_t err;
-    if ((err = phase_upload(b.ph[BP_KICKDRIFT], kd, nkd, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_BIN], bi, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_SCAN], sc, none, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_SCATTER], sp, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_CELLSORT], cs, ncell, 0)) != hipSuccess) return err;
-    if ((err = phase_upload(b.ph[BP_CELLSORT_MD], cm, ncell, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_MDBIN], mb, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_GATHER], ga, nat, 0)) != hipSuccess) return err;
-    if ((err = phase_upload(b.ph[BP_KICK], ki, nat, 0)) != hipSuccess) return err;
     return hipSuccess;
-}
-
-hipError_t nlist_batch_prepare(Batch& b)
-{
-    return b.precision == 32 ? nlist_prepare_t<float>(b) : nlist_prepare_t<double>(b);
 }
 
 void nlist_batch_launch(Batch& b, int phase, hipStream_t s)
 {
-    const bool f32 = b.precision == 32;
     switch (phase) {
-    case BP_KICKDRIFT:
-        if (f32) phase_launch<KickDrift<float>>(b.ph[phase], s);
-        else phase_launch<KickDrift<double>>(b.ph[phase], s);
-        break;
     case BP_BIN: phase_launch<Bin>(b.ph[phase], s); break;
     case BP_SCAN: phase_launch<Scan>(b.ph[phase], s); break;
     case BP_SCATTER: phase_launch<Scatter>(b.ph[phase], s); break;
     case BP_CELLSORT: phase_launch<CellSort>(b.ph[phase], s); break;
-    case BP_CELLSORT_MD: phase_launch<CellSortMD>(b.ph[phase], s); break;
     case BP_MDBIN: phase_launch<MdBin>(b.ph[phase], s); break;
     case BP_GATHER: phase_launch<Gather>(b.ph[phase], s); break;
-    case BP_KICK:
-        if (f32) phase_launch<Kick<float>>(b.ph[phase], s);
-        else phase_launch<Kick<double>>(b.ph[phase], s);
-        break;
     default: break;
     }
 }

@@ -13,6 +13,7 @@
 // Pair displacements are FP64 differences of the staged positions (no per-pair
 // minimum image), the force maths runs in T (float or double), and cutoff
 // decisions use the FP64 squared distance.
+#include "kspace.h"
 #include "state.h"
 #include "tile.h"
 
@@ -424,6 +425,28 @@ struct TilePass {
     }
 };
 
+// MD step: the rho(k) final reduction (workgroups [0, nfin), one per k) and the
+// tile pass (the rest) in one grid.  Both need only the previous kernel's
+// output (rho partials, cell-sorted positions), and the k-space force that
+// follows needs both.
+template <typename T, bool EV, bool M>
+struct TileRho {
+    static constexpr int kThreads = kTileThreads;
+    struct Args {
+        typename RhoFinalT<T, kTileThreads>::Args fin;
+        int nfin;
+        typename TilePass<T, EV, M>::Args tile;
+    };
+    __device__ static void run(const Args& A, int blk, int nblk)
+    {
+        if (blk < A.nfin) {
+            RhoFinalT<T, kTileThreads>::run(A.fin, blk, A.nfin);
+            return;
+        }
+        TilePass<T, EV, M>::run(A.tile, blk - A.nfin, nblk - A.nfin);
+    }
+};
+
 // Deterministic final reduction: pair partials [nb][8] + per-k terms [P][7]
 // + self energy -> ev[10] = {e_lj, e_real, e_self, e_k, W_xx..W_yz}.
 __global__ __launch_bounds__(256) void k_ev_final(const double* __restrict__ ev_part, int nb,
@@ -493,23 +516,43 @@ size_t pair_lds(const Engine& e)
 }
 
 template <typename T, bool EV, bool M>
-void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, bool md, hipStream_t s)
+void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
 {
     k_one<TilePass<T, EV, M>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
-        tile_args<T, EV, M>(e, fx, fy, fz, md));
+        tile_args<T, EV, M>(e, fx, fy, fz, false));
+}
+
+template <typename T, bool M>
+typename TileRho<T, false, M>::Args tile_rho_args(Engine& e, void* fx, void* fy, void* fz,
+                                                   unsigned long long* step_dev)
+{
+    typename TileRho<T, false, M>::Args a;
+    const bool ks = has_kspace(e);
+    if (ks) a.fin = rho_final_args<T, kTileThreads>(e, step_dev);
+    else memset(&a.fin, 0, sizeof a.fin);
+    a.nfin = ks ? e.P : 0;
+    a.tile = tile_args<T, false, M>(e, fx, fy, fz, true);
+    return a;
+}
+
+template <typename T, bool M>
+void tile_rho_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    const auto a = tile_rho_args<T, M>(e, fx, fy, fz, e.capturing ? e.d_step : nullptr);
+    k_one<TileRho<T, false, M>><<<a.nfin + e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(a);
 }
 
 template <typename T, bool M>
 hipError_t pair_prepare_t(Batch& b)
 {
     const size_t R = b.es.size();
-    std::vector<typename TilePass<T, false, M>::Args> ta(R);
+    std::vector<typename TileRho<T, false, M>::Args> ta(R);
     std::vector<int> nt(R);
     size_t lds = 0;
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
-        ta[r] = tile_args<T, false, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz, true);
-        nt[r] = e.ncell * e.split;
+        ta[r] = tile_rho_args<T, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz, e.d_step);
+        nt[r] = ta[r].nfin + e.ncell * e.split;
         const size_t l = pair_lds(e);
         lds = l > lds ? l : lds;
     }
@@ -518,15 +561,15 @@ hipError_t pair_prepare_t(Batch& b)
 
 }  // namespace
 
-void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s, bool md)
+void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s)
 {
     if (e.n == 0) return;
     ScopedTimer t(e, P_PAIR, s);
     const bool multi = e.pp.ntypes > 1;
 #define RBMD_PAIR_M(T, EV)                                                                    \
     do {                                                                                      \
-        if (multi) pair_launch_t<T, EV, true>(e, fx, fy, fz, md, s);                          \
-        else pair_launch_t<T, EV, false>(e, fx, fy, fz, md, s);                               \
+        if (multi) pair_launch_t<T, EV, true>(e, fx, fy, fz, s);                          \
+        else pair_launch_t<T, EV, false>(e, fx, fy, fz, s);                               \
     } while (0)
     if (e.precision == 32) {
         if (ev) RBMD_PAIR_M(float, true);
@@ -536,6 +579,20 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
         else RBMD_PAIR_M(double, false);
     }
 #undef RBMD_PAIR_M
+}
+
+void launch_tile_rho(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    if (e.n == 0) return;
+    ScopedTimer t(e, P_PAIR, s);
+    const bool multi = e.pp.ntypes > 1;
+    if (e.precision == 32) {
+        if (multi) tile_rho_launch_t<float, true>(e, fx, fy, fz, s);
+        else tile_rho_launch_t<float, false>(e, fx, fy, fz, s);
+    } else {
+        if (multi) tile_rho_launch_t<double, true>(e, fx, fy, fz, s);
+        else tile_rho_launch_t<double, false>(e, fx, fy, fz, s);
+    }
 }
 
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s)
@@ -556,7 +613,7 @@ void pair_batch_launch(Batch& b, int phase, hipStream_t s)
 {
 #define RBMD_PB(T, M)                                                                         \
     do {                                                                                      \
-        if (phase == BP_PAIR) phase_launch<TilePass<T, false, M>>(b.ph[phase], s);           \
+        if (phase == BP_PAIR) phase_launch<TileRho<T, false, M>>(b.ph[phase], s);           \
     } while (0)
     if (b.precision == 32) {
         if (b.multi) RBMD_PB(float, true);

@@ -10,6 +10,7 @@
 
 #include "engine.h"
 #include "launch.h"
+#include "../../include/rbmd_force.h"
 
 namespace rbmd {
 
@@ -157,19 +158,18 @@ struct RepState {
 
 // Phases of the batched MD step, one launch each over all replicas.
 enum BatchPh {
-    BP_KICKDRIFT = 0,
-    BP_BIN,
+    BP_KICKDRIFT = 0,  // segment opening kick/drift (md.hip)
+    BP_BIN,            // force-pass list build (nlist.hip)
     BP_SCAN,
     BP_SCATTER,
     BP_CELLSORT,
-    BP_CELLSORT_MD,
     BP_MDBIN,
     BP_GATHER,
-    BP_RHO,
-    BP_RHO_FINAL,
-    BP_PAIR,
-    BP_KFORCE,
-    BP_KICK,
+    BP_RHO,            // rho partials of the force pass (kspace.hip)
+    BP_SORT_RHO,       // MD step: rho partials + cell sort (md.hip)
+    BP_PAIR,           // rho final + tile pass (pair.hip)
+    BP_KFORCE,         // k-space force + closing kick (md.hip)
+    BP_KFORCE_DRIFT,   // ... fused with the next step's opening kick/drift (md.hip)
     BP_NUM
 };
 
@@ -192,6 +192,13 @@ hipError_t pair_batch_prepare(Batch& b);
 void pair_batch_launch(Batch& b, int phase, hipStream_t s);
 hipError_t kspace_batch_prepare(Batch& b);
 void kspace_batch_launch(Batch& b, int phase, hipStream_t s);
+hipError_t md_batch_prepare(Batch& b);
+void md_batch_launch(Batch& b, int phase, hipStream_t s);
+
+inline bool has_kspace(const Engine& e)
+{
+    return (e.kspace == RBMD_KSPACE_RBE || e.kspace == RBMD_KSPACE_EWALD) && e.P > 0;
+}
 
 inline MdBins md_bins(const Engine& e)
 {
@@ -205,21 +212,13 @@ void launch_check(Engine& e, const double* x, const double* y, const double* z, 
 void launch_force_rebuild(Engine& e, hipStream_t s);
 void launch_build(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
 void launch_gather(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
-void launch_kick_drift_check(Engine& e, double* x, double* y, double* z, double* vx, double* vy,
-                             double* vz, const void* fx, const void* fy, const void* fz,
-                             int32_t* img, const double* mass, double dt, const SampleArgs& sa,
-                             hipStream_t s);
 void launch_md_bin(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
-void launch_cellsort_md(Engine& e, const double* x, const double* y, const double* z,
-                        hipStream_t s);
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s);
-void launch_kick(Engine& e, double* vx, double* vy, double* vz, const void* fx, const void* fy,
-                 const void* fz, const double* mass, double dt, hipStream_t s);
 void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s);
 
 // ---- launchers (pair.hip)
-void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s,
-                 bool md = false);
+void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s);
+void launch_tile_rho(Engine& e, void* fx, void* fy, void* fz, hipStream_t s);
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s);
 
 // ---- launchers (kspace.hip)
@@ -229,8 +228,15 @@ void launch_sample(Engine& e, int P, uint64_t seed, const uint64_t* step_host, h
 void launch_kweights_default(Engine& e, hipStream_t s);
 void launch_rho(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
 void launch_kforce(Engine& e, const double* x, const double* y, const double* z, void* fx,
-                   void* fy, void* fz, double* vx, double* vy, double* vz, const double* mass,
-                   double dt, hipStream_t s);
+                   void* fy, void* fz, hipStream_t s);
+
+// ---- launchers (md.hip)
+void launch_md_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
+                    hipStream_t s);
+void launch_md_sort_rho(Engine& e, const RepState& st, hipStream_t s);
+void launch_md_close(Engine& e, const RepState& st, double dt, hipStream_t s);
+void launch_md_close_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
+                          hipStream_t s);
 
 // timing helpers (api.hip)
 void timer_begin(Engine& e, int pass, hipStream_t s, hipEvent_t* ev);

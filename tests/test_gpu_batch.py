@@ -178,9 +178,15 @@ def test_batch_autotune_then_run(gpu):
     b.md_run([tuple(st) for st in states], dt=0.002, nsteps=20, seeds=[40 + r for r in
              range(len(sizes))], step0=1)
     torch.cuda.synchronize()
-    for st, f in zip(states, b.replicas):
+    from copy_gpu_amd import systems
+
+    for r, (st, f, sysr) in enumerate(zip(states, b.replicas, syss)):
         F = torch.stack(st[6:9], 1).double()
         assert torch.isfinite(F).all()
+        # the last step (20) used the draws of step 20; the handle now holds step 21's
+        L = sysr[4]
+        m, _ = systems.sample_kvectors_host(0.5, np.array([L] * 3), 100, 40 + r, 20)
+        f.set_kvectors(_dev(m.reshape(-1), torch.int32))
         Fc = [torch.zeros_like(st[6]) for _ in range(3)]
         f.Execute(*st[:3], *Fc)
         Fc = torch.stack(Fc, 1).double()
