@@ -260,22 +260,35 @@ struct KForceLds {
     T red[kKfSlices][3][kKfAtoms];
 };
 
-// The k-space force of the 64 atoms of workgroup blk; each lane of wave 0 then
-// calls fin(i, f0, f1, f2) with the complete force (pair force + k-space) of
-// its atom i < n.  fin runs with FP contraction off in the caller's body.
-template <typename T, class Fin>
-__device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T>& L, Fin fin)
+// The k-space force of the 64 atoms of workgroup blk.  Each lane of wave 0
+// first calls pre(i) for its atom i < n (loads that the completion needs, issued
+// before the k-sum so that their latency overlaps it), then, after the k-sum,
+// fin(i, f0, f1, f2, pre_value) with the complete force (pair force + k-space).
+// fin runs with FP contraction off in the caller's body.
+template <typename T, class Pre, class Fin>
+__device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T>& L, Pre pre,
+                                    Fin fin)
 {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blk * kKfAtoms + lane;
     const int n = A.n, P = A.P;
     const Box& box = A.box;
+    const bool own = wv == 0 && i < n;
     double sx = 0, sy = 0, sz = 0;
     if (i < n && P > 0) {
         sx = (A.x[i] - box.lo[0]) * box.invL[0];
         sy = (A.y[i] - box.lo[1]) * box.invL[1];
         sz = (A.z[i] - box.lo[2]) * box.invL[2];
+    }
+    T f0 = 0, f1 = 0, f2 = 0, qi = 0;
+    decltype(pre(0)) pv{};
+    if (own) {
+        f0 = A.fx[i];
+        f1 = A.fy[i];
+        f2 = A.fz[i];
+        if (P > 0) qi = (T)A.q[i];
+        pv = pre(i);
     }
     T ax = 0, ay = 0, az = 0;
     for (int base = 0; base < P; base += kKfStage) {
@@ -301,8 +314,7 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
         L.red[wv][2][lane] = az;
         __syncthreads();
     }
-    if (wv == 0 && i < n) {
-        T f0 = A.fx[i], f1 = A.fy[i], f2 = A.fz[i];
+    if (own) {
         if (P > 0) {
             T s0 = 0, s1 = 0, s2 = 0;
             for (int w = 0; w < kKfSlices; ++w) {
@@ -310,14 +322,18 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
                 s1 += L.red[w][1][lane];
                 s2 += L.red[w][2][lane];
             }
-            const T qi = (T)A.q[i];
             f0 = f0 + qi * s0;
             f1 = f1 + qi * s1;
             f2 = f2 + qi * s2;
         }
-        fin(i, f0, f1, f2);
+        fin(i, f0, f1, f2, pv);
     }
 }
+
+// Velocity and mass of atom i for the closing half kick.
+struct KickIn {
+    double v[3], m;
+};
 
 // F += k-space force; with KICK the closing half kick v += dt/2 F/m of the
 // velocity-Verlet step follows (and the MD step epoch advances).
@@ -330,18 +346,30 @@ struct KForce {
 #pragma clang fp contract(off)
         if (KICK && blk == 0 && threadIdx.x == 0) A.flags[3] += 1u;  // MD step epoch
         __shared__ KForceLds<T> L;
-        kforce_block(A, blk, L, [&](int i, T f0, T f1, T f2) {
+        kforce_block(
+            A, blk, L,
+            [&](int i) {
+                KickIn k{};
+                if (KICK) {
+                    k.v[0] = A.vx[i];
+                    k.v[1] = A.vy[i];
+                    k.v[2] = A.vz[i];
+                    k.m = A.mass ? A.mass[i] : 1.0;
+                }
+                return k;
+            },
+            [&](int i, T f0, T f1, T f2, const KickIn& k) {
 #pragma clang fp contract(off)
-            A.fx[i] = f0;
-            A.fy[i] = f1;
-            A.fz[i] = f2;
-            if (KICK) {
-                const double h = 0.5 * A.dt / (A.mass ? A.mass[i] : 1.0);
-                A.vx[i] = A.vx[i] + h * (double)f0;
-                A.vy[i] = A.vy[i] + h * (double)f1;
-                A.vz[i] = A.vz[i] + h * (double)f2;
-            }
-        });
+                A.fx[i] = f0;
+                A.fy[i] = f1;
+                A.fz[i] = f2;
+                if (KICK) {
+                    const double h = 0.5 * A.dt / k.m;
+                    A.vx[i] = k.v[0] + h * (double)f0;
+                    A.vy[i] = k.v[1] + h * (double)f1;
+                    A.vz[i] = k.v[2] + h * (double)f2;
+                }
+            });
     }
 };
 

@@ -73,21 +73,49 @@ struct KickDriftArgs {
     SampleArgs sa;
 };
 
+// Per-atom inputs of the kick/drift besides velocity and force.
+struct KdIn {
+    double x[3], xr[3], m, q;
+    int32_t slot, img[3];
+};
+
+template <typename T>
+__device__ inline KdIn kd_load(const KickDriftArgs<T>& a, int i)
+{
+    KdIn in;
+    in.x[0] = a.x[i];
+    in.x[1] = a.y[i];
+    in.x[2] = a.z[i];
+    in.xr[0] = a.xr[i];
+    in.xr[1] = a.yr[i];
+    in.xr[2] = a.zr[i];
+    in.m = a.mass ? a.mass[i] : 1.0;
+    in.q = a.q[i];
+    in.slot = a.inv_perm[i];
+    in.img[0] = in.img[1] = in.img[2] = 0;
+    if (a.img) {
+        in.img[0] = a.img[3 * i];
+        in.img[1] = a.img[3 * i + 1];
+        in.img[2] = a.img[3 * i + 2];
+    }
+    return in;
+}
+
 // Opening half kick v += dt/2 F/m from velocity v0 and force f, drift
 // x += v dt, wrap into the box with image-flag update, displacement check
 // against the reference positions of the last build, and the gather of the new
 // position into the cell-sorted array, unwrapped against the reference
 // (x_ref + min_image(x - x_ref)) so that every list pair keeps its image.
 template <typename T>
-__device__ inline void kick_drift_atom(const KickDriftArgs<T>& a, int i, double v0, double v1,
-                                       double v2, T f0, T f1, T f2)
+__device__ inline void kick_drift_atom(const KickDriftArgs<T>& a, int i, const KdIn& in, double v0,
+                                       double v1, double v2, T f0, T f1, T f2)
 {
 #pragma clang fp contract(off)
     const Box& box = a.box;
     const double dt = a.dt;
-    const double h = 0.5 * dt / (a.mass ? a.mass[i] : 1.0);
+    const double h = 0.5 * dt / in.m;
     double v[3] = {v0 + h * (double)f0, v1 + h * (double)f1, v2 + h * (double)f2};
-    double p[3] = {a.x[i] + v[0] * dt, a.y[i] + v[1] * dt, a.z[i] + v[2] * dt};
+    double p[3] = {in.x[0] + v[0] * dt, in.x[1] + v[1] * dt, in.x[2] + v[2] * dt};
     int fl[3];
     for (int d = 0; d < 3; ++d) {
         const double f = floor((p[d] - box.lo[d]) * box.invL[d]);
@@ -101,17 +129,17 @@ __device__ inline void kick_drift_atom(const KickDriftArgs<T>& a, int i, double 
     a.y[i] = p[1];
     a.z[i] = p[2];
     if (a.img) {
-        a.img[3 * i] += fl[0];
-        a.img[3 * i + 1] += fl[1];
-        a.img[3 * i + 2] += fl[2];
+        a.img[3 * i] = in.img[0] + fl[0];
+        a.img[3 * i + 1] = in.img[1] + fl[1];
+        a.img[3 * i + 2] = in.img[2] + fl[2];
     }
-    const double dx = min_image(p[0] - a.xr[i], box.L[0], box.invL[0]);
-    const double dy = min_image(p[1] - a.yr[i], box.L[1], box.invL[1]);
-    const double dz = min_image(p[2] - a.zr[i], box.L[2], box.invL[2]);
+    const double dx = min_image(p[0] - in.xr[0], box.L[0], box.invL[0]);
+    const double dy = min_image(p[1] - in.xr[1], box.L[1], box.invL[1]);
+    const double dz = min_image(p[2] - in.xr[2], box.L[2], box.invL[2]);
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (d2 > a.thr2) a.flags[0] = 1u;
     if (d2 != d2) a.flags[1] = 1u;
-    a.xs[a.inv_perm[i]] = make_double4(a.xr[i] + dx, a.yr[i] + dy, a.zr[i] + dz, a.q[i]);
+    a.xs[in.slot] = make_double4(in.xr[0] + dx, in.xr[1] + dy, in.xr[2] + dz, in.q);
 }
 
 // Opening kernel of a segment: kick/drift of every atom (one per thread); the
@@ -128,19 +156,25 @@ struct KickDrift {
         }
         const int i = blk * kMdBlock + threadIdx.x;
         if (i >= a.n) return;
-        kick_drift_atom(a, i, a.vx[i], a.vy[i], a.vz[i], a.fx[i], a.fy[i], a.fz[i]);
+        kick_drift_atom(a, i, kd_load(a, i), a.vx[i], a.vy[i], a.vz[i], a.fx[i], a.fy[i],
+                        a.fz[i]);
     }
 };
 
 // Closing half kick of step s fused with the opening kick/drift of step s+1
 // (and its k-vector draw): the lane that completes atom i's force carries its
-// velocity through both half kicks.
+// velocity through both half kicks; the atom's other inputs are loaded before
+// the k-space sum.
 template <typename T>
 struct KForceDrift {
     static constexpr int kThreads = kKfThreads;
     struct Args {
         KForceArgs<T> kf;
         KickDriftArgs<T> kd;
+    };
+    struct In {
+        double v[3];
+        KdIn kd;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
@@ -155,17 +189,27 @@ struct KForceDrift {
             __syncthreads();  // the LDS is reused for the coefficients
         }
         const KForceArgs<T>& kf = A.kf;
-        kforce_block(kf, blk, lds.kf, [&](int i, T f0, T f1, T f2) {
+        kforce_block(
+            kf, blk, lds.kf,
+            [&](int i) {
+                In in;
+                in.v[0] = kf.vx[i];
+                in.v[1] = kf.vy[i];
+                in.v[2] = kf.vz[i];
+                in.kd = kd_load(A.kd, i);
+                return in;
+            },
+            [&](int i, T f0, T f1, T f2, const In& in) {
 #pragma clang fp contract(off)
-            kf.fx[i] = f0;
-            kf.fy[i] = f1;
-            kf.fz[i] = f2;
-            const double h = 0.5 * kf.dt / (kf.mass ? kf.mass[i] : 1.0);
-            const double v0 = kf.vx[i] + h * (double)f0;
-            const double v1 = kf.vy[i] + h * (double)f1;
-            const double v2 = kf.vz[i] + h * (double)f2;
-            kick_drift_atom(A.kd, i, v0, v1, v2, f0, f1, f2);
-        });
+                kf.fx[i] = f0;
+                kf.fy[i] = f1;
+                kf.fz[i] = f2;
+                const double h = 0.5 * kf.dt / in.kd.m;
+                const double v0 = in.v[0] + h * (double)f0;
+                const double v1 = in.v[1] + h * (double)f1;
+                const double v2 = in.v[2] + h * (double)f2;
+                kick_drift_atom(A.kd, i, in.kd, v0, v1, v2, f0, f1, f2);
+            });
     }
 };
 

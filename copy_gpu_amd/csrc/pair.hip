@@ -22,13 +22,16 @@ namespace rbmd {
 namespace {
 
 
-// erfc(z) e^{z^2} for z >= 0 in FP32, Abramowitz & Stegun 7.1.26 (Hastings):
+// erfc(z) e^{z^2} for z = sqrt(alpha) r >= 0 in FP32, Abramowitz & Stegun 7.1.26
+// (Hastings):
 //   erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) e^{-z^2} + eps,
 //   t = 1 / (1 + p z),  |eps| <= 1.5e-7 absolute.
-// The caller multiplies by e^{-z^2}, which the Gaussian term of G shares.
-__device__ inline float erfc_scaled_as(float z)
+// pa = p sqrt(alpha) is folded by the caller, so t = 1 / (1 + pa r).  The caller
+// multiplies by e^{-z^2}, which the Gaussian term of G shares.
+constexpr double kErfcP = 0.3275911;
+__device__ inline float erfc_scaled_as(float r, float pa)
 {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+    const float t = __builtin_amdgcn_rcpf(fmaf(pa, r, 1.0f));
     float p = 1.061405429f;
     p = fmaf(p, t, -1.453152027f);
     p = fmaf(p, t, 1.421413741f);
@@ -42,6 +45,8 @@ struct PairScalars {
     double alpha, sqrt_alpha, tsp, qqr2e;
     double eps24, sig2;  // type 0 - type 0 (ntypes == 1)
     int ntypes;
+    float nal2e;  // -alpha log2(e): e^{-alpha r^2} = 2^{nal2e r^2}
+    float pa;     // kErfcP sqrt(alpha)
 };
 
 // Scalar force factor fs (force on i = fs * r_ij, r_ij = r_j - r_i) and, with
@@ -49,31 +54,36 @@ struct PairScalars {
 template <typename T>
 struct PairMath;
 
+// FP32 maths without branches: every lane evaluates both terms and the cutoff
+// tests (FP64 d2) select, so a wave needs no exec-mask regions per pair; lanes
+// with ok == false (padding, d2 <= 1e-4) contribute exactly zero.
 template <>
 struct PairMath<float> {
     template <bool LJ>
     __device__ static inline float eval(double d2, double qq_d, float eps24, float sig2,
-                                        const PairScalars& ps, float* e_lj, float* e_re, bool ev)
+                                        const PairScalars& ps, bool ok, float* e_lj,
+                                        float* e_re, bool ev)
     {
         const float r2 = (float)d2;
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         float fs = 0.0f;
-        if (LJ && d2 < ps.rcl2) {
+        if (LJ) {
             const float sr2 = sig2 * r2inv;
             const float sr6 = sr2 * sr2 * sr2;
-            fs = -eps24 * (2.0f * sr6 - 1.0f) * sr6 * r2inv;
-            if (ev) *e_lj += 0.5f * (eps24 * (1.0f / 6.0f)) * (sr6 * sr6 - sr6);
+            const bool in = ok && d2 < ps.rcl2;
+            fs = in ? -eps24 * (2.0f * sr6 - 1.0f) * sr6 * r2inv : 0.0f;
+            if (ev && in) *e_lj += 0.5f * (eps24 * (1.0f / 6.0f)) * (sr6 * sr6 - sr6);
         }
-        if (d2 < ps.rcc2) {
-            const float qq = (float)qq_d;
-            const float r = r2 * rinv;
-            const float e2 = __expf(-(float)ps.alpha * r2);
-            const float erfcv = erfc_scaled_as((float)ps.sqrt_alpha * r) * e2;
-            const float g = erfcv * r2inv + (float)ps.tsp * e2 * rinv;
-            fs -= qq * g * rinv;
-            if (ev) *e_re += 0.5f * qq * erfcv * rinv;
-        }
+        // G(r) / r = e2 r2inv (P rinv + tsp), P = erfc(sqrt(alpha) r) e^{alpha r^2}
+        const float qq = (float)qq_d;
+        const float r = r2 * rinv;
+        const float e2 = __builtin_amdgcn_exp2f(ps.nal2e * r2);
+        const float P = erfc_scaled_as(r, ps.pa);
+        const float gr = e2 * r2inv * fmaf(P, rinv, (float)ps.tsp);
+        const bool in = ok && d2 < ps.rcc2;
+        fs = in ? fmaf(-qq, gr, fs) : fs;
+        if (ev && in) *e_re += 0.5f * qq * (P * e2) * rinv;
         return fs;
     }
 };
@@ -82,10 +92,11 @@ template <>
 struct PairMath<double> {
     template <bool LJ>
     __device__ static inline double eval(double d2, double qq, double eps24, double sig2,
-                                         const PairScalars& ps, double* e_lj, double* e_re,
-                                         bool ev)
+                                         const PairScalars& ps, bool ok, double* e_lj,
+                                         double* e_re, bool ev)
     {
         double fs = 0.0;
+        if (!ok) return fs;
         if (LJ && d2 < ps.rcl2) {
             const double sr2 = sig2 / d2;
             const double sr6 = sr2 * sr2 * sr2;
@@ -108,11 +119,12 @@ struct PairMath<double> {
 // carries the Coulomb unit factor.
 template <typename T, bool EV, bool MULTI, bool LJ>
 __device__ inline void accumulate_pair(double dx, double dy, double dz, double qq, int ti, int tj,
-                                       const PairScalars& ps, const PairParams* __restrict__ pp,
-                                       T& fx, T& fy, T& fz, double* acc)
+                                       bool valid, const PairScalars& ps,
+                                       const PairParams* __restrict__ pp, T& fx, T& fy, T& fz,
+                                       double* acc)
 {
     const double d2 = dx * dx + dy * dy + dz * dz;
-    if (!(d2 > 1e-4)) return;
+    const bool ok = valid && d2 > 1e-4;
     T eps24 = 0, sig2 = 0;
     if (LJ) {
         if (!MULTI) {
@@ -124,11 +136,11 @@ __device__ inline void accumulate_pair(double dx, double dy, double dz, double q
         }
     }
     T e_lj = 0, e_re = 0;
-    const T fs = PairMath<T>::template eval<LJ>(d2, qq, eps24, sig2, ps, &e_lj, &e_re, EV);
+    const T fs = PairMath<T>::template eval<LJ>(d2, qq, eps24, sig2, ps, ok, &e_lj, &e_re, EV);
     fx += fs * (T)dx;
     fy += fs * (T)dy;
     fz += fs * (T)dz;
-    if (EV) {
+    if (EV && ok) {
         const double h = -0.5 * (double)fs;
         acc[0] += (double)e_lj;
         acc[1] += (double)e_re;
@@ -162,7 +174,12 @@ __device__ inline int tile_type(const TileHdr& h, const int32_t* __restrict__ lt
     return ts[h.gstart[st] + (u - h.off[st])];
 }
 
-constexpr int kRound = 2;    // 16-entry chunks per atom per round
+// Row chunks per atom per round, and lanes per home-atom row (16 or 32).  On the
+// 32,768-ion workload, 16 lanes x 2 chunks measured fastest (3 chunks raise the
+// register count past the occupancy the LDS stage allows; 32 lanes shorten the
+// rows but leave more rounds per workgroup).
+constexpr int kRound = 2;
+constexpr int kGroup = 16;
 
 struct PairArgs {
     const double4* xs;
@@ -173,45 +190,52 @@ struct PairArgs {
     int cap;
 };
 
-// Sum over the 16 lanes of each quarter-wave, delivered to all 16: quad xor 1,
-// quad xor 2, half-row mirror, row mirror (DPP lane moves, no LDS).
+// Sum over the kGroup lanes of each lane group: quad xor 1, quad xor 2,
+// half-row mirror, row mirror (row sums in all 16 lanes of each row), and for
+// 32-lane groups row 1 += row 0, row 3 += row 2 (row broadcast 15).  The group
+// sum is valid in lane kSumLane of the group; DPP lane moves, no LDS.
+constexpr int kSumLane = kGroup == 32 ? 16 : 0;
+
 template <typename T>
-__device__ inline T group16_sum(T v);
+__device__ inline T group_sum(T v);
 
 template <>
-__device__ inline float group16_sum<float>(float v)
+__device__ inline float group_sum<float>(float v)
 {
     v += __int_as_float(dpp_i<0xB1>(__float_as_int(v)));
     v += __int_as_float(dpp_i<0x4E>(__float_as_int(v)));
     v += __int_as_float(dpp_i<0x141>(__float_as_int(v)));
     v += __int_as_float(dpp_i<0x140>(__float_as_int(v)));
+    if (kGroup == 32) v += __int_as_float(dpp_i<0x142, 0xa>(__float_as_int(v)));
     return v;
 }
 
 template <>
-__device__ inline double group16_sum<double>(double v)
+__device__ inline double group_sum<double>(double v)
 {
     v += dpp_d<0xB1>(v);
     v += dpp_d<0x4E>(v);
     v += dpp_d<0x141>(v);
     v += dpp_d<0x140>(v);
+    if (kGroup == 32) v += dpp_d<0x142, 0xa>(v);
     return v;
 }
 
-// Maximum of an int over the wave (all lanes get it).
+// Maximum of an int over the wave, for a value uniform over each lane group.
 __device__ inline int wave_max_int(int v)
 {
-    const int a = max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16));
-    const int b = max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48));
-    return max(a, b);
+    int m = max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+    if (kGroup == 16)
+        m = max(m, max(__builtin_amdgcn_readlane(v, 16), __builtin_amdgcn_readlane(v, 48)));
+    return m;
 }
 
-// One wave = four quarter-waves of 16 lanes, each walking one home atom's row
-// (atoms s = a0 + 4 wv + g + 32 t of the workgroup's range).  A round covers
-// 4 x 16 entries per atom: all index loads, then all LDS gathers, then the
-// maths; rounds that can still hold LJ-range entries of any of the four atoms
-// evaluate LJ, later rounds skip it.  Rows that overflowed their capacity scan
-// every tile candidate instead.
+// One wave = 64 / kGroup lane groups, each walking one home atom's row (the
+// workgroup's atoms are dealt to the groups in turn).  A round covers
+// kRound x kGroup entries per atom: all index loads, then all LDS gathers, then
+// the maths; rounds that can still hold LJ-range entries of any of the wave's
+// atoms evaluate LJ, later rounds skip it.  Rows that overflowed their capacity
+// scan every tile candidate instead.
 template <typename T, bool EV, bool MULTI, bool STAGED>
 __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
                                  const double4* __restrict__ lds,
@@ -222,8 +246,9 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
 {
     const int total = h.off[27];
     const int cap = a.cap;
-    const int g = lane >> 4, gl = lane & 15;
-    for (int t0 = a0 + 4 * wv; t0 < a1; t0 += 4 * kTileWaves) {
+    constexpr int kPer = 64 / kGroup;  // atoms per wave
+    const int g = lane / kGroup, gl = lane % kGroup;
+    for (int t0 = a0 + kPer * wv; t0 < a1; t0 += kPer * kTileWaves) {
         const int s = t0 + g;
         const bool live = s < a1;
         const int2 nc = live ? a.ncount[s] : make_int2(0, 0);
@@ -238,14 +263,24 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
         const double qi = pi.w * ps.qqr2e;
         const uint16_t* __restrict__ row = a.nlist + (size_t)(live ? s : h0) * cap;
         T ax = 0, ay = 0, az = 0;
-        for (int base = 0; base < tmax; base += 16 * kRound) {
-            int u[kRound];
+        // row indices of round base (tile-local, -1 past the row's end)
+        auto load_u = [&](int base, int* u) {
 #pragma unroll
             for (int k = 0; k < kRound; ++k) {
-                const int v = base + 16 * k + gl;
+                const int v = base + kGroup * k + gl;
                 const int idx = v < n_in ? v : cap - 1 - (v - n_in);
                 u[k] = v < tl ? (int)row[idx] : -1;
             }
+        };
+        int un[kRound];
+        load_u(0, un);
+        for (int base = 0; base < tmax; base += kGroup * kRound) {
+            // software pipeline: the next round's index loads are in flight
+            // during this round's gathers and maths
+            int u[kRound];
+#pragma unroll
+            for (int k = 0; k < kRound; ++k) u[k] = un[k];
+            if (base + kGroup * kRound < tmax) load_u(base + kGroup * kRound, un);
             double4 pj[kRound];
             int tj[kRound] = {};
 #pragma unroll
@@ -256,34 +291,29 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
             }
 #pragma unroll
             for (int k = 0; k < kRound; ++k) {
-                if (base + 16 * k < nin_max) {
-                    if (u[k] >= 0)
-                        accumulate_pair<T, EV, MULTI, true>(pj[k].x - pi.x, pj[k].y - pi.y,
-                                                            pj[k].z - pi.z, qi * pj[k].w, ti,
-                                                            tj[k], ps, pp, ax, ay, az, acc);
-                } else if (base + 16 * k < tmax) {
-                    if (u[k] >= 0)
-                        accumulate_pair<T, EV, MULTI, false>(pj[k].x - pi.x, pj[k].y - pi.y,
-                                                             pj[k].z - pi.z, qi * pj[k].w, ti,
-                                                             tj[k], ps, pp, ax, ay, az, acc);
-                }
+                if (base + kGroup * k < nin_max)
+                    accumulate_pair<T, EV, MULTI, true>(pj[k].x - pi.x, pj[k].y - pi.y,
+                                                        pj[k].z - pi.z, qi * pj[k].w, ti, tj[k],
+                                                        u[k] >= 0, ps, pp, ax, ay, az, acc);
+                else if (base + kGroup * k < tmax)
+                    accumulate_pair<T, EV, MULTI, false>(pj[k].x - pi.x, pj[k].y - pi.y,
+                                                         pj[k].z - pi.z, qi * pj[k].w, ti, tj[k],
+                                                         u[k] >= 0, ps, pp, ax, ay, az, acc);
             }
         }
         if (__ballot(live && over) != 0ull) {
-            for (int uu = gl; uu < total; uu += 16) {
-                if (live && over) {
-                    const double4 pj = tile_pos<STAGED>(h, lds, a.xs, uu);
-                    const int tj = MULTI ? tile_type<STAGED>(h, ltype, a.ts, uu) : 0;
-                    accumulate_pair<T, EV, MULTI, true>(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
-                                                        qi * pj.w, ti, tj, ps, pp, ax, ay, az,
-                                                        acc);
-                }
+            for (int uu = gl; uu < total; uu += kGroup) {
+                const double4 pj = tile_pos<STAGED>(h, lds, a.xs, uu);
+                const int tj = MULTI ? tile_type<STAGED>(h, ltype, a.ts, uu) : 0;
+                accumulate_pair<T, EV, MULTI, true>(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
+                                                    qi * pj.w, ti, tj, live && over, ps, pp, ax,
+                                                    ay, az, acc);
             }
         }
-        ax = group16_sum(ax);
-        ay = group16_sum(ay);
-        az = group16_sum(az);
-        if (live && gl == 0) {
+        ax = group_sum(ax);
+        ay = group_sum(ay);
+        az = group_sum(az);
+        if (live && gl == kSumLane) {
             const int i = a.perm[s];
             fx[i] = ax;
             fy[i] = ay;
@@ -292,11 +322,6 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
     }
 }
 
-// One workgroup per (home cell, part).  The tile's current positions are staged
-// in LDS shifted to the home cell's image (and types when there are several);
-// each 16-lane quarter-wave walks one home atom's uint16 row -- LJ-range
-// entries first, so later rounds skip the LJ code -- then reduces across its 16
-// lanes.  Rows that overflowed their capacity scan every tile candidate instead.
 // Per-workgroup FP64 energy / virial partial: wave sums, then the 8 waves in order.
 __device__ inline void block_ev(const double* acc, int lane, int wv, double* __restrict__ out)
 {
@@ -318,10 +343,9 @@ __device__ inline void block_ev(const double* acc, int lane, int wv, double* __r
 // builds the Verlet rows of its home atoms (list_wave over the tile's reference
 // positions), then the pair pass: the tile's current positions are staged in
 // LDS shifted to the home cell's image (and types when there are several);
-// each 16-lane quarter-wave walks one home atom's uint16 row -- LJ-range
-// entries first, so later rounds skip the LJ code -- then reduces across its 16
-// lanes.  Tiles larger than the LDS stage take the same steps from global
-// memory.  The rows of a home atom are written and read by the same workgroup,
+// each 32-lane half-wave walks one home atom's uint16 row -- LJ-range entries
+// first, so later rounds skip the LJ code -- then reduces across its 32 lanes.
+// Tiles larger than the LDS stage take the same steps from global memory.  The rows of a home atom are written and read by the same workgroup,
 // so the list build and the pair pass need no launch in between.
 template <typename T, bool EV, bool MULTI>
 struct TilePass {
@@ -496,6 +520,8 @@ PairScalars pair_scalars(const Engine& e)
     ps.eps24 = e.pp.eps24[0];
     ps.sig2 = e.pp.sig2[0];
     ps.ntypes = e.pp.ntypes;
+    ps.nal2e = (float)(-e.pp.alpha * 1.4426950408889634);
+    ps.pa = (float)(kErfcP * e.pp.sqrt_alpha);
     return ps;
 }
 
