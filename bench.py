@@ -451,6 +451,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ewald", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the library's tile-shape autotuner on the starting configuration "
+                         "before the timed steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB three times with the library's plain copy kernel "
                          "(known HBM traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -529,6 +532,10 @@ def main():
         f.Init(Q)
         f.sample_kvectors(P, SEED, 0)
         f.Execute(st.X, st.Y, st.Z, *st.F)
+        if args.tune:
+            t = time.perf_counter()
+            shapes = f.autotune(st.X, st.Y, st.Z, *st.F)
+            log(f"autotune ({time.perf_counter() - t:.2f} s): {shapes}")
         log(f"warmup {W} steps, timed {K} steps (graph={use_graph})")
         el, rebuilds, step = timed(f, st, wl, W, K, 1, use_graph)
         value = n * K / el

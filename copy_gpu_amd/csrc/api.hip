@@ -184,9 +184,18 @@ static void tile_geometry(Engine& E, int64_t n)
     const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
     const double cellv = E.cell[0] * E.cell[1] * E.cell[2];
     const double mean = 27.0 * (double)n / V * cellv;
-    int cap = ((int)(E.tune_stage_scale * mean + 64.0) + 63) / 64 * 64;
     const int per = (int)sizeof(double4) + (E.pp.ntypes > 1 ? 4 : 0);
     const int max_entries = (E.lds_max - 4096) / per;
+    int cap;
+    if (E.tune_stage_scale > 0.0) {
+        cap = ((int)(E.tune_stage_scale * mean + 64.0) + 63) / 64 * 64;
+    } else {
+        // occupancy-aware: room for a tile 5 standard deviations (Poisson) above
+        // the mean, then the largest stage that keeps that many workgroups per CU
+        const int want = ((int)(mean + 5.0 * std::sqrt(mean) + 64.0) + 63) / 64 * 64;
+        const int wg = std::max(1, E.lds_cu / (std::min(want, max_entries) * per + 2048));
+        cap = std::max(want, ((E.lds_cu / wg - 2048) / per) / 64 * 64);
+    }
     if (E.stage_override > 0) cap = E.stage_override;
     E.stage_cap = std::max(64, std::min(cap, max_entries));
     // tiles resident at once (LDS-limited); split a tile over several
@@ -891,7 +900,7 @@ static TilePolicy autotune_search(const TilePolicy& start, Measure measure, doub
     TilePolicy best = start;
     double bc = measure(best);
     for (int split : {1, 2, 3, 4})
-        for (double sc : {1.1, 1.25, 1.5}) {
+        for (double sc : {0.0, 1.1, 1.25, 1.5}) {
             const TilePolicy p{split, sc, start.rho_apb};
             const double c = measure(p);
             if (c >= 0 && c < bc) {

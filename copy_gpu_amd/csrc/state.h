@@ -59,7 +59,9 @@ struct Engine {
     int stage_override = 0;
     // tile shapes chosen by the autotuner (0 / default: heuristics)
     int tune_split = 0;              // workgroups per cell tile
-    double tune_stage_scale = 1.25;  // LDS stage = scale * mean stencil population + 64
+    // LDS stage = scale * mean stencil population + 64; 0: the largest stage that
+    // keeps the most workgroups per CU for a tile 5 sigma above the mean
+    double tune_stage_scale = 0.0;
     int rho_apb = 0;                 // atoms per rho(k) workgroup (multiple of 64)
     double tuned_ms = 0;             // cost of the chosen shapes at the last autotune
     int lds_max = 0;    // device LDS bytes per workgroup
