@@ -30,9 +30,15 @@ constexpr int kSamplerLds = 2048;  // CDF entries staged in LDS by the fused sam
 constexpr int kSortWaveBuf = 1024;  // bucket ids ranked in LDS per cell-sort wave
 
 // ---------------------------------------------------------------- k-vector draw
-// Workgroups [0, ceil(P/64)) of a kernel draw k-vectors l = 64 blk + lane with
-// their first wave; the CDF tables are staged in LDS (cdf, kSamplerLds entries)
-// by the whole workgroup when they fit.  Call from every thread of the group.
+// The first ceil(P / kThreads) workgroups of the kick/drift kernels only draw
+// k-vectors (l = blk * blockDim.x + thread), so the draws run beside the atom
+// work instead of ahead of it; the CDF tables are staged in LDS (cdf,
+// kSamplerLds entries) when they fit.  Call from every thread of the group.
+__device__ inline int sampler_blocks(const SampleArgs& sa, int threads)
+{
+    return (sa.P + threads - 1) / threads;
+}
+
 __device__ inline void sample_block(const SampleArgs& sa, int blk, double* cdf)
 {
     const int len = 2 * (sa.M.x + sa.M.y + sa.M.z) + 3;
@@ -40,8 +46,8 @@ __device__ inline void sample_block(const SampleArgs& sa, int blk, double* cdf)
     if (in_lds)
         for (int k = threadIdx.x; k < len; k += blockDim.x) cdf[k] = sa.cdf[k];
     __syncthreads();
-    const int l = blk * 64 + (int)threadIdx.x;
-    if (threadIdx.x < 64 && l < sa.P) {
+    const int l = blk * (int)blockDim.x + (int)threadIdx.x;
+    if (l < sa.P) {
         const uint64_t step = sa.step_dev ? (uint64_t)(*sa.step_dev) : sa.step;
         const int Ms[3] = {sa.M.x, sa.M.y, sa.M.z};
         int m[3];
@@ -142,19 +148,21 @@ __device__ inline void kick_drift_atom(const KickDriftArgs<T>& a, int i, const K
     a.xs[in.slot] = make_double4(in.xr[0] + dx, in.xr[1] + dy, in.xr[2] + dz, in.q);
 }
 
-// Opening kernel of a segment: kick/drift of every atom (one per thread); the
-// first ceil(P/64) workgroups also draw the step's k-vectors.
+// Opening kernel of a segment: the step's k-vector draw (first workgroups), then
+// the kick/drift of every atom (one per thread).
 template <typename T>
 struct KickDrift {
     static constexpr int kThreads = kMdBlock;
     using Args = KickDriftArgs<T>;
     __device__ static void run(const Args& a, int blk, int)
     {
-        if (blk < (a.sa.P + 63) / 64) {
+        const int ns = sampler_blocks(a.sa, kMdBlock);
+        if (blk < ns) {
             __shared__ double cdf[kSamplerLds];
             sample_block(a.sa, blk, cdf);
+            return;
         }
-        const int i = blk * kMdBlock + threadIdx.x;
+        const int i = (blk - ns) * kMdBlock + threadIdx.x;
         if (i >= a.n) return;
         kick_drift_atom(a, i, kd_load(a, i), a.vx[i], a.vy[i], a.vz[i], a.fx[i], a.fy[i],
                         a.fz[i]);
@@ -184,13 +192,14 @@ struct KForceDrift {
             double cdf[kSamplerLds];
         } lds;
         if (blk == 0 && threadIdx.x == 0) A.kf.flags[3] += 1u;  // MD step epoch
-        if (blk < (A.kd.sa.P + 63) / 64) {
+        const int ns = sampler_blocks(A.kd.sa, kKfThreads);
+        if (blk < ns) {
             sample_block(A.kd.sa, blk, lds.cdf);
-            __syncthreads();  // the LDS is reused for the coefficients
+            return;
         }
         const KForceArgs<T>& kf = A.kf;
         kforce_block(
-            kf, blk, lds.kf,
+            kf, blk - ns, lds.kf,
             [&](int i) {
                 In in;
                 in.v[0] = kf.vx[i];

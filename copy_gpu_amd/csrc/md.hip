@@ -61,16 +61,15 @@ typename KForceDrift<T>::Args close_open_args(Engine& e, const RepState& st, dou
                                          kickdrift_args<T>(e, st, dt, sa_next)};
 }
 
+// sampler workgroups first, then the atom workgroups
 int kickdrift_blocks(const Engine& e, const SampleArgs& sa)
 {
-    const int b = std::max(ceil_div(e.n, kMdBlock), ceil_div(sa.P, 64));
-    return b > 0 ? b : 1;
+    return ceil_div(sa.P, kMdBlock) + std::max(ceil_div(e.n, kMdBlock), 1);
 }
 
 int close_open_blocks(const Engine& e, const SampleArgs& sa)
 {
-    const int b = std::max(kforce_blocks(e), ceil_div(sa.P, 64));
-    return b > 0 ? b : 1;
+    return ceil_div(sa.P, kKfThreads) + std::max(kforce_blocks(e), 1);
 }
 
 template <typename T>
