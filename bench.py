@@ -155,7 +155,7 @@ def kernel_times(f, st, wl, Kt, step):
     st.run(f, wl, Kt, step, use_graph=False)
     torch.cuda.synchronize()
     names = ["check", "bin", "scan", "scatter", "cellsort", "gather", "sample", "rho",
-             "rho_final", "pair", "kforce", "kick"]
+             "rho_final", "pair", "kforce"]
     per = {}
     for nm in names:
         ms, cnt = f.timing(nm)

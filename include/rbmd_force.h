@@ -124,8 +124,9 @@ int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* 
  * x,y,z,vx,vy,vz FP64 (n); f* as in compute and holding the forces of the
  * current positions on entry; img int32 (3n, xyz interleaved) or NULL;
  * mass FP64 (n) or NULL (=1).  With RBE, k-vectors are resampled every step
- * from (seed, step0 + s).  use_graph != 0 captures one step in a HIP graph
- * and replays it.  Positions/velocities/forces are updated in place. */
+ * from (seed, step0 + s).  use_graph != 0 captures eight steps (and one step,
+ * for a remainder) in HIP graphs and replays them; graph and eager runs give
+ * the same bits.  Positions/velocities/forces are updated in place. */
 int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, double* vy,
                 double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
                 double dt, int64_t nsteps, uint64_t seed, uint64_t step0, int32_t use_graph,
@@ -167,9 +168,12 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
 /* Per-kernel device time (HIP events on the launch stream).  enable != 0
  * starts recording around every launch of the named passes; get returns the
  * mean milliseconds per launch and the launch count for one pass name:
- * "check" (also the MD kick/drift kernel), "bin", "scan", "scatter", "cellsort",
- * "gather", "sample", "rho", "rho_final", "pair" (the tile pass: Verlet rows on rebuild
- * steps, then the pair forces), "kforce", "kick", "ev_final". */
+ * "check", "bin", "scan", "scatter", "cellsort", "gather", "sample", "rho",
+ * "rho_final", "pair" (the tile pass: Verlet rows on rebuild steps, then the pair
+ * forces), "kforce", "ev_final".  In MD runs: "check" is the kick/drift kernel
+ * that opens a run segment, "rho" the rho(k) partials + cell-sort kernel, "pair"
+ * the rho(k) final reduction + tile pass, "kforce" the k-space force with the
+ * closing kick (and the next step's kick/drift). */
 int rbmd_force_set_timing(rbmd_force_t h, int32_t enable);
 int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int64_t* launches);
 
@@ -179,11 +183,12 @@ int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream);
 
 /* ---- batched replicas ----
  *
- * Many independent systems of uneven size advanced together: each phase of the
- * velocity-Verlet step (kick/drift/sample, list build, rho(k), pair pass,
- * k-space force + kick) is ONE launch over all replicas, driven by a table of
- * (replica, workgroup) work items with per-replica arguments in device memory,
- * and the whole step is captured once in a HIP graph.  Each replica is an
+ * Many independent systems of uneven size advanced together: each kernel of
+ * the velocity-Verlet step (rho(k) partials + cell sort, rho(k) final + tile
+ * pass, k-space force + kicks + drift + k-draw) is ONE launch over all
+ * replicas, driven by a table of (replica, workgroup) work items with
+ * per-replica arguments in device memory, and eight steps are captured once in
+ * a HIP graph.  Each replica is an
  * ordinary force handle (rbmd_batch_replica) with the ensemble loop of
  * ExecutionNVT::Solve (rbmd/framework/run/ExecutionNVT.cpp:133-159) per replica;
  * a replica advanced in a batch takes bit-identical steps to the same handle
