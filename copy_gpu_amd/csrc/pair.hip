@@ -415,16 +415,17 @@ struct TilePass {
         }
         if (rebuild) {
             if (!staged && part == 0 && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
+            const RefStage rs(lds, A.stage_cap);
             if (staged) {
                 for (int u = threadIdx.x; u < total; u += kTileThreads)
-                    lds[u] = stage_ref(h, A.xs_ref, u);
+                    rs.put(u, stage_ref(h, A.xs_ref, u));
             }
             __syncthreads();
             if (staged)
-                list_wave<true>(h, lds, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                list_wave<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
                                 A.nlist, A.ncount, A.stats);
             else
-                list_wave<false>(h, lds, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                list_wave<false>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
                                  A.nlist, A.ncount, A.stats);
             __syncthreads();  // rows and LDS are reused below
         }

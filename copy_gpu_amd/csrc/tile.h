@@ -157,9 +157,11 @@ __device__ inline int tile_seg(const TileHdr& h, int u)
 // d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the reference's operation order
 // (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  Hits are
 // compacted with ballot + mbcnt into the row as uint16 tile-local indices:
-// LJ-range hits (d2 < r_in^2) from the row start, the others from the row end.
-// Tiles larger than the LDS stage read candidates from global memory; tiles
-// beyond the uint16 range mark their rows overflowed.
+// LJ-range hits (d2 < r_in^2) from the row start, the others from the row end;
+// each candidate costs one masked store and no branches.  The stage is two
+// arrays, (x, y) pairs then z, so the sweep's consecutive reads are
+// bank-conflict-free.  Tiles larger than the LDS stage read candidates from
+// global memory; tiles beyond the uint16 range mark their rows overflowed.
 __device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict__ xs_ref, int u)
 {
 #pragma clang fp contract(off)
@@ -168,8 +170,29 @@ __device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict_
     return make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
 }
 
+// Reference-position stage of list_wave: (x, y) of candidate u at xy[u], z at
+// zz[u]; xy and zz partition an LDS region of stage_cap double4 entries.
+struct RefStage {
+    double2* xy;
+    double* zz;
+    __device__ RefStage(double4* lds, int stage_cap)
+        : xy((double2*)lds), zz((double*)lds + 2 * (size_t)stage_cap)
+    {
+    }
+    __device__ void put(int u, const double4& p) const
+    {
+        xy[u] = make_double2(p.x, p.y);
+        zz[u] = p.z;
+    }
+    __device__ double3 get(int u) const
+    {
+        const double2 a = xy[u];
+        return make_double3(a.x, a.y, zz[u]);
+    }
+};
+
 template <bool STAGED>
-__device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ lds,
+__device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
                                  const double4* __restrict__ xs_ref, int first, int a1, int h0,
                                  int lane, double rl2, double rin2, int cap,
                                  uint16_t* __restrict__ nlist, int2* __restrict__ ncount,
@@ -187,16 +210,27 @@ __device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ l
             continue;
         }
         const int ui = h.off[kHomeSeg] + (s - h0);
-        const double4 pi = STAGED ? lds[ui] : xs_ref[s];
+        double3 pi;
+        if (STAGED) {
+            pi = rs.get(ui);
+        } else {
+            const double4 p = xs_ref[s];
+            pi = make_double3(p.x, p.y, p.z);
+        }
         int n_in = 0, n_out = 0;
         uint16_t* __restrict__ row = nlist + (size_t)s * cap;
         for (int ub = 0; ub < total; ub += 128) {
-            double4 pj[2];
+            double3 pj[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const int u = ub + 64 * k + lane;
                 const int uu = u < total ? u : 0;
-                pj[k] = STAGED ? lds[uu] : stage_ref(h, xs_ref, uu);
+                if (STAGED) {
+                    pj[k] = rs.get(uu);
+                } else {
+                    const double4 p = stage_ref(h, xs_ref, uu);
+                    pj[k] = make_double3(p.x, p.y, p.z);
+                }
             }
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -207,17 +241,12 @@ __device__ inline void list_wave(const TileHdr& h, const double4* __restrict__ l
                 const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
                 const bool hit = (u < total) && ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
                 const bool inner = d2 < rin2;
-                const uint64_t mi = __ballot(hit && inner);
-                const uint64_t mo = __ballot(hit && !inner);
-                if (hit) {
-                    if (inner) {
-                        const int p = n_in + (int)mbcnt64(mi);
-                        if (p < cap) row[p] = (uint16_t)u;
-                    } else {
-                        const int p = n_out + (int)mbcnt64(mo);
-                        if (p < cap) row[cap - 1 - p] = (uint16_t)u;
-                    }
-                }
+                const uint64_t mh = __ballot(hit), mn = __ballot(inner);
+                const uint64_t mi = mh & mn, mo = mh & ~mn;
+                const int pin = n_in + (int)mbcnt64(mi);
+                const int pout = n_out + (int)mbcnt64(mo);
+                const int slot = inner ? pin : cap - 1 - pout;
+                if (hit && (inner ? pin : pout) < cap) row[slot] = (uint16_t)u;
                 n_in += __popcll(mi);
                 n_out += __popcll(mo);
             }
