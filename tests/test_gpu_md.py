@@ -1,0 +1,183 @@
+"""GPU tests of the fused MD step (rho(k) partials + cell sort, rho(k) final + tile
+pass, k-space force + kicks + drift) against the oracle's velocity-Verlet loop.
+
+The oracle loop is kick / drift / forces / kick with FP64 pair forces and the same
+k-space term (oracle.pair_forces, oracle.kspace_forces, oracle.kick, oracle.drift).
+These runs use a small skin so the list is rebuilt many times, which exercises the
+tile pass's binning for the next rebuild and the one-kernel cell sort; they cover
+two LJ types, per-atom masses, image flags, full Ewald k-space and the
+global-memory tile path.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def _make(Lb, n, precision, kspace, P=100, skin=0.3, eps=(1.0,), sigma=(1.0,), tile_stage=0,
+          ewald_tol=1e-12, alpha=0.5):
+    from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
+
+    return LJCutCoulKspaceForce(ForceConfig(
+        box_len=tuple(float(v) for v in Lb), cut_off=2.5, coulomb_cut=5.0, skin=skin,
+        alpha=alpha, kspace=kspace, coulomb_sample_num=P, precision=precision, eps=eps,
+        sigma=sigma, n_max=n, ewald_tol=ewald_tol, tile_stage=tile_stage))
+
+
+def _halfspace_lattice(Lb, alpha, tol):
+    # the handle's Ewald k-list: half space m_x > 0, or m_x = 0 and (m_y, m_z) > 0,
+    # k^2 <= -4 alpha ln(tol), weight 2 exp(-k^2 / 4 alpha)
+    kmax2 = -4 * alpha * np.log(tol)
+    M = [int(np.floor(np.sqrt(kmax2) * Lb[d] / (2 * np.pi))) + 1 for d in range(3)]
+    a, b, g = np.meshgrid(np.arange(0, M[0] + 1), np.arange(-M[1], M[1] + 1),
+                          np.arange(-M[2], M[2] + 1), indexing="ij")
+    a, b, g = a.ravel(), b.ravel(), g.ravel()
+    keep = ~((a == 0) & ((b < 0) | ((b == 0) & (g <= 0))))
+    a, b, g = a[keep], b[keep], g[keep]
+    k2 = ((2 * np.pi * a / Lb[0]) ** 2 + (2 * np.pi * b / Lb[1]) ** 2
+          + (2 * np.pi * g / Lb[2]) ** 2)
+    keep = k2 <= kmax2
+    return (np.stack([a[keep], b[keep], g[keep]], 1).astype(np.int32),
+            2 * np.exp(-k2[keep] / (4 * alpha)))
+
+
+def _oracle_run(orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt, kforces, types=None,
+                eps=(1.0,), sigma=(1.0,), mass=None):
+    lo = np.zeros(3)
+
+    def forces(p, s):
+        Fp, _ = orc.pair_forces(*p, q, lo, Lb, 2.5, 5.0, alpha, types=types, eps=eps,
+                                sigma=sigma)
+        return Fp + kforces(p, s)
+
+    p = list(orc.wrap(x, y, z, lo, Lb))
+    img = np.zeros(3 * x.size, np.int32)
+    v = [vx, vy, vz]
+    Fo = forces(p, 0)
+    for s in range(1, steps + 1):
+        v = orc.kick(v, [Fo[:, 0], Fo[:, 1], Fo[:, 2]], dt, mass)
+        p, img = orc.drift(p, v, dt, lo, Lb, img)
+        Fo = forces(p, s)
+        v = orc.kick(v, [Fo[:, 0], Fo[:, 1], Fo[:, 2]], dt, mass)
+    return np.stack(p, 1), np.stack(v, 1), img, Fo
+
+
+def _gpu_run(f, x, y, z, q, vx, vy, vz, steps, dt, types=None, mass=None, P=None,
+             use_graph=True):
+    n = x.size
+    f.Init(_dev(q), None if types is None else _dev(types, torch.int32))
+    if P is not None:
+        f.sample_kvectors(P, SEED, 0)
+    X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+    F = [torch.zeros(n, device="cuda", dtype=f.fdtype) for _ in range(3)]
+    f.Execute(X, Y, Z, *F)
+    img = torch.zeros(3 * n, device="cuda", dtype=torch.int32)
+    M = None if mass is None else _dev(mass)
+    b0 = f.stats()["builds"]
+    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=steps, seed=SEED, step0=1, img=img, mass=M,
+             use_graph=use_graph)
+    torch.cuda.synchronize()
+    builds = f.stats()["builds"] - b0
+    P_ = np.stack([t.cpu().numpy() for t in (X, Y, Z)], 1)
+    V_ = np.stack([t.cpu().numpy() for t in (VX, VY, VZ)], 1)
+    F_ = np.stack([t.double().cpu().numpy() for t in F], 1)
+    return P_, V_, img.cpu().numpy(), F_, builds
+
+
+def _dpos(a, b, Lb):
+    d = a - b
+    return np.abs(d - Lb * np.rint(d / Lb)).max()
+
+
+def test_md_rebuilds_two_types_masses_ewald_fp64(orc, gpu):
+    """FP64, full Ewald k-space, two LJ types, per-atom masses, skin 0.1 (a list
+    rebuild every few steps): 40 graph-replayed steps against the oracle loop."""
+    from copy_gpu_amd import systems
+
+    _log("case: FP64 Ewald MD, two types, masses, skin 0.1, 40 steps, 2000 ions")
+    n, dt, steps, alpha, tol = 2000, 0.002, 40, 0.5, 1e-12
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=51)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=51)
+    Lb = np.array([L] * 3)
+    types = (np.arange(n) % 3 == 0).astype(np.int32)
+    eps, sig = (1.0, 0.6), (1.0, 1.3)
+    mass = 1.0 + 0.5 * (np.arange(n) % 2)
+    f = _make(Lb, n, 64, "EWALD", skin=0.1, eps=eps, sigma=sig, ewald_tol=tol, alpha=alpha)
+    Pg, Vg, Ig, Fg, builds = _gpu_run(f, x, y, z, q, vx, vy, vz, steps, dt, types=types,
+                                      mass=mass)
+    _log(f"  list builds during the run: {builds}")
+    assert builds >= 5
+    m, w = _halfspace_lattice(Lb, alpha, tol)
+    assert f.stats()["ewald_k"] == m.shape[0]
+    Po, Vo, Io, Fo = _oracle_run(
+        orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt,
+        lambda p, s: orc.kspace_forces(*p, q, Lb, alpha, m, w)[0], types=types, eps=eps,
+        sigma=sig, mass=mass)
+    assert _dpos(Pg, Po, Lb) < 1e-9
+    assert np.abs(Vg - Vo).max() < 1e-8
+    assert np.array_equal(Ig, Io)
+    assert np.linalg.norm(Fg - Fo) / np.linalg.norm(Fo) < 1e-9
+
+
+def test_md_rbe_fp32_rebuilds(orc, gpu):
+    """FP32 forces with device-sampled RBE k-lists (one per step), skin 0.1: 30
+    steps against the oracle loop fed the host draws of the same (seed, step)."""
+    from copy_gpu_amd import systems
+
+    _log("case: FP32 RBE MD, skin 0.1, 30 steps, 3000 ions")
+    n, dt, steps, alpha, P = 3000, 0.002, 30, 0.5, 100
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=52)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=52)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, n, 32, "RBE", P=P, skin=0.1)
+    Pg, Vg, Ig, Fg, builds = _gpu_run(f, x, y, z, q, vx, vy, vz, steps, dt, P=P)
+    _log(f"  list builds during the run: {builds}")
+    assert builds >= 4
+
+    def kf(p, s):
+        mk, S = systems.sample_kvectors_host(alpha, Lb, P, SEED, s)
+        return orc.kspace_forces(*p, q, Lb, alpha, mk, np.full(P, S / P))[0]
+
+    Po, Vo, Io, Fo = _oracle_run(orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt, kf)
+    # FP32 forces (1e-5 relative) integrated over 30 steps of dt = 0.002
+    assert _dpos(Pg, Po, Lb) < 1e-6
+    assert np.abs(Vg - Vo).max() < 1e-3
+    assert np.array_equal(Ig, Io)
+    assert np.linalg.norm(Fg - Fo) / np.linalg.norm(Fo) < 1e-4
+
+
+@pytest.mark.parametrize("stage", [64])
+def test_md_global_memory_tiles_equal_staged(gpu, stage):
+    """Every tile read from global memory (LDS stage 64 atoms) gives the same bits
+    over an MD run, rebuilds included, as the LDS-staged tiles."""
+    from copy_gpu_amd import systems
+
+    _log(f"case: MD with LDS stage {stage} vs default, FP32 RBE, skin 0.1, 20 steps")
+    n, dt, steps, P = 4000, 0.002, 20, 100
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=53)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=53)
+    Lb = np.array([L] * 3)
+    outs = []
+    for ts in (0, stage):
+        f = _make(Lb, n, 32, "RBE", P=P, skin=0.1, tile_stage=ts)
+        res = _gpu_run(f, x, y, z, q, vx, vy, vz, steps, dt, P=P)
+        if ts:
+            assert f.stats()["big_tiles"] > 0
+        outs.append(res[:4])
+        f.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
