@@ -343,10 +343,14 @@ __device__ inline void block_ev(const double* acc, int lane, int wv, double* __r
 // builds the Verlet rows of its home atoms (list_wave over the tile's reference
 // positions), then the pair pass: the tile's current positions are staged in
 // LDS shifted to the home cell's image (and types when there are several);
-// each 32-lane half-wave walks one home atom's uint16 row -- LJ-range entries
-// first, so later rounds skip the LJ code -- then reduces across its 32 lanes.
-// Tiles larger than the LDS stage take the same steps from global memory.  The rows of a home atom are written and read by the same workgroup,
-// so the list build and the pair pass need no launch in between.
+// each group of kGroup lanes walks one home atom's uint16 row -- LJ-range
+// entries first, so later rounds skip the LJ code -- then reduces across its
+// lanes.  Tiles larger than the LDS stage take the same steps from global
+// memory.  The rows of a home atom are written and read by the same workgroup,
+// so the list build and the pair pass need no launch in between.  In MD steps
+// the workgroup finally bins its home atoms for the next rebuild; that comes
+// last so its atomics' latency overlaps the other workgroups' pair work rather
+// than delaying this one's.
 template <typename T, bool EV, bool MULTI>
 struct TilePass {
     static constexpr int kThreads = kTileThreads;
@@ -393,6 +397,56 @@ struct TilePass {
         const int wv = uniform(threadIdx.x >> 6);
         const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
         const int a0 = h0 + (hc * part) / A.split, a1 = h0 + (hc * (part + 1)) / A.split;
+        if (rebuild) {
+            if (!staged && part == 0 && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
+            const RefStage rs(lds, A.stage_cap);
+            if (staged) {
+                for (int u = threadIdx.x; u < total; u += kTileThreads)
+                    rs.put(u, stage_ref(h, A.xs_ref, u));
+            }
+            __syncthreads();
+            if (staged)
+                list_wave<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                                A.nlist, A.ncount, A.stats);
+            else
+                list_wave<false>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                                 A.nlist, A.ncount, A.stats);
+            __syncthreads();  // rows and LDS are reused below
+        }
+        if (staged) {
+            // two entries per trip, both loads issued before either LDS write
+            for (int u0 = threadIdx.x; u0 < total; u0 += 2 * kTileThreads) {
+                int st[2], j[2];
+                double4 p[2];
+                int t[2] = {0, 0};
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int u = u0 + k * kTileThreads;
+                    st[k] = tile_seg(h, u < total ? u : 0);
+                    j[k] = h.gstart[st[k]] + ((u < total ? u : 0) - h.off[st[k]]);
+                    p[k] = a.xs[j[k]];
+                    if (MULTI) t[k] = a.ts[j[k]];
+                }
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int u = u0 + k * kTileThreads;
+                    if (u < total) {
+                        lds[u] = make_double4(p[k].x - h.sx[st[k]], p[k].y - h.sy[st[k]],
+                                              p[k].z - h.sz[st[k]], p[k].w);
+                        if (MULTI) ltype[u] = t[k];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (staged)
+            pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
+                                          A.fx, A.fy, A.fz, acc);
+        else
+            pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
+                                           A.pp, A.fx, A.fy, A.fz, acc);
+        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
         if (A.md) {
             // bins of the next rebuild: clear this step's buffer (consumed by the
             // cell sort) for this cell, bin the current home atoms into the other
@@ -413,40 +467,6 @@ struct TilePass {
                 md_bin_wave(A.bins, 1 - b, act, v, p, A.box, A.dims, A.cell);
             }
         }
-        if (rebuild) {
-            if (!staged && part == 0 && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
-            const RefStage rs(lds, A.stage_cap);
-            if (staged) {
-                for (int u = threadIdx.x; u < total; u += kTileThreads)
-                    rs.put(u, stage_ref(h, A.xs_ref, u));
-            }
-            __syncthreads();
-            if (staged)
-                list_wave<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
-                                A.nlist, A.ncount, A.stats);
-            else
-                list_wave<false>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
-                                 A.nlist, A.ncount, A.stats);
-            __syncthreads();  // rows and LDS are reused below
-        }
-        if (staged) {
-            for (int u = threadIdx.x; u < total; u += kTileThreads) {
-                const int st = tile_seg(h, u);
-                const int j = h.gstart[st] + (u - h.off[st]);
-                const double4 p = a.xs[j];
-                lds[u] = make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
-                if (MULTI) ltype[u] = a.ts[j];
-            }
-        }
-        __syncthreads();
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (staged)
-            pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
-                                          A.fx, A.fy, A.fz, acc);
-        else
-            pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
-                                           A.pp, A.fx, A.fy, A.fz, acc);
-        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
     }
 };
 
