@@ -104,6 +104,7 @@ static void free_all(Engine& E)
 {
     void* ptrs[] = {E.xr,       E.yr,    E.zr,     E.xs_ref, E.xs,        E.perm,     E.scell,
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
+                    E.tile_order,
                     E.md_count, E.md_bucket, E.md_cell, E.md_w,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
@@ -179,6 +180,16 @@ static void ewald_lattice(const Engine& E, std::vector<int32_t>& m, std::vector<
 // Cell-tile geometry for n atoms: workgroups per tile (enough tiles to load
 // every CU several times) and the LDS stage, 1.25x the mean stencil population
 // (the rare larger tile reads its candidates from global memory).
+// Identity tile order (workgroup t computes tile t); the next list build
+// replaces it with the load-balanced order.  Synchronous, outside capture.
+static hipError_t reset_tile_order(Engine& E)
+{
+    if (!E.tile_order) return hipSuccess;
+    std::vector<int32_t> id((size_t)E.ncell * 4);
+    for (size_t k = 0; k < id.size(); ++k) id[k] = (int32_t)k;
+    return hipMemcpy(E.tile_order, id.data(), id.size() * 4, hipMemcpyHostToDevice);
+}
+
 static void tile_geometry(Engine& E, int64_t n)
 {
     const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
@@ -335,6 +346,8 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.atom_cell, N * 4));
     CR(dalloc(&E.atom_rank, N * 4));
     CR(dalloc(&E.cell_count, (size_t)E.ncell * 4));
+    CR(dalloc(&E.tile_order, (size_t)E.ncell * 4 * 4));
+    CR(reset_tile_order(E));
     {
         // per-cell buckets of the MD step's binning: 4x the mean population
         // (a cell beyond it is gathered by a scan instead)
@@ -411,7 +424,9 @@ int rbmd_force_set_atoms(rbmd_force_t h, int64_t n, const double* q, const int32
     if (n > 0 && !q) return fail(E, RBMD_ERR_INVALID, "charges required");
     hipStream_t s = (hipStream_t)stream;
     E.n = n;
+    const int split = E.split;
     tile_geometry(E, n);
+    if (E.split != split) HIP_TRY(reset_tile_order(E));
     if (n > 0) {
         HIP_TRY(hipMemcpyAsync(E.q, q, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
         if (type) {
@@ -882,10 +897,12 @@ struct TilePolicy {
 
 static void apply_policy(Engine& E, const TilePolicy& p)
 {
+    const int split = E.split;
     E.tune_split = p.split;
     E.tune_stage_scale = p.stage_scale;
     E.rho_apb = p.rho_apb;
     tile_geometry(E, E.n);
+    if (E.split != split) (void)reset_tile_order(E);
 }
 
 // Expected share of steps that rebuild the list (weights the build's cost).

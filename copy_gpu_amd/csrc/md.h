@@ -22,6 +22,7 @@
 #pragma once
 #include "kspace.h"
 #include "state.h"
+#include "tile.h"
 
 namespace rbmd {
 
@@ -324,10 +325,16 @@ struct SortRho {
     static constexpr size_t kSortBytes = kWaves * kSortWaveBuf * sizeof(int32_t);
     static constexpr size_t kLdsBytes =
         RhoPartial<T>::kLdsBytes > kSortBytes ? RhoPartial<T>::kLdsBytes : kSortBytes;
+    static_assert(kLdsBytes >= kOrderLds * sizeof(int32_t),
+                  "the tile-order workgroup shares the stage");
     struct Args {
         typename RhoPartial<T>::Args rho;
         int nrho;
         CellSortMDArgs cs;
+        int nsort;  // cell-sort workgroups; the one after them orders the tiles
+        int3 dims;
+        int split, ncu;  // ncu 0: keep the cell order
+        int32_t* tile_order;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
@@ -336,9 +343,19 @@ struct SortRho {
             RhoPartial<T>::run_lds(A.rho, blk, lds);
             return;
         }
+        blk -= A.nrho;
+        if (blk == A.nsort) {
+            // rebuild steps: order the tile workgroups by the new cells' work
+            if (A.ncu > 0 && A.cs.flags[0] != 0u) {
+                const int32_t* count =
+                    A.cs.bins.count + (size_t)(A.cs.flags[3] & 1u) * A.cs.ncell;
+                tile_order_build(count, A.cs.ncell, A.dims, A.split, A.ncu, A.tile_order,
+                                 (int32_t*)lds);
+            }
+            return;
+        }
         const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-        cellsort_md_cell(A.cs, (blk - A.nrho) * kWaves + wv, lane,
-                         (int32_t*)lds + wv * kSortWaveBuf);
+        cellsort_md_cell(A.cs, blk * kWaves + wv, lane, (int32_t*)lds + wv * kSortWaveBuf);
     }
 };
 

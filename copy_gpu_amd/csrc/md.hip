@@ -42,7 +42,12 @@ typename SortRho<T>::Args sortrho_args(Engine& e, const RepState& st, int* block
     else memset(&a.rho, 0, sizeof a.rho);
     a.nrho = nr;
     a.cs = cellsort_md_args(e, st);
-    *blocks = nr + sortrho_cell_blocks(e.ncell);
+    a.nsort = sortrho_cell_blocks(e.ncell);
+    a.dims = make_int3(e.dims[0], e.dims[1], e.dims[2]);
+    a.split = e.split;
+    a.ncu = tile_order_ncu(e.ncell, e.split, e.num_cu);
+    a.tile_order = e.tile_order;
+    *blocks = nr + a.nsort + 1;
     return a;
 }
 

@@ -90,6 +90,9 @@ struct Scan {
         int ncell;
         uint32_t* flags;
         unsigned long long* stats;
+        int3 dims;
+        int split, ncu;  // ncu 0: keep the cell order
+        int32_t* tile_order;
     };
     __device__ static void run(const Args& a, int, int)
     {
@@ -136,6 +139,10 @@ struct Scan {
             __syncthreads();
         }
         if (t == 0) a.start[a.ncell] = carry;
+        if (a.ncu > 0) {
+            __shared__ int32_t olds[kOrderLds];
+            tile_order_build(a.count, a.ncell, a.dims, a.split, a.ncu, a.tile_order, olds);
+        }
     }
 };
 
@@ -317,7 +324,18 @@ Bin::Args bin_args(Engine& e, const double* x, const double* y, const double* z)
                      e.atom_rank, e.flags};
 }
 
-Scan::Args scan_args(Engine& e) { return Scan::Args{e.cell_count, e.cell_start, e.ncell, e.flags, e.stats}; }
+Scan::Args scan_args(Engine& e)
+{
+    return Scan::Args{e.cell_count,
+                      e.cell_start,
+                      e.ncell,
+                      e.flags,
+                      e.stats,
+                      make_int3(e.dims[0], e.dims[1], e.dims[2]),
+                      e.split,
+                      tile_order_ncu(e.ncell, e.split, e.num_cu),
+                      e.tile_order};
+}
 
 Scatter::Args scatter_args(Engine& e)
 {

@@ -375,6 +375,7 @@ struct TilePass {
         int md;        // MD step: bin the home atoms for the next rebuild
         double3 cell;  // cell edge lengths (binning)
         MdBins bins;
+        const int32_t* order;  // workgroup -> cell * split + part (null: identity)
     };
     __device__ static void run(const Args& A, int blk, int)
     {
@@ -382,7 +383,8 @@ struct TilePass {
         extern __shared__ double4 lds[];
         int32_t* ltype = (int32_t*)(lds + A.stage_cap);
         __shared__ TileHdr h;
-        const int c = blk / A.split, part = blk % A.split;
+        const int tb = A.order ? A.order[blk] : blk;
+        const int c = tb / A.split, part = tb % A.split;
         const bool rebuild = A.flags[2] != 0u;
         if (threadIdx.x == 0) {
             // the build kernels of this step have read the request and the cell
@@ -446,7 +448,7 @@ struct TilePass {
         else
             pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
                                            A.pp, A.fx, A.fy, A.fz, acc);
-        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)blk * 8);
+        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)tb * 8);
         if (A.md) {
             // bins of the next rebuild: clear this step's buffer (consumed by the
             // cell sort) for this cell, bin the current home atoms into the other
@@ -470,25 +472,30 @@ struct TilePass {
     }
 };
 
-// MD step: the rho(k) final reduction (workgroups [0, nfin), one per k) and the
-// tile pass (the rest) in one grid.  Both need only the previous kernel's
-// output (rho partials, cell-sorted positions), and the k-space force that
-// follows needs both.
+// MD step: the tile pass and the rho(k) final reduction (nfin workgroups, one
+// per k) in one grid.  Both need only the previous kernel's output
+// (cell-sorted positions, rho partials), and the k-space force that follows
+// needs both.  With the load-balanced tile order the tiles come first, so that
+// workgroup t runs on CU t mod ncu as that order assumes, and the short
+// reductions fill the slots that CUs with fewer tiles leave free; with many
+// dispatch rounds the reductions come first and finish early.
 template <typename T, bool EV, bool M>
 struct TileRho {
     static constexpr int kThreads = kTileThreads;
     struct Args {
         typename RhoFinalT<T, kTileThreads>::Args fin;
-        int nfin;
+        int nfin, ntile;
+        int fin_first;
         typename TilePass<T, EV, M>::Args tile;
     };
-    __device__ static void run(const Args& A, int blk, int nblk)
+    __device__ static void run(const Args& A, int blk, int)
     {
-        if (blk < A.nfin) {
-            RhoFinalT<T, kTileThreads>::run(A.fin, blk, A.nfin);
+        const int t = A.fin_first ? blk - A.nfin : blk;
+        if (t < 0 || t >= A.ntile) {
+            RhoFinalT<T, kTileThreads>::run(A.fin, A.fin_first ? blk : blk - A.ntile, A.nfin);
             return;
         }
-        TilePass<T, EV, M>::run(A.tile, blk - A.nfin, nblk - A.nfin);
+        TilePass<T, EV, M>::run(A.tile, t, A.ntile);
     }
 };
 
@@ -554,7 +561,8 @@ typename TilePass<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void*
         e.xs_ref, e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box,
         e.stage_cap, e.r_list * e.r_list, e.r_in * e.r_in, pair_scalars(e), e.d_pp, (T*)fx,
         (T*)fy, (T*)fz, e.ev_part, e.flags, e.stats, e.nlist, e.ncount, e.cell_count,
-        md ? 1 : 0, make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e)};
+        md ? 1 : 0, make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e),
+        tile_order_ncu(e.ncell, e.split, e.num_cu) > 0 ? e.tile_order : nullptr};
 }
 
 size_t pair_lds(const Engine& e)
@@ -578,6 +586,8 @@ typename TileRho<T, false, M>::Args tile_rho_args(Engine& e, void* fx, void* fy,
     if (ks) a.fin = rho_final_args<T, kTileThreads>(e, step_dev);
     else memset(&a.fin, 0, sizeof a.fin);
     a.nfin = ks ? e.P : 0;
+    a.ntile = e.ncell * e.split;
+    a.fin_first = tile_order_ncu(e.ncell, e.split, e.num_cu) == 0;
     a.tile = tile_args<T, false, M>(e, fx, fy, fz, true);
     return a;
 }

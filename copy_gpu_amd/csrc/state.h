@@ -76,6 +76,9 @@ struct Engine {
     int32_t *perm = nullptr, *scell = nullptr, *ts = nullptr;
     int32_t *cell_count = nullptr, *cell_start = nullptr, *atom_cell = nullptr,
             *atom_rank = nullptr;
+    // tile workgroup -> (cell * split + part): load-balanced when the tiles fit about
+    // one dispatch round (rebuilt with the cells; identity until then)
+    int32_t* tile_order = nullptr;
     // bins of the MD step's next rebuild (double-buffered by the epoch flags[3])
     int32_t* md_count = nullptr;   // [2][ncell]
     int32_t* md_bucket = nullptr;  // [2][ncell][bucket_cap]

@@ -134,6 +134,96 @@ __device__ inline void md_bin_atom(const MdBins& b, int buf, int v, const double
     if (rank < b.cap) b.bucket[(size_t)slot * b.cap + rank] = v;
 }
 
+// ---------------------------------------------------------------- tile order
+// Order of the tile workgroups, for load balance only (a tile's results do not
+// depend on which workgroup computes it).  Used when the tiles fit about one
+// dispatch round (ncell * split <= 3 ncu): then the kernel ends when its most
+// loaded CU does, and the dispatcher gives workgroup t to CU t mod ncu.  The
+// work of cell c is estimated as w_c = n_c x (atoms of its 27-cell stencil);
+// cells are counting-sorted into kOrderBuckets classes of w, heaviest first
+// (order within a class is arbitrary), and the first round takes the heaviest
+// tiles in reverse order, so a CU that starts with a lighter tile receives
+// heavier ones in the later rounds.  With more tiles the dispatcher balances by
+// itself and the cell order (spatial, L2-friendly) is kept.  One workgroup of
+// at least 256 threads runs it, ncell <= kOrderMaxCells; lds holds kOrderLds ints.
+constexpr int kOrderBuckets = 256;
+constexpr int kOrderMaxCells = 1024;
+constexpr int kOrderLds = 2 * kOrderBuckets + 8 + 2 * kOrderMaxCells;
+
+__device__ inline void tile_order_build(const int32_t* __restrict__ count, int ncell, int3 dims,
+                                        int split, int ncu, int32_t* __restrict__ order,
+                                        int32_t* lds)
+{
+    const int nt = (int)blockDim.x, t = (int)threadIdx.x;
+    int32_t* hist = lds;
+    int32_t* cur = lds + kOrderBuckets;
+    int32_t* misc = lds + 2 * kOrderBuckets;  // [0] max weight, [1..4] wave sums
+    int32_t* cnt = misc + 8;
+    int32_t* w = cnt + kOrderMaxCells;
+    for (int k = t; k < kOrderBuckets; k += nt) hist[k] = 0;
+    for (int c = t; c < ncell; c += nt) cnt[c] = count[c];
+    if (t == 0) misc[0] = 0;
+    __syncthreads();
+    int mx = 0;
+    for (int c = t; c < ncell; c += nt) {
+        const int cz = c / (dims.x * dims.y), cy = (c / dims.x) % dims.y, cx = c % dims.x;
+        int sum = 0;
+        for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b)
+                for (int g = -1; g <= 1; ++g) {
+                    const int x = (cx + a + dims.x) % dims.x, y = (cy + b + dims.y) % dims.y,
+                              z = (cz + g + dims.z) % dims.z;
+                    sum += cnt[x + y * dims.x + z * dims.x * dims.y];
+                }
+        const int wc = cnt[c] * sum;
+        w[c] = wc;
+        mx = wc > mx ? wc : mx;
+    }
+    atomicMax(&misc[0], mx);
+    __syncthreads();
+    const long long scale = (long long)misc[0] + 1;
+    auto bucket = [&](int wc) {
+        return kOrderBuckets - 1 - (int)(((long long)wc * kOrderBuckets) / scale);
+    };
+    for (int c = t; c < ncell; c += nt) atomicAdd(&hist[bucket(w[c])], 1);
+    __syncthreads();
+    // exclusive prefix over the buckets: threads [0, 256) one bucket each
+    // (4 waves: wave scans, then the waves' totals)
+    int v = 0, incl = 0;
+    if (t < kOrderBuckets) {
+        v = hist[t];
+        incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if ((t & 63) >= o) incl += u;
+        }
+        if ((t & 63) == 63) misc[1 + (t >> 6)] = incl;
+    }
+    __syncthreads();
+    if (t < kOrderBuckets) {
+        int off = 0;
+        for (int k = 0; k < (t >> 6); ++k) off += misc[1 + k];
+        cur[t] = off + incl - v;
+    }
+    __syncthreads();
+    const int ntile = ncell * split;
+    const int r0 = ncu < ntile ? ncu : ntile;
+    for (int c = t; c < ncell; c += nt) {
+        const int pos = atomicAdd(&cur[bucket(w[c])], 1);
+        for (int p = 0; p < split; ++p) {
+            const int slot = pos * split + p;
+            order[slot < r0 ? r0 - 1 - slot : slot] = c * split + p;
+        }
+    }
+}
+
+// Whether a system's tiles get the load-balanced order (ncu passed to the
+// builder) or keep the cell order (0).
+inline int tile_order_ncu(int ncell, int split, int num_cu)
+{
+    return (ncell <= kOrderMaxCells && ncell * split <= 3 * num_cu) ? num_cu : 0;
+}
+
 // Segment holding tile-local index u (largest st with off[st] <= u; empty
 // segments before it have smaller st, empty ones after it start above u).
 __device__ inline int tile_seg(const TileHdr& h, int u)
