@@ -37,6 +37,10 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 WORKLOADS = {
+    "c1": dict(n=4000, fcc=10, rho=0.8442, rc_lj=2.5, rc_coul=0.0, alpha=0.0, P=0, dt=0.005,
+               steps=1000, kspace="NONE", precision=64, ewald=False, cpu=False,
+               desc="plain LJ fluid, FP64: 4000 atoms on an FCC lattice at density 0.8442, "
+                    "rc 2.5, skin 0.3, T 1.0, dt 0.005, no charges"),
     "c2": dict(n=32768, rho=0.3, rc_lj=2.5, rc_coul=5.0, alpha=0.5, P=100, dt=0.002,
                steps=500, desc="charged LJ electrolyte, plain RBE: 32768 ions, density 0.3, "
                            "rc_LJ 2.5, rc_coul 5.0, alpha 0.5, P=100, dt 0.002"),
@@ -49,6 +53,10 @@ WORKLOADS = {
                desc="learned k-space proposal: 100000 ions, density 0.3, rc_LJ 2.5, "
                     "rc_coul 5.0, alpha 0.5; MLP 4-64-64-1 trained on FP64 |rho(k)|^2 of 50 "
                     "MD snapshots on |m_d| <= 32; learned P=50 vs analytic P=50/100/200/400"),
+    "c6": dict(n=10_000_000, rho=0.3, rc_lj=2.5, rc_coul=6.0, alpha=0.3, P=500, dt=0.002,
+               steps=40, ewald=False, cpu=False,
+               desc="capacity: the large charged-LJ system scaled to 10,000,000 ions, density "
+                    "0.3, rc_LJ 2.5, rc_coul 6.0, alpha 0.3, P=500, dt 0.002"),
     "c5": dict(n=0, replicas=64, n_min=2048, n_max=65536, rho=0.3, rc_lj=2.5, rc_coul=5.0,
                alpha=0.5, P=100, dt=0.002, steps=300, batch=True,
                desc="64 uneven replicas, N_r = floor(2048 * 32^u_r) seeded (log-uniform on "
@@ -62,16 +70,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(name):
-    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
-    (profiles/<name>_traffic.txt, line 'bytes_per_launch: <value>'), else None."""
+def read_traffic(name, key="bytes_per_launch"):
+    """A per-launch figure of a kernel from a committed rocprofv3 --pmc summary
+    (profiles/<name>_traffic.txt, line '<key>: <value>'), else None."""
     path = os.path.join(ROOT, "profiles", f"{name}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+ENGINE_CLOCK_HZ = 2.4e9  # MI355X peak engine clock; one VALU wave-instruction per CU-cycle
 
 
 def cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, target_s):
@@ -112,7 +123,7 @@ def make_force(wl, L, P, kspace="RBE", precision=32, tol=1e-14):
 
     return LJCutCoulKspaceForce(ForceConfig(
         box_len=(L, L, L), cut_off=wl["rc_lj"], coulomb_cut=wl["rc_coul"], skin=0.3,
-        alpha=wl["alpha"], kspace=kspace, coulomb_sample_num=P, precision=precision,
+        alpha=wl["alpha"], kspace=kspace, coulomb_sample_num=max(P, 1), precision=precision,
         n_max=wl["n"], ewald_tol=tol))
 
 
@@ -512,13 +523,21 @@ def main():
         return
     n, P = wl["n"], wl["P"]
     dev = torch.device("cuda:0")
-    log(f"workload {args.workload}: generating {n} ions")
-    x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED)
+    precision = wl.get("precision", 32)
+    kspace = wl.get("kspace", "RBE")
+    if wl.get("fcc"):
+        log(f"workload {args.workload}: FCC lattice of {n} atoms")
+        x, y, z, L = systems.fcc(wl["fcc"], wl["rho"])
+        q = np.zeros(n)
+    else:
+        log(f"workload {args.workload}: generating {n} ions")
+        x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED)
     vx, vy, vz = systems.velocities(n, 1.0, seed=SEED)
     t64 = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to(dev, torch.float64)  # noqa
     Q = t64(q)
+    fdt = torch.float64 if precision == 64 else torch.float32
     st = State(*(t64(a) for a in (x, y, z, vx, vy, vz)),
-               [torch.zeros(n, device=dev, dtype=torch.float32) for _ in range(3)])
+               [torch.zeros(n, device=dev, dtype=fdt) for _ in range(3)])
     use_graph = not args.no_graph
     learned = None
     if wl.get("learned"):
@@ -528,9 +547,10 @@ def main():
             lr["list_builds"]
         eps_total, eps_k = lr["eps_F"], lr["eps_F_kspace_part"]
     else:
-        f = make_force(wl, L, P)
+        f = make_force(wl, L, P, kspace=kspace, precision=precision)
         f.Init(Q)
-        f.sample_kvectors(P, SEED, 0)
+        if kspace == "RBE":
+            f.sample_kvectors(P, SEED, 0)
         f.Execute(st.X, st.Y, st.Z, *st.F)
         if args.tune:
             t = time.perf_counter()
@@ -549,15 +569,18 @@ def main():
 
     if not learned:
         eps_total = eps_k = None
-        if not args.skip_ewald:
+        if not args.skip_ewald and wl.get("ewald", True):
             log("FP64 Ewald reference on the final configuration")
             refs, nkv = ewald_refs(wl, L, Q, [(st.X, st.Y, st.Z)])
             eps_total, eps_k = force_error(f, P, [(st.X, st.Y, st.Z)], refs, 20)
             log(f"force RMS error vs FP64 Ewald: total {eps_total:.3e}, relative to k-space "
                 f"part {eps_k:.3e} ({nkv} half-space k-vectors)")
 
+    valu = read_traffic(f"pair_{args.workload}", "valu_wave_instructions_per_launch")
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    valu_frac = valu / (pair_ms * 1e-3 * n_cu * ENGINE_CLOCK_HZ) if valu else None
     cpu = None
-    if not args.skip_cpu:
+    if not args.skip_cpu and wl.get("cpu", True):
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
         cpu = cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, args.cpu_seconds)
         log(f"CPU baseline: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} threads")
@@ -573,15 +596,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp64" if precision == 64 else "fp32",
         "data": "synthetic",
         "config": {
             "workload": wl["desc"],
             "n_atoms": n,
             "kspace": ("learned MLP proposal (mixture 0.9 learned / 0.1 analytic)"
-                       if learned else "plain RBE (analytic Gaussian sampler)"),
+                       if learned else "plain RBE (analytic Gaussian sampler)"
+                       if kspace == "RBE" else "none (LJ only)"),
             "P": P,
-            "precision": "FP32 pair and k-space maths; FP64 positions, integrator, energy",
+            "precision": ("FP64 throughout" if precision == 64 else
+                          "FP32 pair and k-space maths; FP64 positions, integrator, energy"),
             "hip_graph": use_graph,
             "list_builds_in_timed_steps": rebuilds,
             "force_rms_err_vs_fp64_ewald": eps_total,
@@ -599,6 +624,10 @@ def main():
             "algorithmic_bytes_per_launch": pair_bytes,
             "mean_launch_ms": pair_ms,
             "list_entries": E,
+            "valu_issue_frac": valu_frac,
+            "valu_issue_note": "SQ_INSTS_VALU per launch (profiles/pair_<workload>_traffic.txt) "
+                               "/ (mean launch time x CUs x 2.4 GHz x 1 wave-instruction per "
+                               "CU-cycle); the pass is bound by VALU issue and latency",
         },
         "per_kernel_ms_per_step": per_step_ms,
         "dominant_kernel": dominant,
