@@ -18,6 +18,7 @@
 #include "../../include/rbmd_force.h"
 #include "proposal.h"
 #include "state.h"
+#include "tile.h"
 
 namespace rbmd {
 
@@ -120,11 +121,23 @@ static void free_all(Engine& E)
     if (E.order_event) hipEventDestroy(E.order_event);
 }
 
+// The list build keeps the tile's reference stage and one hit buffer of cap
+// slots per wave in LDS: shrink the stage until both fit a workgroup (larger
+// tiles take the global-memory path).  False if even the smallest stage does not.
+static bool fit_stage(Engine& E)
+{
+    const size_t lim = (size_t)E.lds_max - 4096;
+    while (E.stage_cap > 64 && list_lds_bytes(E.stage_cap, E.cap) > lim) E.stage_cap -= 64;
+    return list_lds_bytes(E.stage_cap, E.cap) <= lim;
+}
+
 static int alloc_list(Engine& E, int cap)
 {
     if (E.nlist) hipFree(E.nlist);
     E.nlist = nullptr;
     E.cap = cap;
+    if (!fit_stage(E))
+        return fail(E, RBMD_ERR_CAPACITY, "Verlet rows of this length do not fit the LDS");
     HIP_TRY(dalloc(&E.nlist, (size_t)E.n_max * (size_t)cap * sizeof(uint16_t)));
     return RBMD_OK;
 }
@@ -209,6 +222,7 @@ static void tile_geometry(Engine& E, int64_t n)
     }
     if (E.stage_override > 0) cap = E.stage_override;
     E.stage_cap = std::max(64, std::min(cap, max_entries));
+    fit_stage(E);
     // tiles resident at once (LDS-limited); split a tile over several
     // workgroups only when there are too few tiles to fill that
     const int lds_wg = E.stage_cap * per + 2048;
@@ -765,7 +779,7 @@ int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int
     for (int64_t i = 0; i < n; ++i) {
         const int64_t s = slot_of[(size_t)i];
         const int ni = nc[(size_t)s].x, no = nc[(size_t)s].y;
-        if (ni + no > E.cap) overflow = true;
+        if (ni + no >= E.cap) overflow = true;
         const int c = scell[(size_t)s];
         if (c != last_c) {
             tile(c, off, gst);
@@ -773,7 +787,7 @@ int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int
         }
         const uint16_t* row = rows.data() + (size_t)s * E.cap;
         for (int k = 0; k < ni + no && !overflow; ++k) {
-            const int u = row[k < ni ? k : E.cap - 1 - (k - ni)];
+            const int u = row[k];
             int st = 26;
             while (off[st] > u) --st;
             if (idx && e < cap) idx[e] = perm[(size_t)(gst[st] + (u - off[st]))];

@@ -6,9 +6,10 @@
 //   xs[n]        double4 (x,y,z,q) current positions, unwrapped against x_ref, cell-sorted
 //   perm[n]      int32  sorted slot -> original atom id
 //   scell[n]     int32  cell of each sorted slot
-//   nlist[n*C]   int32  row-major Verlet rows in sorted ids: LJ-range ("inner") entries
-//                       grow from the row start, the rest ("outer") from the row end
-//   ncount[n]    int2   (inner, outer) per row
+//   nlist[n*C]   uint16 row-major Verlet rows of tile-local indices: the LJ-range
+//                       ("inner") entries, then the rest ("outer"), then one sentinel
+//   ncount[n]    int2   (inner, outer) per row; (n, 0) with n >= C for a row that
+//                       did not fit
 //   cell_count/cell_start  int32 counting-sort histogram and CSR bounds
 //   k-space: kvec (m, w), partial structure factors [blocks][P] (FP64), rho[P], per-k
 //   force coefficients, per-k energy/virial terms

@@ -114,17 +114,12 @@ struct PairMath<double> {
     }
 };
 
-// One pair: fs from PairMath, force accumulated in T (fx += fs * dx), and with
-// EV the energies and the virial W_ab += -1/2 fs dx_a dx_b in FP64.  qq already
-// carries the Coulomb unit factor.
+// Force factor fs of one pair (force on i = fs * r_ij, r_ij = r_j - r_i) and,
+// with EV, its two energies.  qq already carries the Coulomb unit factor.
 template <typename T, bool EV, bool MULTI, bool LJ>
-__device__ inline void accumulate_pair(double dx, double dy, double dz, double qq, int ti, int tj,
-                                       bool valid, const PairScalars& ps,
-                                       const PairParams* __restrict__ pp, T& fx, T& fy, T& fz,
-                                       double* acc)
+__device__ inline T pair_fs(double d2, double qq, int ti, int tj, bool ok, const PairScalars& ps,
+                            const PairParams* __restrict__ pp, T* e_lj, T* e_re)
 {
-    const double d2 = dx * dx + dy * dy + dz * dz;
-    const bool ok = valid && d2 > 1e-4;
     T eps24 = 0, sig2 = 0;
     if (LJ) {
         if (!MULTI) {
@@ -135,13 +130,30 @@ __device__ inline void accumulate_pair(double dx, double dy, double dz, double q
             sig2 = (T)pp->sig2[ti * kMaxTypes + tj];
         }
     }
+    return PairMath<T>::template eval<LJ>(d2, qq, eps24, sig2, ps, ok, e_lj, e_re, EV);
+}
+
+// One pair: fs (with LJ when lj, a wave-uniform choice), then the force
+// accumulated in FP64 (one FMA per component, no conversion of dx), and with EV
+// the energies and the virial W_ab += -1/2 fs dx_a dx_b in FP64.  The two
+// variants differ only in fs, so the accumulation follows the branch once.
+template <typename T, bool EV, bool MULTI>
+__device__ inline void accumulate_pair(bool lj, double dx, double dy, double dz, double qq, int ti,
+                                       int tj, bool valid, const PairScalars& ps,
+                                       const PairParams* __restrict__ pp, double& fx, double& fy,
+                                       double& fz, double* acc)
+{
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const bool ok = valid && d2 > 1e-4;
     T e_lj = 0, e_re = 0;
-    const T fs = PairMath<T>::template eval<LJ>(d2, qq, eps24, sig2, ps, ok, &e_lj, &e_re, EV);
-    fx += fs * (T)dx;
-    fy += fs * (T)dy;
-    fz += fs * (T)dz;
+    const T fs = lj ? pair_fs<T, EV, MULTI, true>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re)
+                    : pair_fs<T, EV, MULTI, false>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re);
+    const double f = (double)fs;
+    fx = fma(f, dx, fx);
+    fy = fma(f, dy, fy);
+    fz = fma(f, dz, fz);
     if (EV && ok) {
-        const double h = -0.5 * (double)fs;
+        const double h = -0.5 * f;
         acc[0] += (double)e_lj;
         acc[1] += (double)e_re;
         acc[2] += h * dx * dx;
@@ -155,14 +167,18 @@ __device__ inline void accumulate_pair(double dx, double dy, double dz, double q
 
 // Position of tile-local candidate u, already moved to the image next to the
 // home cell (from LDS when the tile is staged, else from global memory).
+// The row sentinel u = total reads the tile's dummy entry (staged at lds[total]).
 template <bool STAGED>
 __device__ inline double4 tile_pos(const TileHdr& h, const double4* __restrict__ lds,
                                    const double4* __restrict__ xs, int u)
 {
     if (STAGED) return lds[u];
-    const int st = tile_seg(h, u);
-    const double4 p = xs[h.gstart[st] + (u - h.off[st])];
-    return make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+    const int total = h.off[27];
+    const int uc = u < total ? u : 0;
+    const int st = tile_seg(h, uc);
+    const double4 p = xs[h.gstart[st] + (uc - h.off[st])];
+    const double4 d = h.dummy;
+    return u < total ? make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w) : d;
 }
 
 template <bool STAGED>
@@ -170,8 +186,11 @@ __device__ inline int tile_type(const TileHdr& h, const int32_t* __restrict__ lt
                                 const int32_t* __restrict__ ts, int u)
 {
     if (STAGED) return ltype[u];
-    const int st = tile_seg(h, u);
-    return ts[h.gstart[st] + (u - h.off[st])];
+    const int total = h.off[27];
+    const int uc = u < total ? u : 0;
+    const int st = tile_seg(h, uc);
+    const int t = ts[h.gstart[st] + (uc - h.off[st])];
+    return u < total ? t : 0;
 }
 
 // Row chunks per atom per round, and lanes per home-atom row (16 or 32).  On the
@@ -234,8 +253,13 @@ __device__ inline int wave_max_int(int v)
 // workgroup's atoms are dealt to the groups in turn).  A round covers
 // kRound x kGroup entries per atom: all index loads, then all LDS gathers, then
 // the maths; rounds that can still hold LJ-range entries of any of the wave's
-// atoms evaluate LJ, later rounds skip it.  Rows that overflowed their capacity
-// scan every tile candidate instead.
+// atoms evaluate LJ, later rounds skip it.  When the wave's four rows all
+// fitted their capacity (the common case) a lane reads slot min(v, n) of its
+// row, n the row length: slots past the end read the sentinel, whose dummy
+// entry contributes exactly zero, so the round needs no bounds tests.
+// Otherwise (a row that overflowed, or lane groups past the tile's last atom)
+// the round tests each slot, and rows that overflowed scan every tile candidate
+// instead.
 template <typename T, bool EV, bool MULTI, bool STAGED>
 __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
                                  const double4* __restrict__ lds,
@@ -248,66 +272,75 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
     const int cap = a.cap;
     constexpr int kPer = 64 / kGroup;  // atoms per wave
     const int g = lane / kGroup, gl = lane % kGroup;
+    const uint16_t* __restrict__ rows = a.nlist + (size_t)uniform(h0) * cap;  // the tile's rows
     for (int t0 = a0 + kPer * wv; t0 < a1; t0 += kPer * kTileWaves) {
         const int s = t0 + g;
         const bool live = s < a1;
         const int2 nc = live ? a.ncount[s] : make_int2(0, 0);
         const int n_in = nc.x, tot = nc.x + nc.y;
-        const bool over = tot > cap;
+        const bool over = tot >= cap;
         const int tl = over ? 0 : tot;
         const int tmax = wave_max_int(tl);
         const int nin_max = wave_max_int(over ? 0 : n_in);
+        const bool fast = __ballot(!live || over) == 0ull;
         const int ui = h.off[kHomeSeg] + (live ? s - h0 : 0);
         const double4 pi = STAGED ? lds[ui] : a.xs[live ? s : h0];
         const int ti = MULTI ? (STAGED ? ltype[ui] : a.ts[live ? s : h0]) : 0;
         const double qi = pi.w * ps.qqr2e;
-        const uint16_t* __restrict__ row = a.nlist + (size_t)(live ? s : h0) * cap;
-        T ax = 0, ay = 0, az = 0;
-        // row indices of round base (tile-local, -1 past the row's end)
-        auto load_u = [&](int base, int* u) {
+        const uint32_t rof = (uint32_t)(live ? s - h0 : 0) * (uint32_t)cap + (uint32_t)gl;
+        double ax = 0, ay = 0, az = 0;
+        auto rounds = [&](auto fast_tag) {
+            constexpr bool FAST = decltype(fast_tag)::value;
+            // row indices of round base: tile-local, the sentinel past the end
+            // (FAST), -1 past the end otherwise
+            auto load_u = [&](int base, int* u) {
 #pragma unroll
-            for (int k = 0; k < kRound; ++k) {
-                const int v = base + kGroup * k + gl;
-                const int idx = v < n_in ? v : cap - 1 - (v - n_in);
-                u[k] = v < tl ? (int)row[idx] : -1;
+                for (int k = 0; k < kRound; ++k) {
+                    const int v = base + kGroup * k;
+                    if (FAST)
+                        u[k] = (int)rows[rof + (uint32_t)min(v, tl - gl)];
+                    else
+                        u[k] = v + gl < tl ? (int)rows[rof + (uint32_t)v] : -1;
+                }
+            };
+            int un[kRound];
+            load_u(0, un);
+            for (int base = 0; base < tmax; base += kGroup * kRound) {
+                // software pipeline: the next round's index loads are in flight
+                // during this round's gathers and maths
+                int u[kRound];
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) u[k] = un[k];
+                if (base + kGroup * kRound < tmax) load_u(base + kGroup * kRound, un);
+                double4 pj[kRound];
+                int tj[kRound] = {};
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) {
+                    const int uu = FAST ? u[k] : (u[k] < 0 ? 0 : u[k]);
+                    pj[k] = tile_pos<STAGED>(h, lds, a.xs, uu);
+                    if (MULTI) tj[k] = tile_type<STAGED>(h, ltype, a.ts, uu);
+                }
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) {
+                    if (base + kGroup * k < tmax)
+                        accumulate_pair<T, EV, MULTI>(
+                            base + kGroup * k < nin_max, pj[k].x - pi.x, pj[k].y - pi.y,
+                            pj[k].z - pi.z, qi * pj[k].w, ti, tj[k], FAST || u[k] >= 0, ps, pp,
+                            ax, ay, az, acc);
+                }
             }
         };
-        int un[kRound];
-        load_u(0, un);
-        for (int base = 0; base < tmax; base += kGroup * kRound) {
-            // software pipeline: the next round's index loads are in flight
-            // during this round's gathers and maths
-            int u[kRound];
-#pragma unroll
-            for (int k = 0; k < kRound; ++k) u[k] = un[k];
-            if (base + kGroup * kRound < tmax) load_u(base + kGroup * kRound, un);
-            double4 pj[kRound];
-            int tj[kRound] = {};
-#pragma unroll
-            for (int k = 0; k < kRound; ++k) {
-                const int uu = u[k] < 0 ? 0 : u[k];
-                pj[k] = tile_pos<STAGED>(h, lds, a.xs, uu);
-                if (MULTI) tj[k] = tile_type<STAGED>(h, ltype, a.ts, uu);
-            }
-#pragma unroll
-            for (int k = 0; k < kRound; ++k) {
-                if (base + kGroup * k < nin_max)
-                    accumulate_pair<T, EV, MULTI, true>(pj[k].x - pi.x, pj[k].y - pi.y,
-                                                        pj[k].z - pi.z, qi * pj[k].w, ti, tj[k],
-                                                        u[k] >= 0, ps, pp, ax, ay, az, acc);
-                else if (base + kGroup * k < tmax)
-                    accumulate_pair<T, EV, MULTI, false>(pj[k].x - pi.x, pj[k].y - pi.y,
-                                                         pj[k].z - pi.z, qi * pj[k].w, ti, tj[k],
-                                                         u[k] >= 0, ps, pp, ax, ay, az, acc);
-            }
-        }
+        if (fast)
+            rounds(std::true_type{});
+        else
+            rounds(std::false_type{});
         if (__ballot(live && over) != 0ull) {
             for (int uu = gl; uu < total; uu += kGroup) {
                 const double4 pj = tile_pos<STAGED>(h, lds, a.xs, uu);
                 const int tj = MULTI ? tile_type<STAGED>(h, ltype, a.ts, uu) : 0;
-                accumulate_pair<T, EV, MULTI, true>(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
-                                                    qi * pj.w, ti, tj, live && over, ps, pp, ax,
-                                                    ay, az, acc);
+                accumulate_pair<T, EV, MULTI>(true, pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
+                                              qi * pj.w, ti, tj, live && over, ps, pp, ax, ay,
+                                              az, acc);
             }
         }
         ax = group_sum(ax);
@@ -315,9 +348,9 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
         az = group_sum(az);
         if (live && gl == kSumLane) {
             const int i = a.perm[s];
-            fx[i] = ax;
-            fy[i] = ay;
-            fz[i] = az;
+            fx[i] = (T)ax;
+            fy[i] = (T)ay;
+            fz[i] = (T)az;
         }
     }
 }
@@ -394,7 +427,7 @@ struct TilePass {
         }
         tile_setup(c, A.dims, A.box, A.cell_start, h);
         const int total = h.off[27];
-        const bool staged = total <= A.stage_cap;
+        const bool staged = total < A.stage_cap;  // room for the sentinel's dummy entry
         const int lane = threadIdx.x & 63;
         const int wv = uniform(threadIdx.x >> 6);
         const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
@@ -405,14 +438,18 @@ struct TilePass {
             if (staged) {
                 for (int u = threadIdx.x; u < total; u += kTileThreads)
                     rs.put(u, stage_ref(h, A.xs_ref, u));
+                const int pad = (total + 127) & ~127;  // the sweep reads whole chunks
+                for (int u = total + (int)threadIdx.x; u < pad; u += kTileThreads)
+                    rs.put(u, h.dummy);
             }
             __syncthreads();
+            uint16_t* hb = hit_buffer(lds, A.stage_cap, a.cap, wv);
             if (staged)
-                list_wave<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
+                list_wave<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap, hb,
                                 A.nlist, A.ncount, A.stats);
             else
                 list_wave<false>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
-                                 A.nlist, A.ncount, A.stats);
+                                 hb, A.nlist, A.ncount, A.stats);
             __syncthreads();  // rows and LDS are reused below
         }
         if (staged) {
@@ -438,6 +475,10 @@ struct TilePass {
                         if (MULTI) ltype[u] = t[k];
                     }
                 }
+            }
+            if (threadIdx.x == 0) {
+                lds[total] = h.dummy;
+                if (MULTI) ltype[total] = 0;
             }
         }
         __syncthreads();
@@ -567,7 +608,9 @@ typename TilePass<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void*
 
 size_t pair_lds(const Engine& e)
 {
-    return (size_t)e.stage_cap * (sizeof(double4) + (e.pp.ntypes > 1 ? 4 : 0));
+    const size_t stage = (size_t)e.stage_cap * (sizeof(double4) + (e.pp.ntypes > 1 ? 4 : 0));
+    const size_t build = list_lds_bytes(e.stage_cap, e.cap);
+    return stage > build ? stage : build;
 }
 
 template <typename T, bool EV, bool M>

@@ -22,6 +22,7 @@ struct TileHdr {
     int gstart[27];  // first cell-sorted index of each segment
     int cnt[27];
     double sx[27], sy[27], sz[27];
+    double4 dummy;   // position of the row sentinel `total`: beyond every cutoff, charge 0
 };
 
 #ifdef __HIPCC__
@@ -62,6 +63,10 @@ __device__ inline void tile_setup(int c, int3 dims, const Box& box,
         }
         if (t < 27) h.off[t] = incl - cnt;
         if (t == 26) h.off[27] = incl;
+        if (t == 0)
+            h.dummy = make_double4(box.lo[0] + 8.0 * box.L[0] + 64.0,
+                                   box.lo[1] + 8.0 * box.L[1] + 64.0,
+                                   box.lo[2] + 8.0 * box.L[2] + 64.0, 0.0);
     }
     __syncthreads();
 }
@@ -245,13 +250,24 @@ __device__ inline int tile_seg(const TileHdr& h, int u)
 // tile 64 candidates at a time.  A candidate is kept iff
 // (r_list^2 - d2 > 1e-4) && (d2 > 1e-4) with dx = (x_j - s) - x_i and
 // d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the reference's operation order
-// (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  Hits are
-// compacted with ballot + mbcnt into the row as uint16 tile-local indices:
-// LJ-range hits (d2 < r_in^2) from the row start, the others from the row end;
-// each candidate costs one masked store and no branches.  The stage is two
-// arrays, (x, y) pairs then z, so the sweep's consecutive reads are
+// (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  The stage is
+// two arrays, (x, y) pairs then z, so the sweep's consecutive reads are
 // bank-conflict-free.  Tiles larger than the LDS stage read candidates from
-// global memory; tiles beyond the uint16 range mark their rows overflowed.
+// global memory; tiles beyond the 15-bit index range mark their rows overflowed.
+//
+// Row layout (uint16 tile-local indices, cap slots per home atom): the LJ-range
+// entries (d2 < r_in^2) first, then the others, then one sentinel slot holding
+// the tile's dummy index `total` (staged far from every atom), so a reader may
+// clamp its slot to the row length instead of testing it.  A row of n entries
+// needs n + 1 <= cap slots; ncount = (LJ-range, other) entries, or (n, 0) with
+// n >= cap for a row that did not fit.
+//
+// The sweep compacts hits with one ballot + mbcnt into this wave's LDS hit
+// buffer (cap entries, LJ-range hits tagged with kInnerFlag), then one pass
+// over the buffer writes the row in its final order.
+constexpr int kInnerFlag = 0x8000;
+constexpr int kMaxTile = 0x7fff;  // tile-local indices and the sentinel fit in 15 bits
+
 __device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict__ xs_ref, int u)
 {
 #pragma clang fp contract(off)
@@ -261,12 +277,16 @@ __device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict_
 }
 
 // Reference-position stage of list_wave: (x, y) of candidate u at xy[u], z at
-// zz[u]; xy and zz partition an LDS region of stage_cap double4 entries.
+// zz[u], for u below the stage size rounded up to the sweep's 128 candidates
+// (entries past the tile hold its dummy position, so the sweep needs no bounds
+// test).  The per-wave hit buffers follow (hit_buffer).
+__host__ __device__ inline int ref_stage_len(int stage_cap) { return (stage_cap + 127) & ~127; }
+
 struct RefStage {
     double2* xy;
     double* zz;
     __device__ RefStage(double4* lds, int stage_cap)
-        : xy((double2*)lds), zz((double*)lds + 2 * (size_t)stage_cap)
+        : xy((double2*)lds), zz((double*)lds + 2 * (size_t)ref_stage_len(stage_cap))
     {
     }
     __device__ void put(int u, const double4& p) const
@@ -281,17 +301,29 @@ struct RefStage {
     }
 };
 
+// LDS bytes the list build needs beside the reference stage: cap uint16 per wave.
+inline size_t list_lds_bytes(int stage_cap, int cap)
+{
+    return 24 * (size_t)ref_stage_len(stage_cap) +
+           (size_t)kTileWaves * (size_t)cap * sizeof(uint16_t);
+}
+
+__device__ inline uint16_t* hit_buffer(double4* lds, int stage_cap, int cap, int wv)
+{
+    return (uint16_t*)((double*)lds + 3 * (size_t)ref_stage_len(stage_cap)) + (size_t)wv * cap;
+}
+
 template <bool STAGED>
 __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
                                  const double4* __restrict__ xs_ref, int first, int a1, int h0,
-                                 int lane, double rl2, double rin2, int cap,
+                                 int lane, double rl2, double rin2, int cap, uint16_t* hb,
                                  uint16_t* __restrict__ nlist, int2* __restrict__ ncount,
                                  unsigned long long* __restrict__ stats)
 {
 #pragma clang fp contract(off)
     const int total = h.off[27];
     for (int s = first; s < a1; s += kTileWaves) {
-        if (total > 65535) {
+        if (total >= kMaxTile) {
             if (lane == 0) {
                 ncount[s] = make_int2(cap + 1, 0);
                 atomicAdd(&stats[2], 1ull);
@@ -307,14 +339,13 @@ __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
             const double4 p = xs_ref[s];
             pi = make_double3(p.x, p.y, p.z);
         }
-        int n_in = 0, n_out = 0;
-        uint16_t* __restrict__ row = nlist + (size_t)s * cap;
+        int n = 0;
         for (int ub = 0; ub < total; ub += 128) {
             double3 pj[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const int u = ub + 64 * k + lane;
-                const int uu = u < total ? u : 0;
+                const int uu = STAGED || u < total ? u : 0;
                 if (STAGED) {
                     pj[k] = rs.get(uu);
                 } else {
@@ -329,23 +360,48 @@ __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
                 const double dy = pj[k].y - pi.y;
                 const double dz = pj[k].z - pi.z;
                 const double d2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                const bool hit = (u < total) && ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
-                const bool inner = d2 < rin2;
-                const uint64_t mh = __ballot(hit), mn = __ballot(inner);
-                const uint64_t mi = mh & mn, mo = mh & ~mn;
-                const int pin = n_in + (int)mbcnt64(mi);
-                const int pout = n_out + (int)mbcnt64(mo);
-                const int slot = inner ? pin : cap - 1 - pout;
-                if (hit && (inner ? pin : pout) < cap) row[slot] = (uint16_t)u;
-                n_in += __popcll(mi);
-                n_out += __popcll(mo);
+                // staged: padding entries past the tile are far, so they miss
+                const bool live = STAGED || u < total;
+                const bool hit = live && ((rl2 - d2) > 1e-4) && (d2 > 1e-4);
+                // the same mask as ballot(hit), formed from the compare masks
+                // directly (no per-lane boolean round trip)
+                const uint64_t mh = __builtin_amdgcn_ballot_w64(live) &
+                                    __builtin_amdgcn_ballot_w64((rl2 - d2) > 1e-4) &
+                                    __builtin_amdgcn_ballot_w64(d2 > 1e-4);
+                const int slot = n + (int)mbcnt64(mh);
+                if (hit && slot < cap) hb[slot] = (uint16_t)(d2 < rin2 ? (u | kInnerFlag) : u);
+                n += __popcll(mh);
             }
         }
+        uint16_t* __restrict__ row = nlist + (size_t)s * cap;
+        int n_in = n;
+        if (n < cap) {
+            __syncwarp();  // the hit buffer, written by every lane of this wave
+            n_in = 0;
+            for (int k0 = 0; k0 < n; k0 += 64) {
+                const int k = k0 + lane;
+                n_in += __popcll(__ballot(k < n && (hb[k] & kInnerFlag) != 0));
+            }
+            int ci = 0, co = n_in;
+            for (int k0 = 0; k0 < n; k0 += 64) {
+                const int k = k0 + lane;
+                const int e = k < n ? (int)hb[k] : 0;
+                const bool in = k < n && (e & kInnerFlag) != 0;
+                const bool out = k < n && (e & kInnerFlag) == 0;
+                const uint64_t mi = __ballot(in), mo = __ballot(out);
+                if (in) row[ci + (int)mbcnt64(mi)] = (uint16_t)(e & kMaxTile);
+                if (out) row[co + (int)mbcnt64(mo)] = (uint16_t)e;
+                ci += __popcll(mi);
+                co += __popcll(mo);
+            }
+            if (lane == 0) row[n] = (uint16_t)total;
+            __syncwarp();  // the buffer is reused for the wave's next atom
+        }
         if (lane == 0) {
-            ncount[s] = make_int2(n_in, n_out);
-            if (n_in + n_out > cap) {
+            ncount[s] = n < cap ? make_int2(n_in, n - n_in) : make_int2(n, 0);
+            if (n >= cap) {
                 atomicAdd(&stats[2], 1ull);
-                atomicMax(&stats[4], (unsigned long long)(n_in + n_out));
+                atomicMax(&stats[4], (unsigned long long)(n + 1));
             }
         }
     }
