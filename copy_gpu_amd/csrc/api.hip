@@ -1128,6 +1128,7 @@ void rbmd_batch_destroy(rbmd_batch_t b)
         if (G.graph) hipGraphDestroy(G.graph);
     }
     for (auto& ph : b->b.ph) ph.release();
+    for (auto& ph : b->b.pair_more) ph.release();
     for (rbmd_force_t h : b->reps) rbmd_force_destroy(h);
     if (b->own_stream) hipStreamDestroy(b->own_stream);
     if (b->order_event) hipEventDestroy(b->order_event);
@@ -1207,12 +1208,14 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         BHIP_TRY(hipGetLastError());
     } else {
         std::vector<const void*> key;
-        for (const BatchPhase& ph : B.ph) {
+        auto key_phase = [&](const BatchPhase& ph) {
             key.push_back(ph.args);
             key.push_back(ph.map);
             key.push_back((const void*)(uintptr_t)ph.items);
             key.push_back((const void*)(uintptr_t)ph.lds);
-        }
+        };
+        for (const BatchPhase& ph : B.ph) key_phase(ph);
+        for (const BatchPhase& ph : B.pair_more) key_phase(ph);
         for (int g = 0; g < 2; ++g) {
             Engine::StepGraph& G = b->graphs[g];
             const int steps = g == 0 ? kGraphSteps : 1;

@@ -642,21 +642,54 @@ void tile_rho_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
     k_one<TileRho<T, false, M>><<<a.nfin + e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(a);
 }
 
+// Tile workgroups of a replica that fit one CU at once with its LDS stage.
+inline int tile_wgs_per_cu(const Engine& e)
+{
+    const int per = (int)(pair_lds(e) + 2048);
+    return std::max(1, std::min(Batch::kPairClasses, e.lds_cu / per));
+}
+
+// The batched tile pass: one launch per class of replicas that fit the same
+// number of tile workgroups per CU, with that class's largest stage.  A single
+// launch sized for the largest stage (small boxes, whose three-cell stencil spans
+// the whole replica) would hold every replica to one workgroup per CU.
 template <typename T, bool M>
 hipError_t pair_prepare_t(Batch& b)
 {
     const size_t R = b.es.size();
     std::vector<typename TileRho<T, false, M>::Args> ta(R);
-    std::vector<int> nt(R);
-    size_t lds = 0;
+    std::vector<int> wg(R);
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
         ta[r] = tile_rho_args<T, M>(e, b.st[r].fx, b.st[r].fy, b.st[r].fz, e.d_step);
-        nt[r] = ta[r].nfin + e.ncell * e.split;
-        const size_t l = pair_lds(e);
-        lds = l > lds ? l : lds;
+        wg[r] = tile_wgs_per_cu(e);
     }
-    return phase_upload(b.ph[BP_PAIR], ta, nt, lds);
+    // classes in order of decreasing occupancy; phase c holds class c's replicas
+    int slot = 0;
+    for (int w = Batch::kPairClasses; w >= 1; --w) {
+        std::vector<int> nt(R, 0);
+        size_t lds = 0;
+        bool any = false;
+        for (size_t r = 0; r < R; ++r) {
+            if (wg[r] != w) continue;
+            Engine& e = *b.es[r];
+            nt[r] = ta[r].nfin + e.ncell * e.split;
+            lds = std::max(lds, pair_lds(e));
+            any = true;
+        }
+        if (!any) continue;
+        BatchPhase& ph = slot == 0 ? b.ph[BP_PAIR] : b.pair_more[slot - 1];
+        const hipError_t err = phase_upload(ph, ta, nt, lds);
+        if (err != hipSuccess) return err;
+        ++slot;
+    }
+    // classes no longer present launch nothing
+    for (; slot < Batch::kPairClasses; ++slot) {
+        BatchPhase& ph = slot == 0 ? b.ph[BP_PAIR] : b.pair_more[slot - 1];
+        const hipError_t err = phase_upload(ph, ta, std::vector<int>(R, 0), 0);
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -713,7 +746,10 @@ void pair_batch_launch(Batch& b, int phase, hipStream_t s)
 {
 #define RBMD_PB(T, M)                                                                         \
     do {                                                                                      \
-        if (phase == BP_PAIR) phase_launch<TileRho<T, false, M>>(b.ph[phase], s);           \
+        if (phase == BP_PAIR) {                                                               \
+            phase_launch<TileRho<T, false, M>>(b.ph[BP_PAIR], s);                           \
+            for (const BatchPhase& ph : b.pair_more) phase_launch<TileRho<T, false, M>>(ph, s); \
+        }                                                                                     \
     } while (0)
     if (b.precision == 32) {
         if (b.multi) RBMD_PB(float, true);

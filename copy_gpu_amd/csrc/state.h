@@ -187,6 +187,10 @@ struct Batch {
     int kspace = 0;
     bool multi = false;
     BatchPhase ph[BP_NUM];
+    // the tile pass runs one launch per LDS class of replicas: ph[BP_PAIR] holds
+    // the first class, pair_more the others (pair.hip pair_prepare_t)
+    static constexpr int kPairClasses = 4;
+    BatchPhase pair_more[kPairClasses - 1];
 };
 
 // Batched phases: prepare packs and uploads every replica's arguments (host,
