@@ -106,7 +106,7 @@ static void free_all(Engine& E)
     void* ptrs[] = {E.xr,       E.yr,    E.zr,     E.xs_ref, E.xs,        E.perm,     E.scell,
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
                     E.tile_order,
-                    E.md_count, E.md_bucket, E.md_cell, E.md_w,
+                    E.md_count, E.md_bucket, E.md_cell, E.md_vid, E.md_w,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
                     E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide};
@@ -370,6 +370,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
         CR(dalloc(&E.md_count, (size_t)2 * E.ncell * 4));
         CR(dalloc(&E.md_bucket, (size_t)2 * E.ncell * E.bucket_cap * 4));
         CR(dalloc(&E.md_cell, N * 4));
+        CR(dalloc(&E.md_vid, N * 4));
         CR(dalloc(&E.md_w, N * 3 * 8));
     }
     CR(dalloc(&E.cell_start, ((size_t)E.ncell + 1) * 4));

@@ -47,11 +47,15 @@ struct PairParams {
 // buffer (epoch + 1) & 1 and clears buffer epoch & 1, which the cell sort of the
 // same step has consumed; the k-space force (or kick) kernel then advances the
 // epoch.
+// MD-step bins: an atom is binned under its cell-sorted slot at the binning, so
+// the per-atom records are written in slot order (whole cache lines from a tile's
+// consecutive home atoms) rather than scattered by atom id.
 struct MdBins {
     int32_t* count;        // [2][ncell]
-    int32_t* bucket;       // [2][ncell][cap] atom ids
-    int32_t* cell;         // [n] cell of each atom at its binning
-    double *xw, *yw, *zw;  // [n] wrapped positions at the binning
+    int32_t* bucket;       // [2][ncell][cap] slots at the binning
+    int32_t* cell;         // [n] by slot: cell at the binning
+    int32_t* vid;          // [n] by slot: atom id
+    double *xw, *yw, *zw;  // [n] by slot: wrapped positions at the binning
     int ncell, cap;
 };
 

@@ -226,8 +226,8 @@ struct KForceDrift {
 // ---------------------------------------------------------------- MD cell sort
 // Rebuild-step cell sort from the bins the tile pass made one step earlier
 // (buffer flags[3] & 1): one wave per cell.  The cell's start is the sum of the
-// counts before it; its ids come from the bucket (arrival order) ranked by id,
-// or, for a bucket that overflowed, from a scan of every atom's cell.  The
+// counts before it; its atoms come from the bucket (arrival order) ranked by id,
+// or, for a bucket that overflowed, from a scan of every binning slot's cell.  The
 // sorted slot s of atom v receives perm, inv_perm, the binned position as the
 // new reference, the packed reference and current (unwrapped) positions, its
 // cell and type.  The wave of cell 0 publishes the rebuild decision (flags[2])
@@ -245,11 +245,12 @@ struct CellSortMDArgs {
     unsigned long long* stats;
 };
 
-__device__ inline void cellsort_md_emit(const CellSortMDArgs& a, int c, int v, int s)
+// Atom v, binned in slot so, takes sorted slot s.
+__device__ inline void cellsort_md_emit(const CellSortMDArgs& a, int c, int v, int so, int s)
 {
     a.perm[s] = v;
     a.inv_perm[v] = s;
-    const double w0 = a.bins.xw[v], w1 = a.bins.yw[v], w2 = a.bins.zw[v];
+    const double w0 = a.bins.xw[so], w1 = a.bins.yw[so], w2 = a.bins.zw[so];
     a.xr[v] = w0;
     a.yr[v] = w1;
     a.zr[v] = w2;
@@ -263,7 +264,7 @@ __device__ inline void cellsort_md_emit(const CellSortMDArgs& a, int c, int v, i
 }
 
 // One wave sorts cell c (c >= ncell: nothing); buf holds kSortWaveBuf ids of
-// this wave's own LDS.
+// this wave's own LDS.  The bucket lists binning slots; atoms are ranked by id.
 __device__ inline void cellsort_md_cell(const CellSortMDArgs& a, int c, int lane, int32_t* buf)
 {
     const uint32_t req = a.flags[0];
@@ -290,27 +291,33 @@ __device__ inline void cellsort_md_cell(const CellSortMDArgs& a, int c, int lane
         if (c == a.ncell - 1) a.cell_start[a.ncell] = s0 + cnt;
     }
     if (cnt <= a.bins.cap) {
-        // rank by id; ids staged in LDS when they fit, else read from the bucket
         const int32_t* bk = a.bins.bucket + ((size_t)b * a.ncell + c) * a.bins.cap;
-        const int32_t* src = bk;
         if (cnt <= kSortWaveBuf) {
-            for (int k = lane; k < cnt; k += 64) buf[k] = bk[k];
+            // ids staged in LDS, each lane ranks its own entries
+            for (int k = lane; k < cnt; k += 64) buf[k] = a.bins.vid[bk[k]];
             __syncwarp();
-            src = buf;
-        }
-        for (int k = lane; k < cnt; k += 64) {
-            const int v = src[k];
-            int r = 0;
-            for (int t = 0; t < cnt; ++t) r += (src[t] < v) ? 1 : 0;
-            cellsort_md_emit(a, c, v, s0 + r);
+            for (int k = lane; k < cnt; k += 64) {
+                const int v = buf[k];
+                int r = 0;
+                for (int t = 0; t < cnt; ++t) r += (buf[t] < v) ? 1 : 0;
+                cellsort_md_emit(a, c, v, bk[k], s0 + r);
+            }
+        } else {
+            for (int k = lane; k < cnt; k += 64) {
+                const int v = a.bins.vid[bk[k]];
+                int r = 0;
+                for (int t = 0; t < cnt; ++t) r += (a.bins.vid[bk[t]] < v) ? 1 : 0;
+                cellsort_md_emit(a, c, v, bk[k], s0 + r);
+            }
         }
     } else {
+        // the bucket overflowed: scan every binning slot (order: by slot)
         int found = 0;
         for (int base = 0; base < a.n; base += 64) {
             const int i = base + lane;
             const bool hit = i < a.n && a.bins.cell[i] == c;
             const uint64_t m = __ballot(hit);
-            if (hit) cellsort_md_emit(a, c, i, s0 + found + (int)mbcnt64(m));
+            if (hit) cellsort_md_emit(a, c, a.bins.vid[i], i, s0 + found + (int)mbcnt64(m));
             found += __popcll(m);
         }
     }

@@ -234,7 +234,8 @@ struct MdBin {
         const int i = blk * kBlock + threadIdx.x;
         if (i >= a.n) return;
         const double p[3] = {a.x[i], a.y[i], a.z[i]};
-        md_bin_atom(a.bins, (int)(a.flags[3] & 1u), i, p, a.box, a.dims, a.cell);
+        // any one-to-one slot numbering serves the cell sort: atom i bins in slot i
+        md_bin_atom(a.bins, (int)(a.flags[3] & 1u), i, i, p, a.box, a.dims, a.cell);
     }
 };
 

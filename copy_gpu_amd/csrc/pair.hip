@@ -507,7 +507,7 @@ struct TilePass {
                     p[2] = p4.z;
                     v = a.perm[s];
                 }
-                md_bin_wave(A.bins, 1 - b, act, v, p, A.box, A.dims, A.cell);
+                md_bin_wave(A.bins, 1 - b, act, s, v, p, A.box, A.dims, A.cell);
             }
         }
     }

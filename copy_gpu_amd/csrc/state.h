@@ -82,8 +82,9 @@ struct Engine {
     // bins of the MD step's next rebuild (double-buffered by the epoch flags[3])
     int32_t* md_count = nullptr;   // [2][ncell]
     int32_t* md_bucket = nullptr;  // [2][ncell][bucket_cap]
-    int32_t* md_cell = nullptr;    // [n]
-    double* md_w = nullptr;        // [3][n] wrapped positions at binning
+    int32_t* md_cell = nullptr;    // [n] by slot at binning
+    int32_t* md_vid = nullptr;     // [n] by slot at binning: atom id
+    double* md_w = nullptr;        // [3][n] by slot: wrapped positions at binning
     int bucket_cap = 0;
     uint16_t* nlist = nullptr;  // tile-local indices
     int2* ncount = nullptr;
@@ -212,8 +213,8 @@ inline bool has_kspace(const Engine& e)
 inline MdBins md_bins(const Engine& e)
 {
     const size_t n = (size_t)std::max<int64_t>(e.n_max, 1);
-    return MdBins{e.md_count, e.md_bucket, e.md_cell,  e.md_w,
-                  e.md_w + n, e.md_w + 2 * n, e.ncell, e.bucket_cap};
+    return MdBins{e.md_count, e.md_bucket, e.md_cell,   e.md_vid,    e.md_w,
+                  e.md_w + n,  e.md_w + 2 * n, e.ncell, e.bucket_cap};
 }
 
 // ---- launchers (nlist.hip)

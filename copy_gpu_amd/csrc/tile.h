@@ -95,18 +95,20 @@ __device__ inline int wrap_cell(const double p[3], const Box& box, int3 dims, do
 
 // Wave-aggregated binning for lanes that mostly share one cell (the home atoms
 // of a tile): one atomic per distinct cell of the wave, ranks by ballot order.
-// Every lane of the wave calls it; inactive lanes bin nothing.
-__device__ inline void md_bin_wave(const MdBins& b, int buf, bool active, int v,
+// Atom v sits in slot s.  Every lane of the wave calls it; inactive lanes bin
+// nothing.
+__device__ inline void md_bin_wave(const MdBins& b, int buf, bool active, int s, int v,
                                    const double p[3], const Box& box, int3 dims, double3 cell)
 {
     int c = -1;
     if (active) {
         double w[3];
         c = wrap_cell(p, box, dims, cell, w);
-        b.cell[v] = c;
-        b.xw[v] = w[0];
-        b.yw[v] = w[1];
-        b.zw[v] = w[2];
+        b.cell[s] = c;
+        b.vid[s] = v;
+        b.xw[s] = w[0];
+        b.yw[s] = w[1];
+        b.zw[s] = w[2];
     }
     const int lane = (int)__lane_id();
     uint64_t pending = __ballot(active);
@@ -119,24 +121,25 @@ __device__ inline void md_bin_wave(const MdBins& b, int buf, bool active, int v,
         base = __shfl(base, leader, 64);
         if ((grp >> lane) & 1ull) {
             const int rank = base + (int)mbcnt64(grp);
-            if (rank < b.cap) b.bucket[((size_t)buf * b.ncell + cl) * b.cap + rank] = v;
+            if (rank < b.cap) b.bucket[((size_t)buf * b.ncell + cl) * b.cap + rank] = s;
         }
         pending &= ~grp;
     }
 }
 
-__device__ inline void md_bin_atom(const MdBins& b, int buf, int v, const double p[3],
+__device__ inline void md_bin_atom(const MdBins& b, int buf, int s, int v, const double p[3],
                                    const Box& box, int3 dims, double3 cell)
 {
     double w[3];
     const int c = wrap_cell(p, box, dims, cell, w);
-    b.cell[v] = c;
-    b.xw[v] = w[0];
-    b.yw[v] = w[1];
-    b.zw[v] = w[2];
-    const int slot = buf * b.ncell + c;
-    const int rank = atomicAdd(&b.count[slot], 1);
-    if (rank < b.cap) b.bucket[(size_t)slot * b.cap + rank] = v;
+    b.cell[s] = c;
+    b.vid[s] = v;
+    b.xw[s] = w[0];
+    b.yw[s] = w[1];
+    b.zw[s] = w[2];
+    const int bc = buf * b.ncell + c;
+    const int rank = atomicAdd(&b.count[bc], 1);
+    if (rank < b.cap) b.bucket[(size_t)bc * b.cap + rank] = s;
 }
 
 // ---------------------------------------------------------------- tile order
