@@ -438,7 +438,7 @@ struct TilePass {
             if (staged) {
                 for (int u = threadIdx.x; u < total; u += kTileThreads)
                     rs.put(u, stage_ref(h, A.xs_ref, u));
-                const int pad = (total + 127) & ~127;  // the sweep reads whole chunks
+                const int pad = ref_stage_len(total);  // the sweep reads whole chunks
                 for (int u = total + (int)threadIdx.x; u < pad; u += kTileThreads)
                     rs.put(u, h.dummy);
             }

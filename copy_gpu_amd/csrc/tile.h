@@ -280,10 +280,15 @@ __device__ inline double4 stage_ref(const TileHdr& h, const double4* __restrict_
 }
 
 // Reference-position stage of list_wave: (x, y) of candidate u at xy[u], z at
-// zz[u], for u below the stage size rounded up to the sweep's 128 candidates
+// zz[u], for u below the stage size rounded up to the sweep's kSweepLen candidates
 // (entries past the tile hold its dummy position, so the sweep needs no bounds
 // test).  The per-wave hit buffers follow (hit_buffer).
-__host__ __device__ inline int ref_stage_len(int stage_cap) { return (stage_cap + 127) & ~127; }
+constexpr int kSweep = 4;  // 64-candidate chunks per sweep iteration (independent chains)
+constexpr int kSweepLen = 64 * kSweep;
+__host__ __device__ inline int ref_stage_len(int stage_cap)
+{
+    return (stage_cap + kSweepLen - 1) / kSweepLen * kSweepLen;
+}
 
 struct RefStage {
     double2* xy;
@@ -343,10 +348,10 @@ __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
             pi = make_double3(p.x, p.y, p.z);
         }
         int n = 0;
-        for (int ub = 0; ub < total; ub += 128) {
-            double3 pj[2];
+        for (int ub = 0; ub < total; ub += kSweepLen) {
+            double3 pj[kSweep];
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < kSweep; ++k) {
                 const int u = ub + 64 * k + lane;
                 const int uu = STAGED || u < total ? u : 0;
                 if (STAGED) {
@@ -357,7 +362,7 @@ __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < kSweep; ++k) {
                 const int u = ub + 64 * k + lane;
                 const double dx = pj[k].x - pi.x;
                 const double dy = pj[k].y - pi.y;
