@@ -43,13 +43,20 @@ __device__ inline void sincos2pi<double>(double phi, double* s, double* c)
     sincospi(2.0 * f, s, c);
 }
 
+// Fused multiply-add in T, written out: the k-space sums run with implicit
+// contraction off (so kernels that share them give the same bits), and these
+// are the FMAs they use on purpose.
+__device__ inline float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return fma(a, b, c); }
+
 // Kahan accumulator in T (used for float; a no-op compensation in double).
+// add_prod(a, b) adds a*b, the product and the compensation in one FMA.
 template <typename T>
 struct Kahan {
     T s = 0, c = 0;
-    __device__ inline void add(T v)
+    __device__ inline void add_prod(T a, T b)
     {
-        const T y = v - c;
+        const T y = fmaf(a, b, -c);
         const T t = s + y;
         c = (t - s) - y;
         s = t;
@@ -60,7 +67,7 @@ struct Kahan {
 template <>
 struct Kahan<double> {
     double s = 0;
-    __device__ inline void add(double v) { s += v; }
+    __device__ inline void add_prod(double a, double b) { s = fma(a, b, s); }
     __device__ inline double value() const { return s; }
 };
 
@@ -130,8 +137,8 @@ struct RhoPartial {
                     T sn, cs;
                     sincos2pi<T>(phi, &sn, &cs);
                     const T qa = (T)p.w;
-                    re.add(qa * cs);
-                    im.add(qa * sn);
+                    re.add_prod(qa, cs);
+                    im.add_prod(qa, sn);
                 }
             }
         }
@@ -302,10 +309,10 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
             const double phi = fma(c.m[2], sz, fma(c.m[1], sy, c.m[0] * sx));
             T sn, cs;
             sincos2pi<T>(phi, &sn, &cs);
-            const T g = cs * c.A - sn * c.B;
-            ax += g * c.k[0];
-            ay += g * c.k[1];
-            az += g * c.k[2];
+            const T g = fma_t(cs, c.A, -(sn * c.B));
+            ax = fma_t(g, c.k[0], ax);
+            ay = fma_t(g, c.k[1], ay);
+            az = fma_t(g, c.k[2], az);
         }
     }
     if (P > 0) {
