@@ -626,8 +626,10 @@ def main():
             "list_entries": E,
             "valu_issue_frac": valu_frac,
             "valu_issue_note": "SQ_INSTS_VALU per launch (profiles/pair_<workload>_traffic.txt) "
-                               "/ (mean launch time x CUs x 2.4 GHz x 1 wave-instruction per "
-                               "CU-cycle); the pass is bound by VALU issue and latency",
+                               "/ (mean launch time x CUs x 2.4 GHz x 1 wave64 instruction per "
+                               "CU-cycle, the FP64 and unpacked FP32 rate); some instruction "
+                               "kinds issue faster, so this is an indicator, not a bound; the "
+                               "pass is bound by VALU issue, LDS gathers and latency",
         },
         "per_kernel_ms_per_step": per_step_ms,
         "dominant_kernel": dominant,
