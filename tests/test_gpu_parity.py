@@ -187,6 +187,28 @@ def test_overflowed_rows_use_stencil_fallback(orc, gpu):
     assert _rel(F2, Fo) < 1e-12
 
 
+@pytest.mark.parametrize("precision,tol", [(32, 1e-5), (64, 1e-12)])
+def test_rows_near_capacity_mix_fitted_and_overflowed(orc, gpu, precision, tol):
+    """Capacity 192 against rows of 187 +- 14 entries: in most waves some rows fit
+    (read up to their sentinel) and others overflowed (stencil scan), so the pair
+    rounds take their bounds-tested path.  Forces, energies and virial must not
+    change; the sentinel's dummy entry adds exactly zero."""
+    _log(f"case: list capacity 192 (some rows overflow), precision {precision}")
+    x, y, z, q, L = _ions(6000, seed=21)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, precision=precision, list_capacity=192)
+    f.Init(_dev(q))
+    F = _compute(f, x, y, z, ev=True)
+    st = f.stats()
+    assert 0 < st["overflow_rows"] < x.size, st
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fo, evo = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, 0.5)
+    assert _rel(F, Fo) < tol
+    ev = f.energy_virial()
+    assert abs(ev["e_lj"] - evo[0]) < 10 * tol * abs(evo[0])
+    assert abs(ev["e_real"] - evo[1]) < 10 * tol * abs(evo[1])
+
+
 @pytest.mark.parametrize("stage", [64, 1216])
 def test_tiles_beyond_lds_stage_read_global_memory(orc, gpu, stage):
     """Tiles with more candidates than the LDS stage take the global-memory
