@@ -112,10 +112,19 @@ def cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, target_s):
     el = time.perf_counter() - t
     if rc != 0:
         log("cpu baseline: list slots overflowed (reference fixed-stride list)")
+    # the same path on one thread (the reference's serial VTK-m device), a shorter sample
+    steps1 = int(max(1, min(2000, target_s / 3.0 / max(t1 * threads, 1e-6))))
+    m1 = klist(steps1, 1 + steps)
+    t = time.perf_counter()
+    oracle.faithful_run(pos, vel, frc, q, lo, Lb, wl["rc_lj"], wl["rc_coul"], wl["alpha"], P,
+                        m1, steps1, wl["dt"], 1)
+    el1 = time.perf_counter() - t
     return {"value": n * steps / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "single_thread_value": n * steps1 / el1, "host_cpus": os.cpu_count(),
             "sample": f"{steps} reference-faithful FP32 steps (per-step cell sort, list at rc "
                       f"without skin, N*P structure-factor arrays) of the same {n}-ion "
-                      f"workload, OpenMP over atoms, {el:.1f} s"}
+                      f"workload, OpenMP over atoms, {el:.1f} s; then {steps1} steps on one "
+                      f"thread, {el1:.1f} s"}
 
 
 def make_force(wl, L, P, kspace="RBE", precision=32, tol=1e-14):
