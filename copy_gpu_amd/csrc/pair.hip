@@ -7,9 +7,10 @@
 //   virial convention        rbmd-gpu src/force/src/ljforce_op/rocm/ljforce_op.hip.cu:555-575
 //
 // Mapping: one workgroup per cell tile (home cell + 27-cell stencil staged in
-// LDS at the home cell's periodic image), one wave64 per home atom; the lanes
-// walk the atom's uint16 Verlet row 64 entries at a time (LJ-range entries come
-// first, so only the first iterations evaluate LJ), then a wave reduction.
+// LDS at the home cell's periodic image), kGroup lanes per home atom (four atoms
+// per wave64); the lanes walk the atom's uint16 Verlet row kGroup x kRound
+// entries per round (LJ-range entries come first, so only the first rounds
+// evaluate LJ), then a DPP reduction within the lane group.
 // Pair displacements are FP64 differences of the staged positions (no per-pair
 // minimum image), the force maths runs in T (float or double), and cutoff
 // decisions use the FP64 squared distance.

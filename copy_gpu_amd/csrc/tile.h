@@ -250,7 +250,7 @@ __device__ inline int tile_seg(const TileHdr& h, int u)
 // One workgroup per (home cell, part): the tile's reference positions are
 // staged in LDS already moved to the home cell's image, x_j - s (s the image
 // shift of j's stencil cell), then one wave per home atom i sweeps the whole
-// tile 64 candidates at a time.  A candidate is kept iff
+// tile, kSweep chunks of 64 candidates per iteration.  A candidate is kept iff
 // (r_list^2 - d2 > 1e-4) && (d2 > 1e-4) with dx = (x_j - s) - x_i and
 // d2 = fma(dz,dz, fma(dy,dy, dx*dx)) -- the reference's operation order
 // (RunWorklet.cpp:77-83) and the exact bits the CPU oracle uses.  The stage is
@@ -309,7 +309,8 @@ struct RefStage {
     }
 };
 
-// LDS bytes the list build needs beside the reference stage: cap uint16 per wave.
+// LDS bytes of the list build: the reference stage plus one hit buffer of cap
+// uint16 per wave.
 inline size_t list_lds_bytes(int stage_cap, int cap)
 {
     return 24 * (size_t)ref_stage_len(stage_cap) +
