@@ -198,6 +198,47 @@ def kernel_times(f, st, wl, Kt, step):
     return per_step_ms, pair_bytes, pair_ms, E, step + Kt
 
 
+def pair_pass_at_scale(wname, reps=20):
+    """Pair-pass roofline of another workload at fixed positions (its starting
+    configuration): list built once, then reps force calls timed with HIP events."""
+    import torch
+
+    from copy_gpu_amd import systems
+
+    wl = WORKLOADS[wname]
+    n = wl["n"]
+    x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED)
+    t64 = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to("cuda:0", torch.float64)  # noqa
+    X, Y, Z, Q = t64(x), t64(y), t64(z), t64(q)
+    F = [torch.zeros(n, device="cuda:0", dtype=torch.float32) for _ in range(3)]
+    f = make_force(wl, L, wl["P"])
+    f.Init(Q)
+    f.sample_kvectors(wl["P"], SEED, 0)
+    f.Execute(X, Y, Z, *F)
+    f.Execute(X, Y, Z, *F)
+    torch.cuda.synchronize()
+    f.set_timing(True)
+    for _ in range(reps):
+        f.Execute(X, Y, Z, *F)
+    torch.cuda.synchronize()
+    pair_ms, cnt = f.timing("pair")
+    f.set_timing(False)
+    E = f.stats()["entries"]
+    f.close()
+    nbytes = 4.0 * E + 40.0 * n
+    achieved = nbytes / (pair_ms * 1e-3) / 1e9
+    valu = read_traffic(f"pair_{wname}", "valu_wave_instructions_per_launch")
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    out = {"workload": wl["desc"], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"pair_{wname}"),
+           "algorithmic_bytes_per_launch": nbytes, "mean_launch_ms": pair_ms,
+           "launches": cnt, "list_entries": E,
+           "valu_issue_frac": valu / (pair_ms * 1e-3 * n_cu * ENGINE_CLOCK_HZ) if valu else None}
+    log(f"pair pass at {wname}: {pair_ms * 1e3:.1f} us, {achieved:.0f} GB/s "
+        f"({100 * achieved / HBM_PEAK_GBS:.1f}% of HBM peak)")
+    return out
+
+
 def ewald_refs(wl, L, Q, configs):
     """FP64 full-Ewald total forces and the Ewald k-space part for each configuration."""
     import torch
@@ -471,6 +512,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ewald", action="store_true")
+    ap.add_argument("--skip-scale", action="store_true",
+                    help="with the default workload, skip the pair-pass roofline of the "
+                         "1,048,576-ion workload (c3) reported beside it")
     ap.add_argument("--tune", action="store_true",
                     help="run the library's tile-shape autotuner on the starting configuration "
                          "before the timed steps")
@@ -646,6 +690,9 @@ def main():
     }
     if learned:
         out["learned_vs_plain"] = learned
+    if args.workload == "c2" and not args.skip_scale:
+        # the pair pass at scale, where launch latency no longer dominates it
+        out["roofline_c3_pair_pass"] = pair_pass_at_scale("c3")
     print(json.dumps(out), flush=True)
 
 
