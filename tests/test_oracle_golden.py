@@ -31,7 +31,7 @@ def test_gnear_matches_reference_table(orc):
     # below r = 3 the FP32 cancellation is negligible: plain 6-digit agreement
     near = r < 3.0
     assert np.max(np.abs(g[near] - rows[near, 1]) / rows[near, 1]) < 6e-6
-    # spot value quoted in the survey of the reference tables
+    # spot value: the reference table's row at r = 2.0 reads 0.212367
     assert abs(orc.gnear(0.1, 2.0) - 0.212367) < 1e-6
 
 
