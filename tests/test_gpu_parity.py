@@ -458,6 +458,9 @@ def test_errors_and_empty(gpu):
     _compute(g, np.zeros(0), np.zeros(0), np.zeros(0))  # empty system is a no-op
     with pytest.raises(RBMDError, match="n exceeds"):
         g.Init(_dev(np.zeros(101)))
+    # rows so long that the list build's per-wave hit buffers cannot fit the LDS
+    with pytest.raises(RBMDError, match="do not fit the LDS"):
+        _make(np.array([20.0] * 3), 100, list_capacity=16384)
 
 
 def test_million_ions_invariants(gpu):
