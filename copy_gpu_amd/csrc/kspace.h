@@ -159,9 +159,10 @@ struct RhoPartial {
 };
 
 // ---------------------------------------------------------------- rho final
-// One workgroup of NT threads per k: FP64 sums over the atom-tile partials
-// (strided, then DPP wave sums, then the waves in order: a fixed order), then
-// the force coefficients and the energy / virial terms of this k.
+// One wave per k (NT / 64 k's per workgroup): FP64 sums over the atom-tile
+// partials (each lane strided over the partials, then a DPP wave sum: a fixed
+// order), then the force coefficients and the energy / virial terms of this k.
+// Few workgroups, so beside the tile pass they fit the first dispatch round.
 template <typename T, int NT>
 struct RhoFinalT {
     static constexpr int kThreads = NT;
@@ -177,29 +178,21 @@ struct RhoFinalT {
         double* kev;
         unsigned long long* step_dev;
     };
+    static constexpr int kPerBlock = NT / 64;  // one k per wave
     __device__ static void run(const Args& A, int blk, int)
     {
-        __shared__ double wr[NT / 64][2];
-        const int k = blk, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        const int lane = threadIdx.x & 63;
+        const int k = blk * kPerBlock + (int)(threadIdx.x >> 6);
         const int P = A.P;
-        double r = 0.0, m = 0.0;
-        for (int b = t; b < A.nbx; b += NT) {
-            r += A.kpart[((size_t)b * P + k) * 2];
-            m += A.kpart[((size_t)b * P + k) * 2 + 1];
+        if (k >= P) return;
+        double re = 0.0, im = 0.0;
+        for (int b = lane; b < A.nbx; b += 64) {
+            re += A.kpart[((size_t)b * P + k) * 2];
+            im += A.kpart[((size_t)b * P + k) * 2 + 1];
         }
-        r = wave_sum_dpp(r);
-        m = wave_sum_dpp(m);
+        re = wave_sum_dpp(re);
+        im = wave_sum_dpp(im);
         if (lane == 0) {
-            wr[wv][0] = r;
-            wr[wv][1] = m;
-        }
-        __syncthreads();
-        if (t == 0) {
-            double re = 0.0, im = 0.0;
-            for (int w = 0; w < NT / 64; ++w) {
-                re += wr[w][0];
-                im += wr[w][1];
-            }
             const Box& box = A.box;
             A.rho[2 * k] = re;
             A.rho[2 * k + 1] = im;
@@ -236,6 +229,13 @@ struct RhoFinalT {
 
 template <typename T>
 using RhoFinal = RhoFinalT<T, 256>;
+
+// Workgroups of a rho(k) final reduction of P k-vectors.
+template <typename T, int NT>
+inline int rho_final_blocks(int P)
+{
+    return (P + RhoFinalT<T, NT>::kPerBlock - 1) / RhoFinalT<T, NT>::kPerBlock;
+}
 
 // ---------------------------------------------------------------- k-space force
 // Workgroup = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes

@@ -515,7 +515,7 @@ struct TilePass {
 };
 
 // MD step: the tile pass and the rho(k) final reduction (nfin workgroups, one
-// per k) in one grid.  Both need only the previous kernel's output
+// k per wave) in one grid.  Both need only the previous kernel's output
 // (cell-sorted positions, rho partials), and the k-space force that follows
 // needs both.  With the load-balanced tile order the tiles come first, so that
 // workgroup t runs on CU t mod ncu as that order assumes, and the short
@@ -629,7 +629,7 @@ typename TileRho<T, false, M>::Args tile_rho_args(Engine& e, void* fx, void* fy,
     const bool ks = has_kspace(e);
     if (ks) a.fin = rho_final_args<T, kTileThreads>(e, step_dev);
     else memset(&a.fin, 0, sizeof a.fin);
-    a.nfin = ks ? e.P : 0;
+    a.nfin = ks ? rho_final_blocks<T, kTileThreads>(e.P) : 0;
     a.ntile = e.ncell * e.split;
     a.fin_first = tile_order_ncu(e.ncell, e.split, e.num_cu) == 0;
     a.tile = tile_args<T, false, M>(e, fx, fy, fz, true);
