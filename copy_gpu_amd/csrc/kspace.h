@@ -159,26 +159,64 @@ struct RhoPartial {
 };
 
 // ---------------------------------------------------------------- rho final
-// One wave per k (NT / 64 k's per workgroup): FP64 sums over the atom-tile
-// partials (each lane strided over the partials, then a DPP wave sum: a fixed
-// order), then the force coefficients and the energy / virial terms of this k.
-// Few workgroups, so beside the tile pass they fit the first dispatch round.
+// FP64 sums of each k's atom-tile partials in a fixed order, then its force
+// coefficients and energy / virial terms (rho_final_emit).
+template <typename T>
+struct RhoFinalArgs {
+    const double* kpart;
+    int nbx, P;
+    const int32_t* km;
+    const double* kw;
+    Box box;
+    double alpha, qqr2e;
+    double* rho;
+    KCoef<T>* coef;
+    double* kev;
+    unsigned long long* step_dev;
+};
+
+template <typename T>
+__device__ inline void rho_final_emit(const RhoFinalArgs<T>& A, int k, double re, double im)
+{
+    const Box& box = A.box;
+    A.rho[2 * k] = re;
+    A.rho[2 * k + 1] = im;
+    const double mm[3] = {(double)A.km[3 * k], (double)A.km[3 * k + 1], (double)A.km[3 * k + 2]};
+    double kv[3];
+    for (int d = 0; d < 3; ++d) kv[d] = 2.0 * kPi * mm[d] / box.L[d];
+    const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    const double V = box.L[0] * box.L[1] * box.L[2];
+    const double w = A.kw[k];
+    const double c = w * (-4.0 * kPi) / (V * k2) * A.qqr2e;
+    KCoef<T> kc;
+    for (int d = 0; d < 3; ++d) {
+        kc.m[d] = mm[d];
+        kc.k[d] = (T)kv[d];
+    }
+    kc.A = (T)(c * im);
+    kc.B = (T)(c * re);
+    A.coef[k] = kc;
+    const double e = w * (2.0 * kPi / V) * (re * re + im * im) / k2 * A.qqr2e;
+    const double b = 2.0 * (1.0 / k2 + 1.0 / (4.0 * A.alpha));
+    double* o = A.kev + (size_t)k * 7;
+    o[0] = e;
+    o[1] = e * (1.0 - b * kv[0] * kv[0]);
+    o[2] = e * (1.0 - b * kv[1] * kv[1]);
+    o[3] = e * (1.0 - b * kv[2] * kv[2]);
+    o[4] = -e * b * kv[0] * kv[1];
+    o[5] = -e * b * kv[0] * kv[2];
+    o[6] = -e * b * kv[1] * kv[2];
+    if (k == 0 && A.step_dev) *A.step_dev += 1ull;
+}
+
+// Beside the tile pass (TileRho): one wave per k, NT / 64 k's per workgroup
+// (each lane strided over the partials, then a DPP wave sum), so that few
+// workgroups share the tile pass's first dispatch round.
 template <typename T, int NT>
 struct RhoFinalT {
     static constexpr int kThreads = NT;
-    struct Args {
-        const double* kpart;
-        int nbx, P;
-        const int32_t* km;
-        const double* kw;
-        Box box;
-        double alpha, qqr2e;
-        double* rho;
-        KCoef<T>* coef;
-        double* kev;
-        unsigned long long* step_dev;
-    };
-    static constexpr int kPerBlock = NT / 64;  // one k per wave
+    using Args = RhoFinalArgs<T>;
+    static constexpr int kPerBlock = NT / 64;
     __device__ static void run(const Args& A, int blk, int)
     {
         const int lane = threadIdx.x & 63;
@@ -192,50 +230,51 @@ struct RhoFinalT {
         }
         re = wave_sum_dpp(re);
         im = wave_sum_dpp(im);
-        if (lane == 0) {
-            const Box& box = A.box;
-            A.rho[2 * k] = re;
-            A.rho[2 * k + 1] = im;
-            const double mm[3] = {(double)A.km[3 * k], (double)A.km[3 * k + 1],
-                                  (double)A.km[3 * k + 2]};
-            double kv[3];
-            for (int d = 0; d < 3; ++d) kv[d] = 2.0 * kPi * mm[d] / box.L[d];
-            const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
-            const double V = box.L[0] * box.L[1] * box.L[2];
-            const double w = A.kw[k];
-            const double c = w * (-4.0 * kPi) / (V * k2) * A.qqr2e;
-            KCoef<T> kc;
-            for (int d = 0; d < 3; ++d) {
-                kc.m[d] = mm[d];
-                kc.k[d] = (T)kv[d];
-            }
-            kc.A = (T)(c * im);
-            kc.B = (T)(c * re);
-            A.coef[k] = kc;
-            const double e = w * (2.0 * kPi / V) * (re * re + im * im) / k2 * A.qqr2e;
-            const double b = 2.0 * (1.0 / k2 + 1.0 / (4.0 * A.alpha));
-            double* o = A.kev + (size_t)k * 7;
-            o[0] = e;
-            o[1] = e * (1.0 - b * kv[0] * kv[0]);
-            o[2] = e * (1.0 - b * kv[1] * kv[1]);
-            o[3] = e * (1.0 - b * kv[2] * kv[2]);
-            o[4] = -e * b * kv[0] * kv[1];
-            o[5] = -e * b * kv[0] * kv[2];
-            o[6] = -e * b * kv[1] * kv[2];
-            if (k == 0 && A.step_dev) *A.step_dev += 1ull;
-        }
+        if (lane == 0) rho_final_emit(A, k, re, im);
     }
 };
 
-template <typename T>
-using RhoFinal = RhoFinalT<T, 256>;
-
-// Workgroups of a rho(k) final reduction of P k-vectors.
+// Workgroups of RhoFinalT over P k-vectors.
 template <typename T, int NT>
 inline int rho_final_blocks(int P)
 {
     return (P + RhoFinalT<T, NT>::kPerBlock - 1) / RhoFinalT<T, NT>::kPerBlock;
 }
+
+// Standalone launch (the force pass, where the reduction's latency is exposed):
+// one workgroup of 256 threads per k (strided sums, DPP wave sums, then the four
+// waves in order), a shorter chain per lane than one wave per k.
+template <typename T>
+struct RhoFinal {
+    static constexpr int kThreads = 256;
+    using Args = RhoFinalArgs<T>;
+    __device__ static void run(const Args& A, int blk, int)
+    {
+        __shared__ double wr[kThreads / 64][2];
+        const int k = blk, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        const int P = A.P;
+        double r = 0.0, m = 0.0;
+        for (int b = t; b < A.nbx; b += kThreads) {
+            r += A.kpart[((size_t)b * P + k) * 2];
+            m += A.kpart[((size_t)b * P + k) * 2 + 1];
+        }
+        r = wave_sum_dpp(r);
+        m = wave_sum_dpp(m);
+        if (lane == 0) {
+            wr[wv][0] = r;
+            wr[wv][1] = m;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double re = 0.0, im = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) {
+                re += wr[w][0];
+                im += wr[w][1];
+            }
+            rho_final_emit(A, k, re, im);
+        }
+    }
+};
 
 // ---------------------------------------------------------------- k-space force
 // Workgroup = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes
@@ -391,14 +430,14 @@ inline typename RhoPartial<T>::Args rho_args(Engine& e, const double* x, const d
                                         e.kpart};
 }
 
-template <typename T, int NT>
-inline typename RhoFinalT<T, NT>::Args rho_final_args(Engine& e, unsigned long long* step_dev)
+template <typename T>
+inline RhoFinalArgs<T> rho_final_args(Engine& e, unsigned long long* step_dev)
 {
     int apb, nbx, nkt;
     rho_geometry(e, &apb, &nbx, &nkt);
-    return typename RhoFinalT<T, NT>::Args{e.kpart, nbx,     e.P,   e.km,
-                                           e.kw,    e.box,   e.alpha, e.qqr2e,
-                                           e.rho,   (KCoef<T>*)e.kcoef, e.kev, step_dev};
+    return RhoFinalArgs<T>{e.kpart, nbx,     e.P,   e.km,
+                           e.kw,    e.box,   e.alpha, e.qqr2e,
+                           e.rho,   (KCoef<T>*)e.kcoef, e.kev, step_dev};
 }
 
 template <typename T>

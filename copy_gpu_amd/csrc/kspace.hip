@@ -126,11 +126,11 @@ void launch_rho(Engine& e, const double* x, const double* y, const double* z, hi
     {
         ScopedTimer t(e, P_RHO_FINAL, s);
         if (e.precision == 32)
-            k_one<RhoFinal<float>><<<rho_final_blocks<float, 256>(e.P), 256, 0, s>>>(
-                rho_final_args<float, 256>(e, step_dev));
+            k_one<RhoFinal<float>><<<e.P, RhoFinal<float>::kThreads, 0, s>>>(
+                rho_final_args<float>(e, step_dev));
         else
-            k_one<RhoFinal<double>><<<rho_final_blocks<double, 256>(e.P), 256, 0, s>>>(
-                rho_final_args<double, 256>(e, step_dev));
+            k_one<RhoFinal<double>><<<e.P, RhoFinal<double>::kThreads, 0, s>>>(
+                rho_final_args<double>(e, step_dev));
     }
 }
 

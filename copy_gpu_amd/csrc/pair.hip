@@ -627,7 +627,7 @@ typename TileRho<T, false, M>::Args tile_rho_args(Engine& e, void* fx, void* fy,
 {
     typename TileRho<T, false, M>::Args a;
     const bool ks = has_kspace(e);
-    if (ks) a.fin = rho_final_args<T, kTileThreads>(e, step_dev);
+    if (ks) a.fin = rho_final_args<T>(e, step_dev);
     else memset(&a.fin, 0, sizeof a.fin);
     a.nfin = ks ? rho_final_blocks<T, kTileThreads>(e.P) : 0;
     a.ntile = e.ncell * e.split;
