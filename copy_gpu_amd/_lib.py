@@ -61,6 +61,28 @@ class EnergyVirial(C.Structure):
     ]
 
 
+THERMO_NONE = 0
+THERMO_BERENDSEN = 1
+THERMO_RESCALE = 2
+THERMO_NOSE_HOOVER = 3
+THERMO_LANGEVIN = 4
+
+
+class Thermostat(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("t_target", C.c_double),
+        ("t_damp", C.c_double),
+        ("dof", C.c_double),
+        ("kb", C.c_double),
+        ("xi", C.c_double),
+        ("gamma", C.c_double),
+        ("kbt", C.c_double),
+        ("seed", C.c_uint64),
+        ("noise_step", C.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); the symbol test checks this table against the header
 _P = C.c_void_p
 class ReplicaState(C.Structure):
@@ -81,6 +103,11 @@ _SIGS = {
     "rbmd_force_get_energy_virial": (C.c_int, [_P, C.POINTER(EnergyVirial), _P]),
     "rbmd_md_run": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               C.c_double, C.c_int64, C.c_uint64, C.c_uint64, C.c_int32, _P]),
+    "rbmd_md_set_thermostat": (C.c_int, [_P, C.POINTER(Thermostat)]),
+    "rbmd_md_get_thermostat": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_uint64)]),
+    "rbmd_md_set_temperature_log": (C.c_int, [_P, _P, C.c_int64]),
+    "rbmd_langevin_noise_host": (C.c_int, [C.c_uint64, C.c_uint64, _P]),
     "rbmd_force_get_neighbors": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "rbmd_force_get_kspace": (C.c_int, [_P, C.POINTER(C.c_int32), _P, _P, _P, _P]),
     "rbmd_force_get_stats": (C.c_int, [_P, _P]),

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/rbmd_force.h"
+#include "kspace.h"
 #include "proposal.h"
 #include "state.h"
 #include "tile.h"
@@ -109,7 +110,8 @@ static void free_all(Engine& E)
                     E.md_count, E.md_bucket, E.md_cell, E.md_vid, E.md_w,
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
-                    E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide};
+                    E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide, E.thermo_part,
+                    E.thermo_st};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (E.h_stats) hipHostFree(E.h_stats);
@@ -381,6 +383,13 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.type, N * 4));
     CR(dalloc(&E.d_pp, sizeof(PairParams)));
     CR(dalloc(&E.d_step, 64));
+    // thermostat: one partial per closing-kick workgroup, then scale / xi / T,
+    // the arrival and log counters and the noise step in one 64-byte block
+    E.thermo_part_cap = std::max<int64_t>(((int64_t)N + kKfAtoms - 1) / kKfAtoms, 1);
+    CR(dalloc(&E.thermo_part, (size_t)E.thermo_part_cap * 8));
+    CR(dalloc(&E.thermo_st, 64));
+    E.thermo_cnt = (uint32_t*)(E.thermo_st + 4);
+    E.thermo_noise = (unsigned long long*)(E.thermo_st + 6);
     CR(dalloc(&E.ev, 16 * 8));
     E.ev_part_blocks = E.ncell * 4;  // tile workgroups (split <= 4)
     CR(dalloc(&E.ev_part, (size_t)E.ev_part_blocks * 8 * 8));
@@ -390,6 +399,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(hipMemset(E.flags, 0, 64));
     CR(hipMemset(E.stats, 0, 128));
     CR(hipMemset(E.d_step, 0, 64));
+    CR(hipMemset(E.thermo_st, 0, 64));
     CR(hipMemset(E.ev, 0, 16 * 8));
     CR(hipMemset(E.cell_start, 0, ((size_t)E.ncell + 1) * 4));
     CR(hipMemset(E.cell_count, 0, (size_t)E.ncell * 4));
@@ -637,6 +647,18 @@ static void md_segment(Engine& E, const RepState& st, double dt, uint64_t seed,
         const uint64_t v = *step_host + (uint64_t)k;
         return sample_args(E, E.P, seed, &v);
     };
+    if (E.thermo_type != kThermoNone) {
+        // a thermostat needs the kinetic temperature of every atom between the
+        // closing kick of a step and the opening kick of the next: each step is a
+        // segment of its own (opening kick, structure factor, tile pass, closing kick)
+        for (int k = 0; k < steps; ++k) {
+            launch_md_open(E, st, dt, sa_of(k), s);
+            launch_md_sort_rho(E, st, s);
+            launch_tile_rho(E, st.fx, st.fy, st.fz, s);
+            launch_md_close(E, st, dt, s);
+        }
+        return;
+    }
     launch_md_open(E, st, dt, sa_of(0), s);
     for (int k = 0; k < steps; ++k) {
         launch_md_sort_rho(E, st, s);
@@ -672,6 +694,11 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     // bin the current positions for a rebuild in the first step (later steps'
     // bins come from the tile pass of the step before)
     launch_md_bin(E, x, y, z, s);
+    if (E.thermo_type != kThermoNone) {
+        if (!(thermo_dof(E) > 0))
+            return fail(E, RBMD_ERR_STATE, "thermostat: no degrees of freedom (dof <= 0)");
+        launch_thermo_begin(E, s);
+    }
     RepState rs;
     rs.x = x; rs.y = y; rs.z = z;
     rs.vx = vx; rs.vy = vy; rs.vz = vz;
@@ -698,6 +725,8 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         key.push_back((const void*)(uintptr_t)seed);
         // the proposal's table pointers, lambda and P-dependent weights are baked too
         key.push_back((const void*)(uintptr_t)E.gp_gen);
+        // and the thermostat's constants
+        key.push_back((const void*)(uintptr_t)E.thermo_gen);
         // and the launch shapes the autotuner may change
         for (int64_t v : {(int64_t)E.split, (int64_t)E.stage_cap, (int64_t)E.ncell,
                           (int64_t)E.rho_apb, (int64_t)E.kpart_cap})
@@ -733,6 +762,7 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         for (int64_t k = 0; k < nsteps % kGraphSteps; ++k)
             HIP_TRY(hipGraphLaunch(E.graphs[1].exec, s));
     }
+    if (E.thermo_type != kThermoNone) launch_thermo_end(E, rs, s);
     HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     uint32_t fl[2];
@@ -819,6 +849,68 @@ int rbmd_force_get_kspace(rbmd_force_t h, int32_t* P, int32_t* m, double* w, dou
             if (im) im[k] = r[2 * (size_t)k + 1];
         }
     }
+    return RBMD_OK;
+}
+
+int rbmd_md_set_thermostat(rbmd_force_t h, const rbmd_thermostat* t)
+{
+    if (!h || !t) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (t->type < RBMD_THERMO_NONE || t->type > RBMD_THERMO_LANGEVIN)
+        return fail(E, RBMD_ERR_INVALID, "thermostat: unknown type");
+    if (t->type != RBMD_THERMO_NONE) {
+        if (!(t->t_target > 0) && t->type != RBMD_THERMO_LANGEVIN)
+            return fail(E, RBMD_ERR_INVALID, "thermostat: t_target must be > 0");
+        if (t->type == RBMD_THERMO_BERENDSEN && !(t->t_damp > 0))
+            return fail(E, RBMD_ERR_INVALID, "thermostat: BERENDSEN needs t_damp > 0");
+        if (!(t->dof >= 0) || !(t->kb >= 0))
+            return fail(E, RBMD_ERR_INVALID, "thermostat: dof and kb must be >= 0");
+        if (t->type == RBMD_THERMO_LANGEVIN && (!(t->gamma >= 0) || !(t->kbt >= 0)))
+            return fail(E, RBMD_ERR_INVALID, "thermostat: LANGEVIN needs gamma, kbt >= 0");
+    }
+    E.thermo_type = t->type;
+    E.thermo_t0 = t->t_target;
+    E.thermo_tdamp = t->t_damp;
+    E.thermo_dof = t->dof;
+    E.thermo_kb = t->kb > 0 ? t->kb : 1.0;
+    E.thermo_gamma = t->gamma;
+    E.thermo_kbt = t->kbt;
+    E.thermo_seed = t->seed;
+    ++E.thermo_gen;
+    // device state: scale 1, xi, T 0, counters 0, noise step
+    double st[8] = {1.0, t->xi, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    memcpy(&st[6], &t->noise_step, 8);
+    HIP_TRY(hipMemcpy(E.thermo_st, st, sizeof st, hipMemcpyHostToDevice));
+    return RBMD_OK;
+}
+
+int rbmd_md_get_thermostat(rbmd_force_t h, double* temperature, double* xi, uint64_t* noise_step)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    double st[8];
+    HIP_TRY(hipMemcpy(st, E.thermo_st, sizeof st, hipMemcpyDeviceToHost));
+    if (temperature) *temperature = st[2];
+    if (xi) *xi = st[1];
+    if (noise_step) memcpy(noise_step, &st[6], 8);
+    return RBMD_OK;
+}
+
+int rbmd_md_set_temperature_log(rbmd_force_t h, double* log, int64_t len)
+{
+    if (!h) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (len < 0 || (len > 0 && !log)) return fail(E, RBMD_ERR_INVALID, "temperature log: bad length");
+    E.thermo_log = len > 0 ? log : nullptr;
+    E.thermo_log_len = len;
+    ++E.thermo_gen;
+    return RBMD_OK;
+}
+
+int rbmd_langevin_noise_host(uint64_t seed, uint64_t step, double* g)
+{
+    if (!g) return RBMD_ERR_INVALID;
+    langevin_gaussian(seed, step, g);
     return RBMD_OK;
 }
 

@@ -15,6 +15,7 @@
 //   force coefficients, per-k energy/virial terms
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include <string>
@@ -191,6 +192,58 @@ RBMD_HD inline double draw_kvector(uint64_t seed, uint64_t step, uint32_t l, con
         return w_out;
     const int idx = (m_out[0] + g.Mg) + side * ((m_out[1] + g.Mg) + side * (m_out[2] + g.Mg));
     return g.ratio[idx] / (double)P;
+}
+
+// ---------------------------------------------------------------- NVT thermostats
+// Velocity thermostats of ExecutionNVT::UpdateVelocityByTempConType and
+// TempConTypeForce (rbmd/framework/run/ExecutionNVT.cpp:256-291, 735-747).
+enum : int {
+    kThermoNone = 0,
+    kThermoBerendsen = 1,
+    kThermoRescale = 2,
+    kThermoNoseHoover = 3,
+    kThermoLangevin = 4,
+};
+
+// Constants and device state of a thermostat, by value in the closing-kick kernel.
+// Each workgroup of that kernel stores sum m |v|^2 of its atoms after the closing
+// kick in part[blk] and counts itself in cnt[0]; the last to arrive sums the
+// partials in a fixed order, forms T and the thermostat's coefficient, logs T, and
+// clears cnt[0] for the next step.
+struct ThermoArgs {
+    int type = kThermoNone;
+    int nblk = 0;
+    double* part = nullptr;             // [nblk] per-workgroup sum of m |v|^2
+    uint32_t* cnt = nullptr;            // [0] workgroups arrived, [1] temperature log position
+    double* st = nullptr;               // [0] velocity scale for the next opening kick,
+                                        // [1] Nose-Hoover xi, [2] temperature of the last step
+    unsigned long long* noise = nullptr;  // Langevin noise step
+    double t0 = 0, dt_over_damp = 0, inv_dof_kb = 0, gamma = 0, kbt = 0;
+    uint64_t seed = 0;
+    double* log = nullptr;  // temperature per step (device), or null
+    int64_t log_len = 0;
+};
+
+// Gaussian 3-vector of the Langevin force at a noise step (one vector per step for
+// every atom, as the reference draws it): Philox counters (0x4C414E47, j, step_lo,
+// step_hi) for j = 0, 1, then Box-Muller on (1 - u, u') pairs so the logarithm's
+// argument lies in (0, 1].
+RBMD_HD inline void langevin_gaussian(uint64_t seed, uint64_t step, double g[3])
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t r0[4], r1[4];
+    const uint32_t c0[4] = {0x4C414E47u, 0u, (uint32_t)step, (uint32_t)(step >> 32)};
+    const uint32_t c1[4] = {0x4C414E47u, 1u, (uint32_t)step, (uint32_t)(step >> 32)};
+    philox4x32_10(c0, key, r0);
+    philox4x32_10(c1, key, r1);
+    const double two_pi = 6.283185307179586;
+    const double a = sqrt(-2.0 * log(1.0 - u53(r0[0], r0[1])));
+    const double t = two_pi * u53(r0[2], r0[3]);
+    const double b = sqrt(-2.0 * log(1.0 - u53(r1[0], r1[1])));
+    const double t2 = two_pi * u53(r1[2], r1[3]);
+    g[0] = a * cos(t);
+    g[1] = a * sin(t);
+    g[2] = b * cos(t2);
 }
 
 #ifdef __HIPCC__

@@ -298,6 +298,7 @@ struct KForceArgs {
     const double* mass;
     double dt;
     uint32_t* flags;
+    ThermoArgs th;  // velocity thermostat of the closing kick (KForce<T, true>)
 };
 
 template <typename T>
@@ -376,13 +377,59 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
     }
 }
 
-// Velocity and mass of atom i for the closing half kick.
+// Velocity and mass of atom i for the closing half kick, and the thermostat's
+// per-step inputs (Nose-Hoover friction, Langevin noise vector).
 struct KickIn {
     double v[3], m;
 };
 
+// Thermostat step after the closing kick, in wave 0 of each workgroup (which holds
+// the workgroup's atoms); ke = m |v|^2 of the lane's atom after the closing kick
+// (0 for lanes without one).  The last workgroup to arrive forms
+// T = sum / (dof kB) with the partials summed in a fixed order (the same bits
+// whichever workgroup is last), then the coefficient of the thermostat
+// (ExecutionNVT::UpdateVelocityByTempConType, rbmd/framework/run/ExecutionNVT.cpp:256-291):
+// the velocity scale of Berendsen (sqrt(1 + dt/Tdamp (T0/T - 1))) or of the plain
+// rescale (sqrt(T0/T)), applied by the next opening kick, or the Nose-Hoover
+// friction update xi += dt/2 (T/T0 - 1) / (dt/10).
+__device__ inline void thermo_close(const ThermoArgs& th, int blk, double ke, double dt)
+{
+    const int lane = threadIdx.x & 63;
+    const double s = wave_sum(ke);
+    uint32_t prev = 0;
+    if (lane == 0) {
+        th.part[blk] = s;
+        prev = __hip_atomic_fetch_add(th.cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    prev = __shfl(prev, 0, 64);
+    if (prev != (uint32_t)(th.nblk - 1)) return;
+    double acc = 0;
+    for (int j = lane; j < th.nblk; j += 64)
+        acc += __hip_atomic_load(th.part + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = wave_sum(acc);
+    if (lane != 0) return;
+    const double T = acc * th.inv_dof_kb;
+    double scale = 1.0, xi = th.st[1];
+    if (th.type == kThermoBerendsen) scale = sqrt(1.0 + th.dt_over_damp * (th.t0 / T - 1.0));
+    else if (th.type == kThermoRescale) scale = sqrt(th.t0 / T);
+    else if (th.type == kThermoNoseHoover) xi = xi + 0.5 * dt * (T / th.t0 - 1.0) / (0.1 * dt);
+    th.st[0] = scale;
+    th.st[1] = xi;
+    th.st[2] = T;
+    const uint32_t pos = th.cnt[1];
+    if (th.log && (int64_t)pos < th.log_len) th.log[pos] = T;
+    th.cnt[1] = pos + 1u;
+    if (th.type == kThermoLangevin) th.noise[0] += 1ull;
+    th.cnt[0] = 0u;
+}
+
 // F += k-space force; with KICK the closing half kick v += dt/2 F/m of the
-// velocity-Verlet step follows (and the MD step epoch advances).
+// velocity-Verlet step follows (and the MD step epoch advances), with the
+// thermostat of A.th: the Langevin force F += -m gamma v + sqrt(2 m gamma kBT/dt) g
+// before the kick (UnderdampedLangevinWorklet, rbmd/framework/run/worklet/RunWorklet.cpp:1700-1728),
+// the Nose-Hoover half step v += dt/2 (F/m - xi v) after it
+// (UpdateVelocityNoseHooverWorklet, RunWorklet.cpp:1784-1804), and the kinetic
+// temperature of the kicked velocities (thermo_close).
 template <typename T, bool KICK>
 struct KForce {
     static constexpr int kThreads = kKfThreads;
@@ -392,6 +439,8 @@ struct KForce {
 #pragma clang fp contract(off)
         if (KICK && blk == 0 && threadIdx.x == 0) A.flags[3] += 1u;  // MD step epoch
         __shared__ KForceLds<T> L;
+        const ThermoArgs& th = A.th;
+        double ke = 0;
         kforce_block(
             A, blk, L,
             [&](int i) {
@@ -406,16 +455,39 @@ struct KForce {
             },
             [&](int i, T f0, T f1, T f2, const KickIn& k) {
 #pragma clang fp contract(off)
+                if (KICK && th.type == kThermoLangevin) {
+                    double g[3];
+                    langevin_gaussian(th.seed, th.noise[0], g);
+                    const double mg = k.m * th.gamma;
+                    const double amp = sqrt(2.0 * k.m * th.gamma * th.kbt / A.dt);
+                    f0 = (T)((double)f0 - mg * k.v[0] + amp * g[0]);
+                    f1 = (T)((double)f1 - mg * k.v[1] + amp * g[1]);
+                    f2 = (T)((double)f2 - mg * k.v[2] + amp * g[2]);
+                }
                 A.fx[i] = f0;
                 A.fy[i] = f1;
                 A.fz[i] = f2;
                 if (KICK) {
                     const double h = 0.5 * A.dt / k.m;
-                    A.vx[i] = k.v[0] + h * (double)f0;
-                    A.vy[i] = k.v[1] + h * (double)f1;
-                    A.vz[i] = k.v[2] + h * (double)f2;
+                    double v0 = k.v[0] + h * (double)f0;
+                    double v1 = k.v[1] + h * (double)f1;
+                    double v2 = k.v[2] + h * (double)f2;
+                    if (th.type != kThermoNone) {
+                        ke = k.m * (v0 * v0 + v1 * v1 + v2 * v2);
+                        if (th.type == kThermoNoseHoover) {
+                            const double xi = th.st[1], hd = 0.5 * A.dt;
+                            v0 = v0 + hd * ((double)f0 / k.m - xi * v0);
+                            v1 = v1 + hd * ((double)f1 / k.m - xi * v1);
+                            v2 = v2 + hd * ((double)f2 / k.m - xi * v2);
+                        }
+                    }
+                    A.vx[i] = v0;
+                    A.vy[i] = v1;
+                    A.vz[i] = v2;
                 }
             });
+        if (KICK && th.type != kThermoNone && (threadIdx.x >> 6) == 0)
+            thermo_close(th, blk, ke, A.dt);
     }
 };
 

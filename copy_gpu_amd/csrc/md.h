@@ -78,6 +78,7 @@ struct KickDriftArgs {
     const int32_t* inv_perm;
     double4* xs;
     SampleArgs sa;
+    const double* vscale = nullptr;  // thermostat velocity scale (v = c v before the kick)
 };
 
 // Per-atom inputs of the kick/drift besides velocity and force.
@@ -165,8 +166,53 @@ struct KickDrift {
         }
         const int i = (blk - ns) * kMdBlock + threadIdx.x;
         if (i >= a.n) return;
+        if (a.vscale) {
+            // Berendsen / rescale coefficient of the step before
+            // (UpdateVelocityRescaleWorklet, rbmd/framework/run/worklet/RunWorklet.cpp:1818-1830)
+            const double c = *a.vscale;
+            kick_drift_atom(a, i, kd_load(a, i), c * a.vx[i], c * a.vy[i], c * a.vz[i], a.fx[i],
+                            a.fy[i], a.fz[i]);
+            return;
+        }
         kick_drift_atom(a, i, kd_load(a, i), a.vx[i], a.vy[i], a.vz[i], a.fx[i], a.fy[i],
                         a.fz[i]);
+    }
+};
+
+// Start of a thermostatted run: scale 1 for the first opening kick, no arrivals,
+// temperature log from position 0.  One thread.
+struct ThermoBegin {
+    static constexpr int kThreads = 64;
+    struct Args {
+        double* st;
+        uint32_t* cnt;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        if (blk != 0 || threadIdx.x != 0) return;
+        a.st[0] = 1.0;
+        a.cnt[0] = 0u;
+        a.cnt[1] = 0u;
+    }
+};
+
+// End of a thermostatted run: v = c v with the scale the last step's closing kick
+// formed (the opening kick of a following step would otherwise apply it).
+struct VScale {
+    static constexpr int kThreads = kMdBlock;
+    struct Args {
+        double *vx, *vy, *vz;
+        const double* scale;
+        int n;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int i = blk * kMdBlock + threadIdx.x;
+        if (i >= a.n) return;
+        const double c = *a.scale;
+        a.vx[i] = c * a.vx[i];
+        a.vy[i] = c * a.vy[i];
+        a.vz[i] = c * a.vz[i];
     }
 };
 

@@ -13,15 +13,41 @@ namespace {
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// Thermostat constants and device state for the closing kick (type none when the
+// handle has no thermostat).
+ThermoArgs thermo_args(const Engine& e, double dt)
+{
+    ThermoArgs t;
+    if (e.thermo_type == kThermoNone) return t;
+    t.type = e.thermo_type;
+    t.nblk = std::max(kforce_blocks(e), 1);
+    t.part = e.thermo_part;
+    t.cnt = e.thermo_cnt;
+    t.st = e.thermo_st;
+    t.noise = e.thermo_noise;
+    t.t0 = e.thermo_t0;
+    t.dt_over_damp = e.thermo_tdamp > 0 ? dt / e.thermo_tdamp : 0.0;
+    t.inv_dof_kb = 1.0 / (thermo_dof(e) * e.thermo_kb);
+    t.gamma = e.thermo_gamma;
+    t.kbt = e.thermo_kbt;
+    t.seed = e.thermo_seed;
+    t.log = e.thermo_log;
+    t.log_len = e.thermo_log_len;
+    return t;
+}
+
 template <typename T>
 KickDriftArgs<T> kickdrift_args(Engine& e, const RepState& st, double dt, const SampleArgs& sa)
 {
     const double h = 0.5 * e.skin;
-    return KickDriftArgs<T>{st.x,   st.y,   st.z,   st.vx,         st.vy,
-                            st.vz,  (const T*)st.fx, (const T*)st.fy, (const T*)st.fz,
-                            st.img, st.mass, dt,    (int)e.n,      e.box,
-                            e.xr,   e.yr,   e.zr,   h * h,         e.flags,
-                            e.q,    e.atom_rank /* inverse perm */, e.xs, sa};
+    KickDriftArgs<T> a{st.x,   st.y,   st.z,   st.vx,         st.vy,
+                       st.vz,  (const T*)st.fx, (const T*)st.fy, (const T*)st.fz,
+                       st.img, st.mass, dt,    (int)e.n,      e.box,
+                       e.xr,   e.yr,   e.zr,   h * h,         e.flags,
+                       e.q,    e.atom_rank /* inverse perm */, e.xs, sa};
+    if (e.thermo_type == kThermoBerendsen || e.thermo_type == kThermoRescale)
+        a.vscale = e.thermo_st;
+    return a;
 }
 
 CellSortMDArgs cellsort_md_args(Engine& e, const RepState& st)
@@ -54,8 +80,10 @@ typename SortRho<T>::Args sortrho_args(Engine& e, const RepState& st, int* block
 template <typename T>
 KForceArgs<T> close_args(Engine& e, const RepState& st, double dt)
 {
-    return kforce_args<T>(e, st.x, st.y, st.z, st.fx, st.fy, st.fz, st.vx, st.vy, st.vz,
-                          st.mass, dt, has_kspace(e));
+    KForceArgs<T> a = kforce_args<T>(e, st.x, st.y, st.z, st.fx, st.fy, st.fz, st.vx, st.vy,
+                                     st.vz, st.mass, dt, has_kspace(e));
+    a.th = thermo_args(e, dt);
+    return a;
 }
 
 template <typename T>
@@ -166,6 +194,19 @@ void launch_md_close_open(Engine& e, const RepState& st, double dt, const Sample
     ScopedTimer t(e, P_KFORCE, s);
     if (e.precision == 32) md_close_open_t<float>(e, st, dt, sa, s);
     else md_close_open_t<double>(e, st, dt, sa, s);
+}
+
+void launch_thermo_begin(Engine& e, hipStream_t s)
+{
+    k_one<ThermoBegin><<<1, ThermoBegin::kThreads, 0, s>>>(
+        ThermoBegin::Args{e.thermo_st, e.thermo_cnt});
+}
+
+void launch_thermo_end(Engine& e, const RepState& st, hipStream_t s)
+{
+    if (e.thermo_type != kThermoBerendsen && e.thermo_type != kThermoRescale) return;
+    k_one<VScale><<<std::max(ceil_div(e.n, kMdBlock), 1), kMdBlock, 0, s>>>(
+        VScale::Args{st.vx, st.vy, st.vz, e.thermo_st, (int)e.n});
 }
 
 hipError_t md_batch_prepare(Batch& b)

@@ -125,6 +125,20 @@ struct Engine {
     int ev_part_blocks = 0;
     double* ev = nullptr;       // [10]
 
+    // NVT thermostat of rbmd_md_run (rbmd_md_set_thermostat)
+    int thermo_type = 0;
+    double thermo_t0 = 0, thermo_tdamp = 0, thermo_dof = 0, thermo_kb = 1;
+    double thermo_gamma = 0, thermo_kbt = 0;
+    uint64_t thermo_seed = 0;
+    double* thermo_part = nullptr;    // [thermo_part_cap] closing-kick workgroup partials
+    int64_t thermo_part_cap = 0;
+    double* thermo_st = nullptr;      // [4] scale, xi, last T, noise step (as counter below)
+    uint32_t* thermo_cnt = nullptr;   // [2] arrivals, log position
+    unsigned long long* thermo_noise = nullptr;  // [1] Langevin noise step
+    double* thermo_log = nullptr;     // caller's device array of T per step, or null
+    int64_t thermo_log_len = 0;
+    uint64_t thermo_gen = 0;          // bumped by every thermostat change (keys the MD graph)
+
     // host pinned mirror of stats
     unsigned long long* h_stats = nullptr;
 
@@ -210,6 +224,13 @@ inline bool has_kspace(const Engine& e)
     return (e.kspace == RBMD_KSPACE_RBE || e.kspace == RBMD_KSPACE_EWALD) && e.P > 0;
 }
 
+// Degrees of freedom of the kinetic temperature: the thermostat's own, or 3n - 3
+// (ExecutionNVT::ComputeTempe, rbmd/framework/run/ExecutionNVT.cpp:772-775).
+inline double thermo_dof(const Engine& e)
+{
+    return e.thermo_dof > 0 ? e.thermo_dof : 3.0 * (double)e.n - 3.0;
+}
+
 inline MdBins md_bins(const Engine& e)
 {
     const size_t n = (size_t)std::max<int64_t>(e.n_max, 1);
@@ -245,6 +266,10 @@ void launch_md_open(Engine& e, const RepState& st, double dt, const SampleArgs& 
                     hipStream_t s);
 void launch_md_sort_rho(Engine& e, const RepState& st, hipStream_t s);
 void launch_md_close(Engine& e, const RepState& st, double dt, hipStream_t s);
+// thermostatted runs: state reset before the first step, last velocity scale after
+// the last (no kernel without a rescaling thermostat)
+void launch_thermo_begin(Engine& e, hipStream_t s);
+void launch_thermo_end(Engine& e, const RepState& st, hipStream_t s);
 void launch_md_close_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
                           hipStream_t s);
 

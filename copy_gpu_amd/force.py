@@ -235,6 +235,36 @@ class LJCutCoulKspaceForce:
         return ms.value, cnt.value
 
     # ------------------------------------------------------------ velocity Verlet
+    _THERMO = {"NONE": L.THERMO_NONE, "BERENDSEN": L.THERMO_BERENDSEN,
+               "TEMP_RESCALE": L.THERMO_RESCALE, "NOSE_HOOVER": L.THERMO_NOSE_HOOVER,
+               "LANGEVIN": L.THERMO_LANGEVIN}
+
+    def set_thermostat(self, temp_ctrl_type="NONE", temperature=(1.0, 1.0, 0.5), dof=0.0,
+                       kb=1.0, xi=0.0, gamma=100.0, kbt=1.0, seed=0, noise_step=0):
+        """NVT thermostat of md_run, keyed as the reference JSON ("temp_ctrl_type",
+        "temperature" = [T_target, T_end, Tdamp]; ExecutionNVT::InitParameters,
+        run/ExecutionNVT.cpp:840-841).  dof 0 means 3n - 3."""
+        t = L.Thermostat(self._THERMO[temp_ctrl_type], float(temperature[0]),
+                         float(temperature[2]), float(dof), float(kb), float(xi), float(gamma),
+                         float(kbt), int(seed), int(noise_step))
+        self._check(self.lib.rbmd_md_set_thermostat(self.h, C.byref(t)), "set_thermostat")
+
+    def thermostat_state(self) -> dict:
+        """Temperature of the last step (before its rescale), Nose-Hoover xi, next noise step."""
+        T, xi, ns = C.c_double(), C.c_double(), C.c_uint64()
+        self._check(self.lib.rbmd_md_get_thermostat(self.h, C.byref(T), C.byref(xi),
+                                                    C.byref(ns)), "get_thermostat")
+        return {"T": T.value, "xi": xi.value, "noise_step": ns.value}
+
+    def set_temperature_log(self, log: torch.Tensor | None):
+        """FP64 device tensor receiving the temperature of each step of later runs."""
+        if log is not None:
+            assert log.dtype == torch.float64 and log.is_cuda and log.is_contiguous()
+        self._temp_log = log  # keep the memory alive while the handle may write it
+        n = 0 if log is None else log.numel()
+        self._check(self.lib.rbmd_md_set_temperature_log(self.h, _ptr(log), n),
+                    "set_temperature_log")
+
     def md_run(self, x, y, z, vx, vy, vz, fx, fy, fz, dt, nsteps, seed=0, step0=0, img=None,
                mass=None, use_graph=True, stream=None):
         """nsteps of kick-drift-force-kick (ExecutionNVT::Solve order) updating all arrays."""

@@ -29,6 +29,13 @@ def philox4x32(ctr, key):
     return out
 
 
+def langevin_noise(seed: int, step: int):
+    """The Langevin thermostat's Gaussian 3-vector of a noise step (host draw)."""
+    g = np.zeros(3)
+    L.lib().rbmd_langevin_noise_host(int(seed), int(step), _p(g))
+    return g
+
+
 def fcc(nc: int, rho: float):
     n = 4 * nc ** 3
     x, y, z = (np.zeros(n) for _ in range(3))

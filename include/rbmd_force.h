@@ -132,6 +132,48 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
                 double dt, int64_t nsteps, uint64_t seed, uint64_t step0, int32_t use_graph,
                 void* stream);
 
+/* NVT velocity thermostats of rbmd_md_run ("temp_ctrl_type"), in LJ units
+ * (fmt2v = mvv2e = 1).  Replaces ExecutionNVT::ComputeTempe,
+ * UpdateVelocityByTempConType and TempConTypeForce (run/ExecutionNVT.cpp:749-787,
+ * 256-291, 735-747) with rbmd-gpu's temperature controllers
+ * (src/simulate_pipeline/src/controller/temperature_controller/temperature_controller_op/rocm/update_temperature_op.hip.cu:9-95).
+ * After each step's closing kick the kinetic temperature T = sum m v^2 / (dof kB)
+ * is formed on the device; then
+ *   BERENDSEN    v *= sqrt(1 + dt/t_damp (t_target/T - 1))
+ *   TEMP_RESCALE v *= sqrt(t_target/T)
+ *   NOSE_HOOVER  v += dt/2 (F/m - xi v), then xi += dt/2 (T/t_target - 1) / (dt/10)
+ *   LANGEVIN     F += -m gamma v + sqrt(2 m gamma kbt / dt) g before the closing
+ *                kick, g one Gaussian 3-vector per step shared by all atoms (noise
+ *                stream (seed, noise_step), advanced once per step).
+ * A thermostatted run launches four kernels per step instead of three. */
+#define RBMD_THERMO_NONE 0
+#define RBMD_THERMO_BERENDSEN 1
+#define RBMD_THERMO_RESCALE 2
+#define RBMD_THERMO_NOSE_HOOVER 3
+#define RBMD_THERMO_LANGEVIN 4
+typedef struct {
+    int32_t type;        /* RBMD_THERMO_*                                          */
+    double t_target;     /* "temperature"[0]                                       */
+    double t_damp;       /* "temperature"[2] (BERENDSEN)                           */
+    double dof;          /* degrees of freedom; 0 = 3n - 3 (3n for the in-built lattice) */
+    double kb;           /* Boltzmann constant; 0 = 1                              */
+    double xi;           /* NOSE_HOOVER friction at the start (the reference: 0)  */
+    double gamma;        /* LANGEVIN friction (the reference: 100)                 */
+    double kbt;          /* LANGEVIN kB T (the reference: 1)                       */
+    uint64_t seed;       /* LANGEVIN noise stream                                  */
+    uint64_t noise_step; /* LANGEVIN noise step of the next step                   */
+} rbmd_thermostat;
+int rbmd_md_set_thermostat(rbmd_force_t h, const rbmd_thermostat* t);
+/* Temperature of the last step (after its closing kick, before the rescale), the
+ * Nose-Hoover friction and the next Langevin noise step.  Synchronising. */
+int rbmd_md_get_thermostat(rbmd_force_t h, double* temperature, double* xi,
+                           uint64_t* noise_step);
+/* Device array receiving T of step j of each following thermostatted run
+ * (j < len); log NULL or len 0 stops logging. */
+int rbmd_md_set_temperature_log(rbmd_force_t h, double* log, int64_t len);
+/* The Langevin noise vector g (3 doubles) of a noise step, on the host. */
+int rbmd_langevin_noise_host(uint64_t seed, uint64_t step, double* g);
+
 /* Current full Verlet list in ORIGINAL atom ids as CSR (HOST arrays):
  * offsets[n+1], idx[cap]; rows unsorted.  Returns the entry count in *count
  * (if larger than cap only the offsets are written).  Synchronising.

@@ -48,6 +48,13 @@ def lib():
             "orc_drift": (None, [i64, P, P, P, P, P, P, d, P, P, P]),
             "orc_faithful_run": (C.c_int, [i64, P, P, P, P, P, P, d, d, d, i32, P, i32, d, i32]),
             "orc_num_threads": (C.c_int, []),
+            "orc_temperature": (d, [i64, P, P, P, P, d, d, d]),
+            "orc_scale_velocities": (None, [i64, P, P, P, d]),
+            "orc_rescale_coeff": (d, [d, d]),
+            "orc_berendsen_coeff": (d, [d, d, d, d]),
+            "orc_nose_hoover_kick": (None, [i64, P, P, P, P, P, P, P, d, d, d]),
+            "orc_nose_hoover_xi": (d, [d, d, d, d]),
+            "orc_langevin_force": (None, [i64, P, P, P, P, P, P, P, d, d, d, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(_lib, k)
@@ -173,6 +180,52 @@ def drift(xyz, v, dt, lo, Lb, img=None):
     lib().orc_drift(p[0].size, _p(p[0]), _p(p[1]), _p(p[2]), _p(v[0]), _p(v[1]), _p(v[2]),
                     float(dt), _p(_d(lo)), _p(_d(Lb)), _p(im))
     return p, im
+
+
+def temperature(v, mass=None, dof=None, kB=1.0, mvv2e=1.0):
+    """Kinetic temperature; dof defaults to 3n - 3 (read-in systems without SHAKE)."""
+    v = [_d(a) for a in v]
+    n = v[0].size
+    dof = 3.0 * n - 3.0 if dof is None else float(dof)
+    return lib().orc_temperature(n, _p(v[0]), _p(v[1]), _p(v[2]),
+                                 _p(None if mass is None else _d(mass)), float(mvv2e), dof,
+                                 float(kB))
+
+
+def scale_velocities(v, c):
+    v = [_d(a).copy() for a in v]
+    lib().orc_scale_velocities(v[0].size, _p(v[0]), _p(v[1]), _p(v[2]), float(c))
+    return v
+
+
+def rescale_coeff(T, T0):
+    return lib().orc_rescale_coeff(float(T), float(T0))
+
+
+def berendsen_coeff(T, T0, dt, tdamp):
+    return lib().orc_berendsen_coeff(float(T), float(T0), float(dt), float(tdamp))
+
+
+def nose_hoover_kick(v, f, dt, xi, mass=None, fmt2v=1.0):
+    v = [_d(a).copy() for a in v]
+    f = [_d(a) for a in f]
+    lib().orc_nose_hoover_kick(v[0].size, _p(v[0]), _p(v[1]), _p(v[2]), _p(f[0]), _p(f[1]),
+                               _p(f[2]), _p(None if mass is None else _d(mass)), float(dt),
+                               float(fmt2v), float(xi))
+    return v
+
+
+def nose_hoover_xi(xi, T, T0, dt):
+    return lib().orc_nose_hoover_xi(float(xi), float(T), float(T0), float(dt))
+
+
+def langevin_force(f, v, g, gamma, kbt, dt, mass=None):
+    f = [_d(a).copy() for a in f]
+    v = [_d(a) for a in v]
+    lib().orc_langevin_force(f[0].size, _p(f[0]), _p(f[1]), _p(f[2]), _p(v[0]), _p(v[1]),
+                             _p(v[2]), _p(None if mass is None else _d(mass)), float(gamma),
+                             float(kbt), float(dt), _p(_d(g)))
+    return f
 
 
 def faithful_run(pos, vel, force, q, lo, Lb, rc_lj, rc_coul, alpha, P, m_steps, nsteps, dt,

@@ -510,6 +510,86 @@ void orc_drift(int64_t n, double* x, double* y, double* z, const double* vx, con
     orc_wrap(n, x, y, z, lo, L, img);
 }
 
+/* ------------------------------------------------------------------ */
+/* NVT velocity thermostats                                              */
+/* ------------------------------------------------------------------ */
+
+/* Kinetic temperature T = mvv2e * sum_i m_i |v_i|^2 / (dof * kB)
+ * (ComputerKineticEnergyWorklet, run/worklet/RunWorklet.cpp:1806-1816, reduced and
+ * divided in ExecutionNVT::ComputeTempe, run/ExecutionNVT.cpp:749-787: 0.5 * sum /
+ * (dof * kB / 2) with dof = 3n - 3 for read-in systems, 3n for the in-built lattice). */
+double orc_temperature(int64_t n, const double* vx, const double* vy, const double* vz,
+                       const double* mass, double mvv2e, double dof, double kB)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        double m = mass ? mass[i] : 1.0;
+        s += mvv2e * (m * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]));
+    }
+    return s / (dof * kB);
+}
+
+/* v *= c (UpdateVelocityRescaleWorklet, run/worklet/RunWorklet.cpp:1818-1830). */
+void orc_scale_velocities(int64_t n, double* vx, double* vy, double* vz, double c)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        vx[i] = c * vx[i];
+        vy[i] = c * vy[i];
+        vz[i] = c * vz[i];
+    }
+}
+
+/* Rescale coefficients of ExecutionNVT::UpdateVelocityByTempConType
+ * (run/ExecutionNVT.cpp:272-289): TEMP_RESCALE sqrt(T0 / T), BERENDSEN
+ * sqrt(1 + dt / Tdamp * (T0 / T - 1)). */
+double orc_rescale_coeff(double T, double T0) { return sqrt(T0 / T); }
+
+double orc_berendsen_coeff(double T, double T0, double dt, double tdamp)
+{
+    return sqrt(1.0 + dt / tdamp * (T0 / T - 1.0));
+}
+
+/* Nose-Hoover half step v += 1/2 dt (F/m - xi v) fmt2v
+ * (UpdateVelocityNoseHooverWorklet, run/worklet/RunWorklet.cpp:1784-1804), applied
+ * after the closing kick with the friction xi of the step before. */
+void orc_nose_hoover_kick(int64_t n, double* vx, double* vy, double* vz, const double* fx,
+                          const double* fy, const double* fz, const double* mass, double dt,
+                          double fmt2v, double xi)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double m = mass ? mass[i] : 1.0;
+        vx[i] = vx[i] + 0.5 * dt * (fx[i] / m - xi * vx[i]) * fmt2v;
+        vy[i] = vy[i] + 0.5 * dt * (fy[i] / m - xi * vy[i]) * fmt2v;
+        vz[i] = vz[i] + 0.5 * dt * (fz[i] / m - xi * vz[i]) * fmt2v;
+    }
+}
+
+/* Friction update xi += 1/2 dt (T / T0 - 1) / tauT with tauT = dt / 10
+ * (run/ExecutionNVT.cpp:269-270). */
+double orc_nose_hoover_xi(double xi, double T, double T0, double dt)
+{
+    double tauT = 0.1 * dt;
+    return xi + 0.5 * dt * (T / T0 - 1.0) / tauT;
+}
+
+/* Underdamped Langevin force F += -m gamma v + sqrt(2 m gamma kBT / dt) g with one
+ * Gaussian 3-vector g per step shared by every atom (UnderdampedLangevinWorklet,
+ * run/worklet/RunWorklet.cpp:1700-1728; g drawn in ExecutionNVT::ComputeAllForce,
+ * run/ExecutionNVT.cpp:347-352; applied after the force by TempConTypeForce,
+ * :735-747).  v is the velocity at the force evaluation (after the opening kick). */
+void orc_langevin_force(int64_t n, double* fx, double* fy, double* fz, const double* vx,
+                        const double* vy, const double* vz, const double* mass, double gamma,
+                        double kbt, double dt, const double* g)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double m = mass ? mass[i] : 1.0;
+        double amp = sqrt(2.0 * m * gamma * kbt / dt);
+        fx[i] = fx[i] - m * gamma * vx[i] + amp * g[0];
+        fy[i] = fy[i] - m * gamma * vy[i] + amp * g[1];
+        fz[i] = fz[i] - m * gamma * vz[i] + amp * g[2];
+    }
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

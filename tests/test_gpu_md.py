@@ -227,3 +227,144 @@ def test_md_lj_fcc_fp64_matches_oracle_and_conserves_energy(orc, gpu):
     e1 = energy(P1, V1)
     _log(f"  total energy {e0:.6f} -> {e1:.6f} after 1000 steps")
     assert abs(e1 - e0) < 5e-3 * abs(e0)
+
+
+def _oracle_thermo_run(orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt, kforces, mass,
+                       kind, T0, tdamp, gamma, kbt, noise):
+    """The reference's NVT step order (ExecutionNVT::Solve): kick, drift, force (+ the
+    Langevin term), kick, temperature, thermostat."""
+    lo = np.zeros(3)
+
+    def forces(p):
+        Fp, _ = orc.pair_forces(*p, q, lo, Lb, 2.5, 5.0, alpha)
+        return Fp + kforces(p)
+
+    p = list(orc.wrap(x, y, z, lo, Lb))
+    img = np.zeros(3 * x.size, np.int32)
+    v = [vx, vy, vz]
+    F = forces(p)
+    F = [F[:, 0], F[:, 1], F[:, 2]]
+    xi, Ts = 0.0, []
+    for s in range(steps):
+        v = orc.kick(v, F, dt, mass)
+        p, img = orc.drift(p, v, dt, lo, Lb, img)
+        F = forces(p)
+        F = [F[:, 0], F[:, 1], F[:, 2]]
+        if kind == "LANGEVIN":
+            F = orc.langevin_force(F, v, noise(s), gamma, kbt, dt, mass)
+        v = orc.kick(v, F, dt, mass)
+        T = orc.temperature(v, mass)
+        Ts.append(T)
+        if kind == "BERENDSEN":
+            v = orc.scale_velocities(v, orc.berendsen_coeff(T, T0, dt, tdamp))
+        elif kind == "TEMP_RESCALE":
+            v = orc.scale_velocities(v, orc.rescale_coeff(T, T0))
+        elif kind == "NOSE_HOOVER":
+            v = orc.nose_hoover_kick(v, F, dt, xi, mass)
+            xi = orc.nose_hoover_xi(xi, T, T0, dt)
+    return np.stack(p, 1), np.stack(v, 1), img, np.array(Ts), xi
+
+
+@pytest.mark.parametrize("kind", ["BERENDSEN", "TEMP_RESCALE", "NOSE_HOOVER", "LANGEVIN"])
+def test_md_thermostat_matches_oracle(orc, gpu, kind):
+    """FP64 Ewald NVT run of 30 steps (three 8-step graph replays and six single
+    steps) with per-atom masses and list rebuilds, against the oracle's loop in the
+    reference's order: positions, velocities, image flags, the temperature of every
+    step, and the Nose-Hoover friction."""
+    from copy_gpu_amd import systems
+
+    _log(f"case: NVT {kind}, FP64 Ewald, 1500 ions, 30 steps")
+    n, dt, steps, alpha, tol = 1500, 0.002, 30, 0.5, 1e-12
+    T0, tdamp, gamma, kbt, nseed, ns0 = 1.1, 0.05, 100.0, 1.0, 77, 5
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=61)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=61)
+    Lb = np.array([L] * 3)
+    mass = 1.0 + 0.25 * (np.arange(n) % 3)
+    f = _make(Lb, n, 64, "EWALD", skin=0.1, ewald_tol=tol, alpha=alpha)
+    f.Init(_dev(q))
+    X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+    F = [torch.zeros(n, device="cuda", dtype=f.fdtype) for _ in range(3)]
+    f.Execute(X, Y, Z, *F)
+    img = torch.zeros(3 * n, device="cuda", dtype=torch.int32)
+    log = torch.zeros(steps + 2, device="cuda", dtype=torch.float64)
+    f.set_thermostat(kind, (T0, T0, tdamp), gamma=gamma, kbt=kbt, seed=nseed, noise_step=ns0)
+    f.set_temperature_log(log)
+    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=steps, seed=SEED, step0=1, img=img,
+             mass=_dev(mass), use_graph=True)
+    torch.cuda.synchronize()
+    st = f.thermostat_state()
+    Pg = np.stack([t.cpu().numpy() for t in (X, Y, Z)], 1)
+    Vg = np.stack([t.cpu().numpy() for t in (VX, VY, VZ)], 1)
+    Tg = log.cpu().numpy()
+    m, w = _halfspace_lattice(Lb, alpha, tol)
+    Po, Vo, Io, To, xio = _oracle_thermo_run(
+        orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt,
+        lambda p: orc.kspace_forces(*p, q, Lb, alpha, m, w)[0], mass, kind, T0, tdamp, gamma,
+        kbt, lambda s: systems.langevin_noise(nseed, ns0 + s))
+    _log(f"  T first/last {To[0]:.4f}/{To[-1]:.4f}; max |dT|/T "
+         f"{np.abs(Tg[:steps] - To).max() / To.max():.2e}; xi {st['xi']:.6f} vs {xio:.6f}")
+    assert _dpos(Pg, Po, Lb) < 1e-9
+    assert np.abs(Vg - Vo).max() < 1e-8
+    assert np.array_equal(img.cpu().numpy(), Io)
+    assert np.allclose(Tg[:steps], To, rtol=1e-11, atol=0)
+    assert np.all(Tg[steps:] == 0)  # entries past the run's steps stay untouched
+    assert st["T"] == Tg[steps - 1]
+    if kind == "NOSE_HOOVER":
+        assert abs(st["xi"] - xio) <= 1e-9 * max(1.0, abs(xio))
+    if kind == "LANGEVIN":
+        assert st["noise_step"] == ns0 + steps
+    if kind == "TEMP_RESCALE":
+        # the velocities returned are rescaled to T0 exactly (up to rounding)
+        assert abs(orc.temperature([Vg[:, 0], Vg[:, 1], Vg[:, 2]], mass) - T0) < 1e-12
+
+
+def test_md_thermostat_graph_equals_eager_and_resets(gpu):
+    """A Berendsen run gives the same bits replayed from graphs and launched eagerly,
+    and switching the thermostat off restores the plain step."""
+    from copy_gpu_amd import systems
+
+    _log("case: NVT Berendsen graph vs eager, then thermostat off, FP32 RBE, 3000 ions")
+    n, dt, steps = 3000, 0.002, 20
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=62)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=62)
+    Lb = np.array([L] * 3)
+
+    def run(use_graph, thermo):
+        f = _make(Lb, n, 32, "RBE", P=100, skin=0.2)
+        f.Init(_dev(q))
+        f.sample_kvectors(100, SEED, 0)
+        X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+        F = [torch.zeros(n, device="cuda", dtype=f.fdtype) for _ in range(3)]
+        f.Execute(X, Y, Z, *F)
+        if thermo is not None:
+            f.set_thermostat(thermo, (1.3, 1.3, 0.1))
+        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=steps, seed=SEED, step0=1,
+                 use_graph=use_graph)
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in (X, Y, Z, VX, VY, VZ)], f
+
+    a, _ = run(True, "BERENDSEN")
+    b, _ = run(False, "BERENDSEN")
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+    c, _ = run(True, "NONE")
+    d, _ = run(True, None)
+    for u, v in zip(c, d):
+        assert np.array_equal(u, v)
+    # the thermostat acted: the velocities differ from the plain run's
+    assert not np.array_equal(a[3], c[3])
+
+
+def test_md_thermostat_rejects_bad_parameters(gpu):
+    from copy_gpu_amd import systems
+    from copy_gpu_amd.force import RBMDError
+
+    _log("case: thermostat parameter errors")
+    x, y, z, q, L = systems.random_ions(1500, 0.3, 0.8, seed=63)
+    f = _make(np.array([L] * 3), 1500, 32, "NONE")
+    with pytest.raises(RBMDError, match="t_damp"):
+        f.set_thermostat("BERENDSEN", (1.0, 1.0, 0.0))
+    with pytest.raises(RBMDError, match="t_target"):
+        f.set_thermostat("TEMP_RESCALE", (0.0, 0.0, 0.5))
+    with pytest.raises(RBMDError, match="dof"):
+        f.set_thermostat("TEMP_RESCALE", (1.0, 1.0, 0.5), dof=-1.0)

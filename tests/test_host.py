@@ -91,3 +91,33 @@ def test_fcc_generator():
     assert x.size == 4000
     assert abs(Lb - 10 * (4 / 0.8442) ** (1 / 3)) < 1e-12
     assert abs(4000 / Lb ** 3 - 0.8442) < 1e-12
+
+
+def test_langevin_noise_stream():
+    """The Langevin noise vector: Box-Muller on Philox counters (0x4C414E47, j, step_lo,
+    step_hi), checked against the same transform of the Philox words in numpy, and its
+    moments over many steps (zero mean, unit variance, uncorrelated components)."""
+    from copy_gpu_amd import systems
+
+    def u53(a, b):
+        return ((int(a) >> 5) * 67108864.0 + (int(b) >> 6)) / 9007199254740992.0
+
+    seed = 0x1234_5678_9ABC_DEF0
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    for step in (0, 1, 7, (5 << 32) + 3):
+        r0 = systems.philox4x32((0x4C414E47, 0, step & 0xFFFFFFFF, step >> 32), key)
+        r1 = systems.philox4x32((0x4C414E47, 1, step & 0xFFFFFFFF, step >> 32), key)
+        a = np.sqrt(-2.0 * np.log(1.0 - u53(r0[0], r0[1])))
+        t = 2 * np.pi * u53(r0[2], r0[3])
+        b = np.sqrt(-2.0 * np.log(1.0 - u53(r1[0], r1[1])))
+        t2 = 2 * np.pi * u53(r1[2], r1[3])
+        want = np.array([a * np.cos(t), a * np.sin(t), b * np.cos(t2)])
+        got = systems.langevin_noise(seed, step)
+        assert np.allclose(got, want, rtol=1e-14, atol=1e-15), (got, want)
+    G = np.array([systems.langevin_noise(11, s) for s in range(20000)])
+    assert np.abs(G.mean(0)).max() < 4.5 / np.sqrt(20000)
+    assert np.abs(G.var(0) - 1).max() < 0.05
+    C = np.corrcoef(G.T)
+    assert np.abs(C - np.eye(3)).max() < 0.04
+    # the stream depends on the seed
+    assert not np.array_equal(systems.langevin_noise(11, 3), systems.langevin_noise(12, 3))
