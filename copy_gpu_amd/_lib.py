@@ -83,6 +83,18 @@ class Thermostat(C.Structure):
     ]
 
 
+class DataInfo(C.Structure):
+    _fields_ = [
+        ("natoms", C.c_int64),
+        ("ntypes", C.c_int32),
+        ("lo", C.c_double * 3),
+        ("hi", C.c_double * 3),
+        ("has_masses", C.c_int32),
+        ("has_pair_coeffs", C.c_int32),
+        ("has_velocities", C.c_int32),
+    ]
+
+
 # name -> (restype, argtypes); the symbol test checks this table against the header
 _P = C.c_void_p
 class ReplicaState(C.Structure):
@@ -144,6 +156,10 @@ _SIGS = {
     "rbmd_sys_random": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_uint64, _P, _P, _P]),
     "rbmd_sys_charges": (C.c_int, [C.c_int64, C.c_uint64, _P]),
     "rbmd_sys_velocities": (C.c_int, [C.c_int64, C.c_double, _P, C.c_uint64, _P, _P, _P]),
+    "rbmd_read_data_info": (C.c_int, [C.c_char_p, C.POINTER(DataInfo)]),
+    "rbmd_read_data": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64] + [_P] * 12),
+    "rbmd_write_dump": (C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.c_int64] + [_P] * 6),
+    "rbmd_io_last_error": (C.c_char_p, []),
     "rbmd_sample_kvectors_host": (C.c_int, [C.c_double, _P, C.c_int32, C.c_uint64, C.c_uint64,
                                             _P, C.POINTER(C.c_double)]),
 }

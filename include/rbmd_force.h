@@ -347,6 +347,41 @@ int rbmd_sys_velocities(int64_t n, double T, const double* mass, uint64_t seed, 
 int rbmd_sample_kvectors_host(double alpha, const double* L, int32_t P, uint64_t seed,
                               uint64_t step, int32_t* m, double* S_out);
 
+/* ------------------------------------------------------------------ */
+/* Input and output formats (host, no GPU)                              */
+/* ------------------------------------------------------------------ */
+
+/* LAMMPS data files as the reference reads them with "read_data"
+ * (rbmd-gpu src/reader/src/atomic_reader.cpp:24-58, 135-230, 322-350).
+ * rbmd_read_data_info reads the header (atom count, types, box) and which of
+ * the Masses / Pair Coeffs / Velocities sections are present. */
+typedef struct {
+    int64_t natoms;
+    int32_t ntypes;
+    double lo[3], hi[3];
+    int32_t has_masses, has_pair_coeffs, has_velocities;
+} rbmd_data_info;
+int rbmd_read_data_info(const char* path, rbmd_data_info* info);
+/* Atoms lines of atom_style "atomic" (id type x y z), "charge" (id type q x y z)
+ * or "full" (id mol type q x y z), each with optional image flags: the atom with
+ * id k fills index k - 1 of type (0-based), q, x, y, z and img (3n, xyz
+ * interleaved); Velocities lines (id vx vy vz) fill vx, vy, vz; Masses and Pair
+ * Coeffs (type eps sigma) fill mass, eps, sigma per type.  HOST arrays; any but
+ * type/x/y/z may be NULL.  n must equal the file's atom count and every id in
+ * 1..n must appear once. */
+int rbmd_read_data(const char* path, const char* atom_style, int64_t n, int32_t* type,
+                   double* q, double* x, double* y, double* z, int32_t* img, double* vx,
+                   double* vy, double* vz, double* mass, double* eps, double* sigma);
+/* One trajectory frame in the reference's dump layout (TrajectoryOutput::Execute,
+ * rbmd-parallel/framework/output/TrajectoryOutput.cpp:21-44): ITEM: TIMESTEP,
+ * NUMBER OF ATOMS, BOX BOUNDS pp pp pp, ATOMS id type x y z (ids and types
+ * 1-based), positions to 17 significant digits.  append != 0 adds the frame. */
+int rbmd_write_dump(const char* path, int32_t append, int64_t step, int64_t n, const double* lo,
+                    const double* hi, const int32_t* type, const double* x, const double* y,
+                    const double* z);
+/* Message of the last failed I/O call of this thread. */
+const char* rbmd_io_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,13 +9,17 @@ committed outputs are data only, so the tests never read the reference:
                        rbmd/res/json_files/Gnear_table_leq2_2.txt (r in [0.1, 2)) and
                        Gnear_table_geq2_2.txt (r in [2, 11.97]); columns of the source:
                        index, r, step, G, dG/dr.
-  lj_salt_6000.npz     positions, charges, types and box of the reference example
-                       rbmd/res/json_files/LJ_salt/box_102030/rbmd.data (LAMMPS
-                       write_data, 6000 +/-1 LJ ions, box 10 x 20 x 30, atom style charge:
-                       id type q x y z ix iy iz), rows sorted by atom id.
+  lj_salt_6000.npz     positions, charges, types, image flags, velocities and box of
+                       the reference example rbmd/res/json_files/LJ_salt/box_102030/rbmd.data
+                       (LAMMPS write_data, 6000 +/-1 LJ ions, box 10 x 20 x 30, atom style
+                       charge: id type q x y z ix iy iz; Velocities: id vx vy vz), rows
+                       sorted by atom id.
+  lj_salt_6000.data.gz that example data file itself, gzip-compressed, as the input of
+                       the data-file reader's parity test.
 """
 from __future__ import annotations
 
+import gzip
 import os
 import sys
 
@@ -48,6 +52,7 @@ def lj_salt():
     lo = np.zeros(3)
     hi = np.zeros(3)
     atoms = []
+    vel = {}
     section = None
     with open(path) as f:
         for raw in f:
@@ -63,13 +68,20 @@ def lj_salt():
                 section = toks[0]
                 continue
             if section == "Atoms" and len(toks) >= 6:
+                im = [int(t) for t in toks[6:9]] if len(toks) >= 9 else [0, 0, 0]
                 atoms.append((int(toks[0]), int(toks[1]), float(toks[2]), float(toks[3]),
-                              float(toks[4]), float(toks[5])))
+                              float(toks[4]), float(toks[5]), *im))
+            if section == "Velocities" and len(toks) == 4:
+                vel[int(toks[0])] = (float(toks[1]), float(toks[2]), float(toks[3]))
     atoms.sort()
     a = np.array(atoms)
+    v = np.array([vel.get(int(i), (np.nan,) * 3) for i in a[:, 0]])
     np.savez_compressed(os.path.join(OUT, "lj_salt_6000.npz"), ids=a[:, 0].astype(np.int64),
                         types=a[:, 1].astype(np.int32), q=a[:, 2], x=a[:, 3], y=a[:, 4],
-                        z=a[:, 5], lo=lo, hi=hi)
+                        z=a[:, 5], img=a[:, 6:9].astype(np.int32), v=v, lo=lo, hi=hi)
+    with open(path, "rb") as src, open(os.path.join(OUT, "lj_salt_6000.data.gz"), "wb") as f, \
+            gzip.GzipFile(fileobj=f, mode="wb", mtime=0) as dst:
+        dst.write(src.read())
     return len(atoms)
 
 
