@@ -182,8 +182,9 @@ def nvt_leg(f, st, wl, W, K, step, use_graph):
     return {"thermostat": "BERENDSEN", "t_target": 1.0, "t_damp": 0.5,
             "value": n * K / el, "unit": "atom-steps/s", "ms_per_step": el * 1e3 / K,
             "steps": K, "list_builds": rebuilds, "temperature_last_step": T,
-            "note": "four kernels per step (the closing kick reduces the kinetic temperature "
-                    "and the next opening kick applies the rescale) instead of three"}
+            "note": "five kernels per step instead of three: opening kick (applying the "
+                    "rescale), structure factor, tile pass, closing kick (per-workgroup "
+                    "sums of m v^2) and a one-workgroup temperature reduction"}
 
 
 def kernel_times(f, st, wl, Kt, step):
@@ -536,6 +537,10 @@ def main():
                     help="with the default workload, skip the legs reported beside it: the "
                          "pair-pass roofline of the 1,048,576-ion workload (c3) and the NVT "
                          "(Berendsen) run")
+    ap.add_argument("--thermostat", default=None,
+                    choices=["BERENDSEN", "TEMP_RESCALE", "NOSE_HOOVER", "LANGEVIN"],
+                    help="time the MD steps under this NVT thermostat (temperature 1.0, "
+                         "Tdamp 0.5) instead of plain velocity Verlet")
     ap.add_argument("--tune", action="store_true",
                     help="run the library's tile-shape autotuner on the starting configuration "
                          "before the timed steps")
@@ -630,7 +635,10 @@ def main():
             t = time.perf_counter()
             shapes = f.autotune(st.X, st.Y, st.Z, *st.F)
             log(f"autotune ({time.perf_counter() - t:.2f} s): {shapes}")
-        log(f"warmup {W} steps, timed {K} steps (graph={use_graph})")
+        if args.thermostat:
+            f.set_thermostat(args.thermostat, (1.0, 1.0, 0.5))
+        log(f"warmup {W} steps, timed {K} steps (graph={use_graph}, "
+            f"thermostat={args.thermostat or 'none'})")
         el, rebuilds, step = timed(f, st, wl, W, K, 1, use_graph)
         value = n * K / el
         log(f"timed {K} steps: {el * 1e3:.2f} ms -> {value:.4e} atom-steps/s, "
@@ -682,6 +690,7 @@ def main():
             "precision": ("FP64 throughout" if precision == 64 else
                           "FP32 pair and k-space maths; FP64 positions, integrator, energy"),
             "hip_graph": use_graph,
+            "thermostat": args.thermostat,
             "list_builds_in_timed_steps": rebuilds,
             "force_rms_err_vs_fp64_ewald": eps_total,
             "force_rms_err_vs_fp64_ewald_kspace_part": eps_k,
@@ -711,7 +720,7 @@ def main():
     }
     if learned:
         out["learned_vs_plain"] = learned
-    if args.workload == "c2" and not args.skip_scale:
+    if args.workload == "c2" and not args.skip_scale and not args.thermostat:
         log("NVT (Berendsen) run of the same steps")
         out["nvt_berendsen"] = nvt_leg(f, st, wl, W, K, step, use_graph)
         log(f"NVT: {out['nvt_berendsen']['value']:.4e} atom-steps/s, "

@@ -384,7 +384,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.d_pp, sizeof(PairParams)));
     CR(dalloc(&E.d_step, 64));
     // thermostat: one partial per closing-kick workgroup, then scale / xi / T,
-    // the arrival and log counters and the noise step in one 64-byte block
+    // the log position and the noise step in one 64-byte block
     E.thermo_part_cap = std::max<int64_t>(((int64_t)N + kKfAtoms - 1) / kKfAtoms, 1);
     CR(dalloc(&E.thermo_part, (size_t)E.thermo_part_cap * 8));
     CR(dalloc(&E.thermo_st, 64));
@@ -650,12 +650,14 @@ static void md_segment(Engine& E, const RepState& st, double dt, uint64_t seed,
     if (E.thermo_type != kThermoNone) {
         // a thermostat needs the kinetic temperature of every atom between the
         // closing kick of a step and the opening kick of the next: each step is a
-        // segment of its own (opening kick, structure factor, tile pass, closing kick)
+        // segment of its own (opening kick, structure factor, tile pass, closing
+        // kick) followed by the one-workgroup temperature reduction
         for (int k = 0; k < steps; ++k) {
             launch_md_open(E, st, dt, sa_of(k), s);
             launch_md_sort_rho(E, st, s);
             launch_tile_rho(E, st.fx, st.fy, st.fz, s);
             launch_md_close(E, st, dt, s);
+            launch_thermo_reduce(E, dt, s);
         }
         return;
     }

@@ -205,16 +205,16 @@ enum : int {
     kThermoLangevin = 4,
 };
 
-// Constants and device state of a thermostat, by value in the closing-kick kernel.
-// Each workgroup of that kernel stores sum m |v|^2 of its atoms after the closing
-// kick in part[blk] and counts itself in cnt[0]; the last to arrive sums the
-// partials in a fixed order, forms T and the thermostat's coefficient, logs T, and
-// clears cnt[0] for the next step.
+// Constants and device state of a thermostat, by value in the closing-kick kernel
+// and the reduction after it.  Each workgroup of the closing kernel stores sum
+// m |v|^2 of its atoms after the closing kick in part[blk]; the reduction kernel
+// sums the partials in a fixed order, forms T and the thermostat's coefficient,
+// and logs T.
 struct ThermoArgs {
     int type = kThermoNone;
     int nblk = 0;
     double* part = nullptr;             // [nblk] per-workgroup sum of m |v|^2
-    uint32_t* cnt = nullptr;            // [0] workgroups arrived, [1] temperature log position
+    uint32_t* cnt = nullptr;            // [1] temperature log position ([0] unused)
     double* st = nullptr;               // [0] velocity scale for the next opening kick,
                                         // [1] Nose-Hoover xi, [2] temperature of the last step
     unsigned long long* noise = nullptr;  // Langevin noise step

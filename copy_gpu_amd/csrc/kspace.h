@@ -383,32 +383,16 @@ struct KickIn {
     double v[3], m;
 };
 
-// Thermostat step after the closing kick, in wave 0 of each workgroup (which holds
-// the workgroup's atoms); ke = m |v|^2 of the lane's atom after the closing kick
-// (0 for lanes without one).  The last workgroup to arrive forms
-// T = sum / (dof kB) with the partials summed in a fixed order (the same bits
-// whichever workgroup is last), then the coefficient of the thermostat
+// Thermostat coefficient from the kinetic energy sum of a step (sum m |v|^2 after
+// the closing kick): T = sum / (dof kB), then the coefficient of the thermostat
 // (ExecutionNVT::UpdateVelocityByTempConType, rbmd/framework/run/ExecutionNVT.cpp:256-291):
 // the velocity scale of Berendsen (sqrt(1 + dt/Tdamp (T0/T - 1))) or of the plain
 // rescale (sqrt(T0/T)), applied by the next opening kick, or the Nose-Hoover
-// friction update xi += dt/2 (T/T0 - 1) / (dt/10).
-__device__ inline void thermo_close(const ThermoArgs& th, int blk, double ke, double dt)
+// friction update xi += dt/2 (T/T0 - 1) / (dt/10); T is logged and the Langevin
+// noise step advanced.  One thread.
+__device__ inline void thermo_finish(const ThermoArgs& th, double sum, double dt)
 {
-    const int lane = threadIdx.x & 63;
-    const double s = wave_sum(ke);
-    uint32_t prev = 0;
-    if (lane == 0) {
-        th.part[blk] = s;
-        prev = __hip_atomic_fetch_add(th.cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    prev = __shfl(prev, 0, 64);
-    if (prev != (uint32_t)(th.nblk - 1)) return;
-    double acc = 0;
-    for (int j = lane; j < th.nblk; j += 64)
-        acc += __hip_atomic_load(th.part + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    acc = wave_sum(acc);
-    if (lane != 0) return;
-    const double T = acc * th.inv_dof_kb;
+    const double T = sum * th.inv_dof_kb;
     double scale = 1.0, xi = th.st[1];
     if (th.type == kThermoBerendsen) scale = sqrt(1.0 + th.dt_over_damp * (th.t0 / T - 1.0));
     else if (th.type == kThermoRescale) scale = sqrt(th.t0 / T);
@@ -420,16 +404,41 @@ __device__ inline void thermo_close(const ThermoArgs& th, int blk, double ke, do
     if (th.log && (int64_t)pos < th.log_len) th.log[pos] = T;
     th.cnt[1] = pos + 1u;
     if (th.type == kThermoLangevin) th.noise[0] += 1ull;
-    th.cnt[0] = 0u;
 }
+
+// The step's kinetic energy from the closing kernel's per-workgroup partials, summed
+// by one workgroup in a fixed order (the same bits on every run), then
+// thermo_finish.  Runs after the closing kernel, so no ordering between
+// workgroups of one kernel is needed.
+struct ThermoReduce {
+    static constexpr int kThreads = 256;
+    struct Args {
+        ThermoArgs th;
+        double dt;
+    };
+    __device__ static void run(const Args& a, int, int)
+    {
+        __shared__ double red[kThreads / 64];
+        const int t = threadIdx.x;
+        double acc = 0;
+        for (int j = t; j < a.th.nblk; j += kThreads) acc += a.th.part[j];
+        acc = wave_sum(acc);
+        if ((t & 63) == 0) red[t >> 6] = acc;
+        __syncthreads();
+        if (t != 0) return;
+        double s = 0;
+        for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+        thermo_finish(a.th, s, a.dt);
+    }
+};
 
 // F += k-space force; with KICK the closing half kick v += dt/2 F/m of the
 // velocity-Verlet step follows (and the MD step epoch advances), with the
 // thermostat of A.th: the Langevin force F += -m gamma v + sqrt(2 m gamma kBT/dt) g
 // before the kick (UnderdampedLangevinWorklet, rbmd/framework/run/worklet/RunWorklet.cpp:1700-1728),
 // the Nose-Hoover half step v += dt/2 (F/m - xi v) after it
-// (UpdateVelocityNoseHooverWorklet, RunWorklet.cpp:1784-1804), and the kinetic
-// temperature of the kicked velocities (thermo_close).
+// (UpdateVelocityNoseHooverWorklet, RunWorklet.cpp:1784-1804), and each
+// workgroup's sum of m |v|^2 of the kicked velocities (reduced by ThermoReduce).
 template <typename T, bool KICK>
 struct KForce {
     static constexpr int kThreads = kKfThreads;
@@ -486,8 +495,11 @@ struct KForce {
                     A.vz[i] = v2;
                 }
             });
-        if (KICK && th.type != kThermoNone && (threadIdx.x >> 6) == 0)
-            thermo_close(th, blk, ke, A.dt);
+        if (KICK && th.type != kThermoNone && (threadIdx.x >> 6) == 0) {
+            // wave 0 holds the workgroup's atoms: one partial per workgroup
+            const double s = wave_sum(ke);
+            if ((threadIdx.x & 63) == 0) th.part[blk] = s;
+        }
     }
 };
 

@@ -179,8 +179,8 @@ struct KickDrift {
     }
 };
 
-// Start of a thermostatted run: scale 1 for the first opening kick, no arrivals,
-// temperature log from position 0.  One thread.
+// Start of a thermostatted run: scale 1 for the first opening kick, temperature
+// log from position 0.  One thread.
 struct ThermoBegin {
     static constexpr int kThreads = 64;
     struct Args {
@@ -191,7 +191,6 @@ struct ThermoBegin {
     {
         if (blk != 0 || threadIdx.x != 0) return;
         a.st[0] = 1.0;
-        a.cnt[0] = 0u;
         a.cnt[1] = 0u;
     }
 };

@@ -202,6 +202,12 @@ void launch_thermo_begin(Engine& e, hipStream_t s)
         ThermoBegin::Args{e.thermo_st, e.thermo_cnt});
 }
 
+void launch_thermo_reduce(Engine& e, double dt, hipStream_t s)
+{
+    k_one<ThermoReduce><<<1, ThermoReduce::kThreads, 0, s>>>(
+        ThermoReduce::Args{thermo_args(e, dt), dt});
+}
+
 void launch_thermo_end(Engine& e, const RepState& st, hipStream_t s)
 {
     if (e.thermo_type != kThermoBerendsen && e.thermo_type != kThermoRescale) return;

@@ -133,7 +133,7 @@ struct Engine {
     double* thermo_part = nullptr;    // [thermo_part_cap] closing-kick workgroup partials
     int64_t thermo_part_cap = 0;
     double* thermo_st = nullptr;      // [4] scale, xi, last T, noise step (as counter below)
-    uint32_t* thermo_cnt = nullptr;   // [2] arrivals, log position
+    uint32_t* thermo_cnt = nullptr;   // [1] temperature log position
     unsigned long long* thermo_noise = nullptr;  // [1] Langevin noise step
     double* thermo_log = nullptr;     // caller's device array of T per step, or null
     int64_t thermo_log_len = 0;
@@ -269,6 +269,8 @@ void launch_md_close(Engine& e, const RepState& st, double dt, hipStream_t s);
 // thermostatted runs: state reset before the first step, last velocity scale after
 // the last (no kernel without a rescaling thermostat)
 void launch_thermo_begin(Engine& e, hipStream_t s);
+// after each closing kick: the kinetic temperature and the thermostat's coefficient
+void launch_thermo_reduce(Engine& e, double dt, hipStream_t s);
 void launch_thermo_end(Engine& e, const RepState& st, hipStream_t s);
 void launch_md_close_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
                           hipStream_t s);
