@@ -651,13 +651,20 @@ static void md_segment(Engine& E, const RepState& st, double dt, uint64_t seed,
         // a thermostat needs the kinetic temperature of every atom between the
         // closing kick of a step and the opening kick of the next: each step is a
         // segment of its own (opening kick, structure factor, tile pass, closing
-        // kick) followed by the one-workgroup temperature reduction
+        // kick) and the temperature is reduced before the next opening kick's
+        // velocities are used
+        // (for up to kFuseThermoPartials closing-kick workgroups the opening kick of
+        // step k > 0 reduces step k-1's temperature itself, every workgroup re-reading
+        // the partials, and only the segment's last step needs the reduction kernel;
+        // above that the re-reads would grow as n^2 and every step launches it)
+        constexpr int kFuseThermoPartials = 2048;
+        const bool fuse = std::max(kforce_blocks(E), 1) <= kFuseThermoPartials;
         for (int k = 0; k < steps; ++k) {
-            launch_md_open(E, st, dt, sa_of(k), s);
+            launch_md_open(E, st, dt, sa_of(k), s, fuse && k > 0);
             launch_md_sort_rho(E, st, s);
             launch_tile_rho(E, st.fx, st.fy, st.fz, s);
             launch_md_close(E, st, dt, s);
-            launch_thermo_reduce(E, dt, s);
+            if (!fuse || k + 1 == steps) launch_thermo_reduce(E, dt, s);
         }
         return;
     }

@@ -390,20 +390,45 @@ struct KickIn {
 // rescale (sqrt(T0/T)), applied by the next opening kick, or the Nose-Hoover
 // friction update xi += dt/2 (T/T0 - 1) / (dt/10); T is logged and the Langevin
 // noise step advanced.  One thread.
+__device__ inline double thermo_scale(const ThermoArgs& th, double sum)
+{
+    const double T = sum * th.inv_dof_kb;
+    if (th.type == kThermoBerendsen) return sqrt(1.0 + th.dt_over_damp * (th.t0 / T - 1.0));
+    if (th.type == kThermoRescale) return sqrt(th.t0 / T);
+    return 1.0;
+}
+
 __device__ inline void thermo_finish(const ThermoArgs& th, double sum, double dt)
 {
     const double T = sum * th.inv_dof_kb;
-    double scale = 1.0, xi = th.st[1];
-    if (th.type == kThermoBerendsen) scale = sqrt(1.0 + th.dt_over_damp * (th.t0 / T - 1.0));
-    else if (th.type == kThermoRescale) scale = sqrt(th.t0 / T);
-    else if (th.type == kThermoNoseHoover) xi = xi + 0.5 * dt * (T / th.t0 - 1.0) / (0.1 * dt);
-    th.st[0] = scale;
+    double xi = th.st[1];
+    if (th.type == kThermoNoseHoover) xi = xi + 0.5 * dt * (T / th.t0 - 1.0) / (0.1 * dt);
+    th.st[0] = thermo_scale(th, sum);
     th.st[1] = xi;
     th.st[2] = T;
     const uint32_t pos = th.cnt[1];
     if (th.log && (int64_t)pos < th.log_len) th.log[pos] = T;
     th.cnt[1] = pos + 1u;
     if (th.type == kThermoLangevin) th.noise[0] += 1ull;
+}
+
+// Sum of the closing kernel's per-workgroup partials by a workgroup of NT threads
+// in a fixed order (thread t takes j = t, t + NT, ...; then waves, then the NT/64
+// wave sums in order): every workgroup that calls it gets the same bits.  All
+// threads of the workgroup must call it.
+template <int NT>
+__device__ inline double thermo_sum(const ThermoArgs& th)
+{
+    __shared__ double red[NT / 64];
+    const int t = threadIdx.x;
+    double acc = 0;
+    for (int j = t; j < th.nblk; j += NT) acc += th.part[j];
+    acc = wave_sum(acc);
+    if ((t & 63) == 0) red[t >> 6] = acc;
+    __syncthreads();
+    double s = 0;
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    return s;
 }
 
 // The step's kinetic energy from the closing kernel's per-workgroup partials, summed
@@ -418,17 +443,8 @@ struct ThermoReduce {
     };
     __device__ static void run(const Args& a, int, int)
     {
-        __shared__ double red[kThreads / 64];
-        const int t = threadIdx.x;
-        double acc = 0;
-        for (int j = t; j < a.th.nblk; j += kThreads) acc += a.th.part[j];
-        acc = wave_sum(acc);
-        if ((t & 63) == 0) red[t >> 6] = acc;
-        __syncthreads();
-        if (t != 0) return;
-        double s = 0;
-        for (int w = 0; w < kThreads / 64; ++w) s += red[w];
-        thermo_finish(a.th, s, a.dt);
+        const double s = thermo_sum<kThreads>(a.th);
+        if (threadIdx.x == 0) thermo_finish(a.th, s, a.dt);
     }
 };
 

@@ -79,6 +79,7 @@ struct KickDriftArgs {
     double4* xs;
     SampleArgs sa;
     const double* vscale = nullptr;  // thermostat velocity scale (v = c v before the kick)
+    ThermoArgs th;  // set: reduce the step before's temperature here (no ThermoReduce)
 };
 
 // Per-atom inputs of the kick/drift besides velocity and force.
@@ -164,12 +165,22 @@ struct KickDrift {
             sample_block(a.sa, blk, cdf);
             return;
         }
+        double c = 1.0;
+        if (a.th.type != kThermoNone) {
+            // the step before's temperature, summed by every workgroup in the same
+            // order as ThermoReduce; the first atom workgroup also logs it and
+            // updates the thermostat state (which no other workgroup reads here)
+            const double s = thermo_sum<kMdBlock>(a.th);
+            if (blk == ns && threadIdx.x == 0) thermo_finish(a.th, s, a.dt);
+            c = thermo_scale(a.th, s);
+        } else if (a.vscale) {
+            c = *a.vscale;
+        }
         const int i = (blk - ns) * kMdBlock + threadIdx.x;
         if (i >= a.n) return;
-        if (a.vscale) {
+        if (a.vscale || a.th.type != kThermoNone) {
             // Berendsen / rescale coefficient of the step before
             // (UpdateVelocityRescaleWorklet, rbmd/framework/run/worklet/RunWorklet.cpp:1818-1830)
-            const double c = *a.vscale;
             kick_drift_atom(a, i, kd_load(a, i), c * a.vx[i], c * a.vy[i], c * a.vz[i], a.fx[i],
                             a.fy[i], a.fz[i]);
             return;

@@ -106,10 +106,15 @@ int close_open_blocks(const Engine& e, const SampleArgs& sa)
 }
 
 template <typename T>
-void md_open_t(Engine& e, const RepState& st, double dt, const SampleArgs& sa, hipStream_t s)
+void md_open_t(Engine& e, const RepState& st, double dt, const SampleArgs& sa, bool fuse_thermo,
+               hipStream_t s)
 {
-    k_one<KickDrift<T>><<<kickdrift_blocks(e, sa), kMdBlock, 0, s>>>(
-        kickdrift_args<T>(e, st, dt, sa));
+    KickDriftArgs<T> a = kickdrift_args<T>(e, st, dt, sa);
+    if (fuse_thermo && e.thermo_type != kThermoNone) {
+        a.th = thermo_args(e, dt);
+        a.vscale = nullptr;
+    }
+    k_one<KickDrift<T>><<<kickdrift_blocks(e, sa), kMdBlock, 0, s>>>(a);
 }
 
 template <typename T>
@@ -167,11 +172,11 @@ hipError_t md_prepare_t(Batch& b)
 // ================================================================= launchers
 
 void launch_md_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
-                    hipStream_t s)
+                    hipStream_t s, bool fuse_thermo)
 {
     ScopedTimer t(e, P_CHECK, s);
-    if (e.precision == 32) md_open_t<float>(e, st, dt, sa, s);
-    else md_open_t<double>(e, st, dt, sa, s);
+    if (e.precision == 32) md_open_t<float>(e, st, dt, sa, fuse_thermo, s);
+    else md_open_t<double>(e, st, dt, sa, fuse_thermo, s);
 }
 
 void launch_md_sort_rho(Engine& e, const RepState& st, hipStream_t s)

@@ -262,8 +262,10 @@ void launch_kforce(Engine& e, const double* x, const double* y, const double* z,
                    void* fy, void* fz, hipStream_t s);
 
 // ---- launchers (md.hip)
+// fuse_thermo: this opening kick also reduces the temperature of the step before
+// (whose closing kick then needs no ThermoReduce launch)
 void launch_md_open(Engine& e, const RepState& st, double dt, const SampleArgs& sa,
-                    hipStream_t s);
+                    hipStream_t s, bool fuse_thermo = false);
 void launch_md_sort_rho(Engine& e, const RepState& st, hipStream_t s);
 void launch_md_close(Engine& e, const RepState& st, double dt, hipStream_t s);
 // thermostatted runs: state reset before the first step, last velocity scale after

@@ -368,3 +368,43 @@ def test_md_thermostat_rejects_bad_parameters(gpu):
         f.set_thermostat("TEMP_RESCALE", (0.0, 0.0, 0.5))
     with pytest.raises(RBMDError, match="dof"):
         f.set_thermostat("TEMP_RESCALE", (1.0, 1.0, 0.5), dof=-1.0)
+
+
+def test_md_thermostat_large_system_reduction(gpu):
+    """140,000 ions (more closing-kick workgroups than the opening kick re-sums, so the
+    one-workgroup reduction runs after every step): under TEMP_RESCALE every step's
+    velocities leave at T0 exactly, so the returned velocities' temperature is T0 and
+    graph and eager runs agree bit for bit."""
+    from copy_gpu_amd import systems
+
+    _log("case: NVT TEMP_RESCALE, 140000 ions, per-step reduction kernel, 10 steps")
+    n, dt, steps, T0 = 140000, 0.002, 10, 1.2
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=64)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=64)
+    Lb = np.array([L] * 3)
+    outs = []
+    for use_graph in (True, False):
+        f = _make(Lb, n, 32, "RBE", P=100, skin=0.3)
+        f.Init(_dev(q))
+        f.sample_kvectors(100, SEED, 0)
+        X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+        F = [torch.zeros(n, device="cuda", dtype=f.fdtype) for _ in range(3)]
+        f.Execute(X, Y, Z, *F)
+        log = torch.zeros(steps, device="cuda", dtype=torch.float64)
+        f.set_thermostat("TEMP_RESCALE", (T0, T0, 0.5))
+        f.set_temperature_log(log)
+        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=steps, seed=SEED, step0=1,
+                 use_graph=use_graph)
+        torch.cuda.synchronize()
+        outs.append(([t.cpu().numpy() for t in (X, Y, Z, VX, VY, VZ)], log.cpu().numpy()))
+    (a, la), (b, lb) = outs
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+    assert np.array_equal(la, lb)
+    v = a[3:]
+    T = sum(float(np.dot(c, c)) for c in v) / (3 * n - 3)
+    _log(f"  T per step {la[0]:.6f} .. {la[-1]:.6f}; returned {T:.15f}")
+    assert abs(T - T0) < 1e-12
+    # every step logged a temperature (the random packing heats within each step, so
+    # the closing-kick values sit above T0)
+    assert np.all(np.isfinite(la)) and np.all(la > 0) and np.unique(la).size == steps
