@@ -182,9 +182,9 @@ def nvt_leg(f, st, wl, W, K, step, use_graph):
     return {"thermostat": "BERENDSEN", "t_target": 1.0, "t_damp": 0.5,
             "value": n * K / el, "unit": "atom-steps/s", "ms_per_step": el * 1e3 / K,
             "steps": K, "list_builds": rebuilds, "temperature_last_step": T,
-            "note": "five kernels per step instead of three: opening kick (applying the "
-                    "rescale), structure factor, tile pass, closing kick (per-workgroup "
-                    "sums of m v^2) and a one-workgroup temperature reduction"}
+            "note": "four kernels per step instead of three: opening kick (summing the "
+                    "step before's temperature and applying the rescale), structure factor, "
+                    "tile pass, closing kick (per-workgroup sums of m v^2)"}
 
 
 def kernel_times(f, st, wl, Kt, step):

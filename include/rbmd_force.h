@@ -145,7 +145,9 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
  *   LANGEVIN     F += -m gamma v + sqrt(2 m gamma kbt / dt) g before the closing
  *                kick, g one Gaussian 3-vector per step shared by all atoms (noise
  *                stream (seed, noise_step), advanced once per step).
- * A thermostatted run launches five kernels per step instead of three. */
+ * A thermostatted run launches four kernels per step instead of three (the
+ * opening kick also sums the step before's temperature), plus a one-workgroup
+ * temperature reduction after the last step of each segment. */
 #define RBMD_THERMO_NONE 0
 #define RBMD_THERMO_BERENDSEN 1
 #define RBMD_THERMO_RESCALE 2
