@@ -66,6 +66,7 @@ THERMO_BERENDSEN = 1
 THERMO_RESCALE = 2
 THERMO_NOSE_HOOVER = 3
 THERMO_LANGEVIN = 4
+THERMO_MEASURE = 5
 
 
 class Thermostat(C.Structure):

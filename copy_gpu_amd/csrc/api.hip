@@ -865,10 +865,11 @@ int rbmd_md_set_thermostat(rbmd_force_t h, const rbmd_thermostat* t)
 {
     if (!h || !t) return RBMD_ERR_INVALID;
     Engine& E = h->e;
-    if (t->type < RBMD_THERMO_NONE || t->type > RBMD_THERMO_LANGEVIN)
+    if (t->type < RBMD_THERMO_NONE || t->type > RBMD_THERMO_MEASURE)
         return fail(E, RBMD_ERR_INVALID, "thermostat: unknown type");
     if (t->type != RBMD_THERMO_NONE) {
-        if (!(t->t_target > 0) && t->type != RBMD_THERMO_LANGEVIN)
+        if (!(t->t_target > 0) && t->type != RBMD_THERMO_LANGEVIN &&
+            t->type != RBMD_THERMO_MEASURE)
             return fail(E, RBMD_ERR_INVALID, "thermostat: t_target must be > 0");
         if (t->type == RBMD_THERMO_BERENDSEN && !(t->t_damp > 0))
             return fail(E, RBMD_ERR_INVALID, "thermostat: BERENDSEN needs t_damp > 0");

@@ -203,6 +203,7 @@ enum : int {
     kThermoRescale = 2,
     kThermoNoseHoover = 3,
     kThermoLangevin = 4,
+    kThermoMeasure = 5,  // no velocity change: the temperature is only measured and logged
 };
 
 // Constants and device state of a thermostat, by value in the closing-kick kernel

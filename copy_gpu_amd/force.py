@@ -237,7 +237,7 @@ class LJCutCoulKspaceForce:
     # ------------------------------------------------------------ velocity Verlet
     _THERMO = {"NONE": L.THERMO_NONE, "BERENDSEN": L.THERMO_BERENDSEN,
                "TEMP_RESCALE": L.THERMO_RESCALE, "NOSE_HOOVER": L.THERMO_NOSE_HOOVER,
-               "LANGEVIN": L.THERMO_LANGEVIN}
+               "LANGEVIN": L.THERMO_LANGEVIN, "MEASURE": L.THERMO_MEASURE}
 
     def set_thermostat(self, temp_ctrl_type="NONE", temperature=(1.0, 1.0, 0.5), dof=0.0,
                        kb=1.0, xi=0.0, gamma=100.0, kbt=1.0, seed=0, noise_step=0):

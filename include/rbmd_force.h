@@ -153,6 +153,8 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
 #define RBMD_THERMO_RESCALE 2
 #define RBMD_THERMO_NOSE_HOOVER 3
 #define RBMD_THERMO_LANGEVIN 4
+#define RBMD_THERMO_MEASURE 5   /* velocities untouched; T of every step measured and
+                                   logged, as ComputeTempe does for the thermo output */
 typedef struct {
     int32_t type;        /* RBMD_THERMO_*                                          */
     double t_target;     /* "temperature"[0]                                       */

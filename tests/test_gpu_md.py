@@ -408,3 +408,51 @@ def test_md_thermostat_large_system_reduction(gpu):
     # every step logged a temperature (the random packing heats within each step, so
     # the closing-kick values sit above T0)
     assert np.all(np.isfinite(la)) and np.all(la > 0) and np.unique(la).size == steps
+
+
+def test_thermo_output_row_matches_oracle(orc, gpu):
+    """The reference's thermo row after an MD run with the measure-only thermostat:
+    per-atom LJ, near and far (k-space + self) energies against the oracle's FP64 Ewald
+    energies at the returned positions, the kinetic energy and temperature against the
+    returned velocities (untouched by the thermostat), Residual between two rows."""
+    from copy_gpu_amd import systems, thermo
+
+    _log("case: thermo rows, FP64 Ewald, 1500 ions, MEASURE thermostat")
+    n, dt, alpha, tol = 1500, 0.002, 0.5, 1e-12
+    x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=65)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=65)
+    Lb = np.array([L] * 3)
+    mass = 1.0 + 0.5 * (np.arange(n) % 2)
+    f = _make(Lb, n, 64, "EWALD", skin=0.3, ewald_tol=tol, alpha=alpha)
+    f.Init(_dev(q))
+    X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+    F = [torch.zeros(n, device="cuda", dtype=f.fdtype) for _ in range(3)]
+    f.Execute(X, Y, Z, *F)
+    f.set_thermostat("MEASURE")
+    M = _dev(mass)
+    rows = []
+    for k in range(2):
+        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=10, seed=SEED, step0=1 + 10 * k,
+                 mass=M)
+        rows.append(thermo.thermo_row(f, X, Y, Z, 10 * (k + 1), dt,
+                                      rows[-1] if rows else None))
+    torch.cuda.synchronize()
+    r = rows[-1]
+    p = [t.cpu().numpy() for t in (X, Y, Z)]
+    v = [t.cpu().numpy() for t in (VX, VY, VZ)]
+    _, evp = orc.pair_forces(*p, q, np.zeros(3), Lb, 2.5, 5.0, alpha)
+    m, w = _halfspace_lattice(Lb, alpha, tol)
+    _, evk = orc.kspace_forces(*p, q, Lb, alpha, m, w)
+    e_self = orc.self_energy(q, alpha)
+    assert abs(r["VanderWaalsEnergy"] - evp[0] / n) < 1e-11 * abs(evp[0] / n)
+    assert abs(r["NearCoulombicEnergy"] - evp[1] / n) < 1e-11 * abs(evp[1] / n)
+    far = (evk[0] + e_self) / n
+    assert abs(r["FarCoulombicEnergy"] - far) < 1e-10 * abs(far)
+    ke = 0.5 * sum(float(np.dot(mass * c, c)) for c in v) / n
+    assert abs(r["KinticEnergy"] - ke) < 1e-12 * ke
+    assert abs(r["Temperature"] - orc.temperature(v, mass)) < 1e-12 * r["Temperature"]
+    assert r["TotalEnergy"] == r["PotentialEnergy"] + r["KinticEnergy"]
+    assert r["Residual"] == r["PotentialEnergy"] - rows[0]["PotentialEnergy"]
+    text = thermo.format_rows(rows)
+    assert text.splitlines()[0].split() == list(thermo.COLUMNS)
+    assert float(text.splitlines()[2].split()[2]) == r["VanderWaalsEnergy"]
