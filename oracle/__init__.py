@@ -46,7 +46,14 @@ def lib():
             "orc_self_energy": (d, [i64, P, d, d]),
             "orc_kick": (None, [i64, P, P, P, P, P, P, P, d, d]),
             "orc_drift": (None, [i64, P, P, P, P, P, P, d, P, P, P]),
+            "orc_philox4x32": (None, [P, P, P]),
+            "orc_rbl_select": (C.c_int, [C.c_uint64, C.c_uint32, i32, i32, C.c_uint32]),
+            "orc_rbl_forces": (None, [i64, P, P, P, P, P, P, P, P, P, d, d, d, d, d, i32,
+                                      C.c_uint64, C.c_uint32, P, P, P, P]),
+            "orc_neighbors_images": (i64, [i64, P, P, P, P, d, P, P, i64]),
             "orc_faithful_run": (C.c_int, [i64, P, P, P, P, P, P, d, d, d, i32, P, i32, d, i32]),
+            "orc_faithful_run_f64": (C.c_int, [i64, P, P, P, P, P, P, d, d, d, i32, P, i32, d,
+                                               i32]),
             "orc_num_threads": (C.c_int, []),
             "orc_temperature": (d, [i64, P, P, P, P, d, d, d]),
             "orc_scale_velocities": (None, [i64, P, P, P, d]),
@@ -118,6 +125,41 @@ def pair_forces(x, y, z, q, lo, Lb, rc_lj, rc_coul, alpha, qqr2e=1.0, types=None
                           _p(_d(lo)), _p(_d(Lb)), float(rc_lj), float(rc_coul), float(alpha),
                           float(qqr2e), _p(fx), _p(fy), _p(fz), _p(ev))
     return np.stack([fx, fy, fz], 1), ev
+
+
+def philox4x32(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(_p(c), _p(k), _p(out))
+    return out
+
+
+def rbl_select(seed, epoch, i, j, pice):
+    return bool(lib().orc_rbl_select(int(seed), int(epoch), int(i), int(j), int(pice)))
+
+
+def rbl_forces(x, y, z, q, lo, Lb, rc_lj, rc_coul, alpha, r_core, pice, seed, epoch,
+               qqr2e=1.0, types=None, eps=(1.0,), sigma=(1.0,)):
+    """Random batch list near forces (core + selected shell, mean removed) and ev8."""
+    x, y, z, q = _d(x), _d(y), _d(z), _d(q)
+    n = x.size
+    fx, fy, fz = np.zeros(n), np.zeros(n), np.zeros(n)
+    ev = np.zeros(8)
+    t = None if types is None else np.ascontiguousarray(types, np.int32)
+    lib().orc_rbl_forces(n, _p(x), _p(y), _p(z), _p(q), _p(t), _p(_d(eps)), _p(_d(sigma)),
+                         _p(_d(lo)), _p(_d(Lb)), float(rc_lj), float(rc_coul), float(alpha),
+                         float(qqr2e), float(r_core), int(pice), int(seed), int(epoch), _p(fx),
+                         _p(fy), _p(fz), _p(ev))
+    return np.stack([fx, fy, fz], 1), ev
+
+
+def neighbors_images(x, y, z, Lb, r_list):
+    """Every periodic image (shifts k L, k in {-1, 0, 1}^3) of every atom within r_list
+    of each wrapped atom, as CSR of atom ids (an id once per image)."""
+    x, y, z = _d(x), _d(y), _d(z)
+    return _csr(lib().orc_neighbors_images, x.size, x.size, _p(x), _p(y), _p(z), _p(_d(Lb)),
+                float(r_list))
 
 
 def rbe_S(alpha, Lb):
@@ -230,12 +272,15 @@ def langevin_force(f, v, g, gamma, kbt, dt, mass=None):
 
 def faithful_run(pos, vel, force, q, lo, Lb, rc_lj, rc_coul, alpha, P, m_steps, nsteps, dt,
                  nthreads=1):
-    """Reference-faithful FP32 steps (arrays updated in place; AoS float32 [n,3])."""
+    """Reference-faithful steps (arrays updated in place; AoS [n,3] float32 for the
+    FP32 build, float64 for the FP64 build; q None: no charges)."""
+    dt_ = pos.dtype
+    assert dt_ in (np.float32, np.float64)
     for a in (pos, vel, force):
-        assert a.dtype == np.float32 and a.flags.c_contiguous
-    qf = np.ascontiguousarray(q, np.float32)
-    m = np.ascontiguousarray(m_steps, np.int32)
-    return lib().orc_faithful_run(pos.shape[0], _p(pos), _p(vel), _p(force), _p(qf),
-                                  _p(_d(lo)), _p(_d(Lb)), float(rc_lj), float(rc_coul),
-                                  float(alpha), int(P), _p(m), int(nsteps), float(dt),
-                                  int(nthreads))
+        assert a.dtype == dt_ and a.flags.c_contiguous
+    qf = None if q is None else np.ascontiguousarray(q, dt_)
+    m = np.ascontiguousarray(m_steps if m_steps is not None else np.zeros(3), np.int32)
+    fn = lib().orc_faithful_run if dt_ == np.float32 else lib().orc_faithful_run_f64
+    return fn(pos.shape[0], _p(pos), _p(vel), _p(force), _p(qf), _p(_d(lo)), _p(_d(Lb)),
+              float(rc_lj), float(rc_coul), float(alpha), int(P), _p(m), int(nsteps), float(dt),
+              int(nthreads))

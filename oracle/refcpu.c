@@ -115,6 +115,39 @@ int64_t orc_neighbors_brute(int64_t n, const double* x, const double* y, const d
     return e <= cap ? e : -e;
 }
 
+/* All-pairs neighbour list over periodic images, for boxes narrower than two list
+ * radii: j (i itself included) appears once for every shift k L, k in {-1,0,1}^3,
+ * whose image satisfies the criterion above with dx = (x_j - k L) - x_i (the
+ * reference's PeriodicCoordOffset order, RunWorklet.cpp:77-79).  The 27-cell stencil
+ * of a grid with cells at least r_list wide (ContPointLocator.h:44-53) visits exactly
+ * these images when the box is at least r_list wide.  Positions must be wrapped. */
+int64_t orc_neighbors_images(int64_t n, const double* x, const double* y, const double* z,
+                             const double* L, double r_list, int64_t* offsets, int32_t* idx,
+                             int64_t cap)
+{
+    const double rl2 = r_list * r_list;
+    int64_t e = 0;
+    offsets[0] = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t j = 0; j < n; ++j) {
+            for (int s = 0; s < 27; ++s) {
+                const double kx = (double)(s / 9 - 1), ky = (double)((s / 3) % 3 - 1),
+                             kz = (double)(s % 3 - 1);
+                double dx = (x[j] - kx * L[0]) - x[i];
+                double dy = (y[j] - ky * L[1]) - y[i];
+                double dz = (z[j] - kz * L[2]) - z[i];
+                double d2 = orc_d2(dx, dy, dz);
+                if ((rl2 - d2) > 1e-4 && d2 > 1e-4) {
+                    if (e < cap) idx[e] = (int32_t)j;
+                    ++e;
+                }
+            }
+        }
+        offsets[i + 1] = e;
+    }
+    return e <= cap ? e : -e;
+}
+
 /* Cell grid of the reference locator: dims = trunc(L / r) per axis
  * (ContPointLocator::SetRange, locator/ContPointLocator.h:44-53, with
  * num_cycles = 1), cell of a point = trunc((p - lo)/dxdydz) clamped to the
@@ -239,26 +272,73 @@ double orc_gnear(double alpha, double r)
     return erfc(sa * r) / (r * r) + 2.0 * sa * exp(-alpha * r * r) / (sqrt(ORC_PI) * r);
 }
 
-/* Per-atom pair forces, energy and virial over every j within the cutoffs.
- *   LJ  (ExecForceFunction::ComputeLJForce, forceFunction/ExecForceFunction.h:74-96):
- *       f_i = -24 sqrt(e_i e_j) (2s-1) s / d2 * r_ij, s = (sig_ij^2/d2)^3,
- *       sig_ij = (sig_i+sig_j)/2, r_ij = r_j - r_i, when 1e-4 < d2 < rc_lj^2.
- *   Coulomb (ComputeNearEnergyForce + Gnear, ExecForceFunction.h:125-140,217-224):
- *       f_i = -q_i q_j G(alpha,d) r_ij / d, when 0.01 < d < rc_coul, times qqr2e.
- *   Energies (ComputePotentialEn :296-313, ComputeNearEleEnergy :315-329):
- *       1/2 * 4 eps (s^2 - s) and 1/2 * q_i q_j erfc(sqrt(a) d)/d per directed pair.
- *   Virial, per directed pair, W_ab += -1/2 fs r_a r_b with f_i = fs * r_ij (the
- *       convention of the reference GPU kernel ComputeLJForceViral,
- *       rbmd-gpu src/force/src/ljforce_op/rocm/ljforce_op.hip.cu:555-575).
- * The loop runs over all j of the 27-cell stencil (or all j when the grid has
- * fewer than 3 cells per axis), i.e. the set the full Verlet list of
- * RunWorklet.cpp:170-237 visits.  No same-molecule exclusion: in-built ions
- * have molecule id = atom id (InitCondition/MeshFreeCondition.cpp:120-128).
- * ev (nullable): [E_lj, E_real, W_xx, W_yy, W_zz, W_xy, W_xz, W_yz]. */
-void orc_pair_forces(int64_t n, const double* x, const double* y, const double* z,
-                     const double* q, const int32_t* type, const double* eps, const double* sig,
-                     const double* lo, const double* L, double rc_lj, double rc_coul,
-                     double alpha, double qqr2e, double* fx, double* fy, double* fz, double* ev)
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of
+ * (hi0, lo0) = 0xD2511F53 c0, (hi1, lo1) = 0xCD9E8D57 c2,
+ * c = (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0), key += (0x9E3779B9, 0xBB67AE85).
+ * Written here on its own (the product library has its own copy); pinned by the
+ * Random123 known-answer vectors. */
+void orc_philox4x32(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0;
+        const uint32_t h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+        c0 = h1 ^ c1 ^ k0;
+        c1 = l1;
+        c2 = h0 ^ c3 ^ k1;
+        c3 = l0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Random batch list shell selection of the pair (i, j) (atom ids, i the atom whose
+ * list holds j) at list build `epoch`: Philox4x32-10 word 0 of counter
+ * (i, j, epoch, 0x52424C53) under key (seed_lo, seed_hi), divisible by pice.  The
+ * reference keeps every pice_num-th shell pair of its stencil sweep
+ * (ComputeRBLNeighboursOnceWorklet, run/worklet/RunWorklet.cpp:690-697); the
+ * counter-based draw replaces that order, which the cell sort's ties decide. */
+int orc_rbl_select(uint64_t seed, uint32_t epoch, int32_t i, int32_t j, uint32_t pice)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)j, epoch, 0x52424C53u};
+    uint32_t r[4];
+    orc_philox4x32(ctr, key, r);
+    return r[0] % pice == 0u;
+}
+
+/* Per-pair weight of the near force: 1 for the full list; for a random batch
+ * list (rbl != NULL) the reference's core / shell criteria
+ * (ComputeRBLNeighboursOnceWorklet, RunWorklet.cpp:686-703, and the pice_num factor
+ * of ComputeNearForceRBLWorklet, :1015-1022): 1 when r_core^2 - d2 > 1e-4 and
+ * d2 > 1e-4, pice for a selected pair with rc^2 - d2 > 1e-4 and d2 - r_core^2 > 1e-4
+ * (rc = the larger cutoff), 0 otherwise. */
+typedef struct {
+    double rs2, rc2;
+    uint32_t pice, epoch;
+    uint64_t seed;
+} orc_rbl;
+
+static double orc_pair_weight(const orc_rbl* rb, double d2, int64_t i, int64_t j)
+{
+    if (!rb) return 1.0;
+    if ((rb->rs2 - d2 > 1e-4) && (d2 > 1e-4)) return 1.0;
+    if ((rb->rc2 - d2 > 1e-4) && (d2 - rb->rs2 > 1e-4) &&
+        orc_rbl_select(rb->seed, rb->epoch, (int32_t)i, (int32_t)j, rb->pice))
+        return (double)rb->pice;
+    return 0.0;
+}
+
+static void orc_pair_forces_w(int64_t n, const double* x, const double* y, const double* z,
+                              const double* q, const int32_t* type, const double* eps,
+                              const double* sig, const double* lo, const double* L, double rc_lj,
+                              double rc_coul, double alpha, double qqr2e, const orc_rbl* rb,
+                              double* fx, double* fy, double* fz, double* ev)
 {
     double rmax = rc_lj > rc_coul ? rc_lj : rc_coul;
     orc_cells c;
@@ -295,6 +375,8 @@ void orc_pair_forces(int64_t n, const double* x, const double* y, const double* 
                 double dz = orc_mi(z[j], z[i], L[2]);
                 double d2 = orc_d2(dx, dy, dz);
                 if (!(d2 > 1e-4)) continue;
+                const double w = orc_pair_weight(rb, d2, i, j);
+                if (w == 0.0) continue;
                 double fs = 0.0;
                 if (d2 < rcl2) {
                     int tj = type ? type[j] : 0;
@@ -303,7 +385,7 @@ void orc_pair_forces(int64_t n, const double* x, const double* y, const double* 
                     double s2 = s_ij * s_ij / d2;
                     double s6 = s2 * s2 * s2;
                     fs += -24.0 * e_ij * (2.0 * s6 - 1.0) * s6 / d2;
-                    acc[0] += 0.5 * 4.0 * e_ij * (s6 * s6 - s6);
+                    acc[0] += w * (0.5 * 4.0 * e_ij * (s6 * s6 - s6));
                 }
                 if (d2 < rcc2) {
                     double d = sqrt(d2);
@@ -311,8 +393,9 @@ void orc_pair_forces(int64_t n, const double* x, const double* y, const double* 
                     double er = erfc(sa * d);
                     double g = er / d2 + tsp * exp(-alpha * d2) / d;
                     fs += -qq * g / d;
-                    acc[1] += 0.5 * qq * er / d;
+                    acc[1] += w * (0.5 * qq * er / d);
                 }
+                fs *= w;
                 fxi += fs * dx;
                 fyi += fs * dy;
                 fzi += fs * dz;
@@ -332,6 +415,61 @@ void orc_pair_forces(int64_t n, const double* x, const double* y, const double* 
     if (ev)
         for (int k = 0; k < 8; ++k) ev[k] = acc[k];
     orc_cells_free(&c);
+}
+
+/* Per-atom pair forces, energy and virial over every j within the cutoffs.
+ *   LJ  (ExecForceFunction::ComputeLJForce, forceFunction/ExecForceFunction.h:74-96):
+ *       f_i = -24 sqrt(e_i e_j) (2s-1) s / d2 * r_ij, s = (sig_ij^2/d2)^3,
+ *       sig_ij = (sig_i+sig_j)/2, r_ij = r_j - r_i, when 1e-4 < d2 < rc_lj^2.
+ *   Coulomb (ComputeNearEnergyForce + Gnear, ExecForceFunction.h:125-140,217-224):
+ *       f_i = -q_i q_j G(alpha,d) r_ij / d, when 0.01 < d < rc_coul, times qqr2e.
+ *   Energies (ComputePotentialEn :296-313, ComputeNearEleEnergy :315-329):
+ *       1/2 * 4 eps (s^2 - s) and 1/2 * q_i q_j erfc(sqrt(a) d)/d per directed pair.
+ *   Virial, per directed pair, W_ab += -1/2 fs r_a r_b with f_i = fs * r_ij (the
+ *       convention of the reference GPU kernel ComputeLJForceViral,
+ *       rbmd-gpu src/force/src/ljforce_op/rocm/ljforce_op.hip.cu:555-575).
+ * The loop runs over all j of the 27-cell stencil (or all j when the grid has
+ * fewer than 3 cells per axis), i.e. the set the full Verlet list of
+ * RunWorklet.cpp:170-237 visits.  No same-molecule exclusion: in-built ions
+ * have molecule id = atom id (InitCondition/MeshFreeCondition.cpp:120-128).
+ * ev (nullable): [E_lj, E_real, W_xx, W_yy, W_zz, W_xy, W_xz, W_yz]. */
+void orc_pair_forces(int64_t n, const double* x, const double* y, const double* z,
+                     const double* q, const int32_t* type, const double* eps, const double* sig,
+                     const double* lo, const double* L, double rc_lj, double rc_coul,
+                     double alpha, double qqr2e, double* fx, double* fy, double* fz, double* ev)
+{
+    orc_pair_forces_w(n, x, y, z, q, type, eps, sig, lo, L, rc_lj, rc_coul, alpha, qqr2e, NULL,
+                      fx, fy, fz, ev);
+}
+
+/* Random batch list near forces (ExecutionMD::ComputeRBLNearForce,
+ * run/ExecutionMD.cpp:460-566): the pair forces of orc_pair_forces over the core
+ * pairs (weight 1) and the shell pairs orc_rbl_select keeps at build `epoch`
+ * (weight pice), energies and virial weighted alike; then the mean force over all
+ * atoms, summed in atom order, is subtracted from every atom (SumRBLCorrForceWorklet,
+ * run/worklet/RunWorklet.cpp:1304-1321).  rc of the shell = max(rc_lj, rc_coul). */
+void orc_rbl_forces(int64_t n, const double* x, const double* y, const double* z,
+                    const double* q, const int32_t* type, const double* eps, const double* sig,
+                    const double* lo, const double* L, double rc_lj, double rc_coul,
+                    double alpha, double qqr2e, double r_core, int32_t pice, uint64_t seed,
+                    uint32_t epoch, double* fx, double* fy, double* fz, double* ev)
+{
+    const double rc = rc_lj > rc_coul ? rc_lj : rc_coul;
+    orc_rbl rb = {r_core * r_core, rc * rc, (uint32_t)pice, epoch, seed};
+    orc_pair_forces_w(n, x, y, z, q, type, eps, sig, lo, L, rc_lj, rc_coul, alpha, qqr2e, &rb,
+                      fx, fy, fz, ev);
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < n; ++i) {
+        m[0] += fx[i];
+        m[1] += fy[i];
+        m[2] += fz[i];
+    }
+    for (int d = 0; d < 3; ++d) m[d] /= (double)n;
+    for (int64_t i = 0; i < n; ++i) {
+        fx[i] -= m[0];
+        fy[i] -= m[1];
+        fz[i] -= m[2];
+    }
 }
 
 /* ------------------------------------------------------------------ */
