@@ -28,6 +28,8 @@ KSPACE_NONE = 0
 KSPACE_RBE = 1
 KSPACE_EWALD = 2
 RBMD_EV = 1
+NEIGHBOR_VERLET = 0
+NEIGHBOR_RBL = 1
 
 
 class ForceParams(C.Structure):
@@ -48,6 +50,11 @@ class ForceParams(C.Structure):
         ("ewald_tol", C.c_double),
         ("list_capacity", C.c_int32),
         ("tile_stage", C.c_int32),
+        ("neighbor_type", C.c_int32),
+        ("r_core", C.c_double),
+        ("rbl_pice", C.c_int32),
+        ("rbl_seed", C.c_uint64),
+        ("full_list", C.c_int32),
     ]
 
 
@@ -81,6 +88,16 @@ class Thermostat(C.Structure):
         ("kbt", C.c_double),
         ("seed", C.c_uint64),
         ("noise_step", C.c_uint64),
+    ]
+
+
+class KspaceParams(C.Structure):
+    _fields_ = [
+        ("box_lo", C.c_double * 3),
+        ("box_len", C.c_double * 3),
+        ("alpha", C.c_double),
+        ("qqr2e", C.c_double),
+        ("precision", C.c_int32),
     ]
 
 
@@ -130,6 +147,9 @@ _SIGS = {
     "rbmd_force_get_timing": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]),
     "rbmd_util_copy": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "rbmd_kspace_compute": (C.c_int, [C.POINTER(KspaceParams), C.c_int64] + [_P] * 4 +
+                            [C.c_int32] + [_P] * 8),
+    "rbmd_kspace_last_error": (C.c_char_p, []),
     "rbmd_batch_create": (C.c_int, [C.c_int32, _P, _P, C.POINTER(C.c_void_p)]),
     "rbmd_batch_destroy": (None, [_P]),
     "rbmd_batch_last_error": (C.c_char_p, [_P]),

@@ -20,8 +20,8 @@ LIB = os.path.join(HERE, "librbmd_force.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RBMD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "nlist.hip", "pair.hip", "kspace.hip", "md.hip", "mlp.hip", "systems.cpp",
-           "proposal.cpp", "io.cpp"]
+SOURCES = ["api.hip", "nlist.hip", "pair.hip", "kspace.hip", "kcompute.hip", "md.hip", "mlp.hip",
+           "systems.cpp", "proposal.cpp", "io.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/rbmd_force.h"
+#include "half.h"
 #include "kspace.h"
 #include "proposal.h"
 #include "state.h"
@@ -111,7 +112,7 @@ static void free_all(Engine& E)
                     E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
                     E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide, E.thermo_part,
-                    E.thermo_st};
+                    E.thermo_st, E.rbl_part, E.rbl_mean, E.planes};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (E.h_stats) hipHostFree(E.h_stats);
@@ -129,8 +130,8 @@ static void free_all(Engine& E)
 static bool fit_stage(Engine& E)
 {
     const size_t lim = (size_t)E.lds_max - 4096;
-    while (E.stage_cap > 64 && list_lds_bytes(E.stage_cap, E.cap) > lim) E.stage_cap -= 64;
-    return list_lds_bytes(E.stage_cap, E.cap) <= lim;
+    while (E.stage_cap > 64 && list_lds_bytes(E.stage_cap, E.cap, E.rbl) > lim) E.stage_cap -= 64;
+    return list_lds_bytes(E.stage_cap, E.cap, E.rbl) <= lim;
 }
 
 static int alloc_list(Engine& E, int cap)
@@ -205,8 +206,39 @@ static hipError_t reset_tile_order(Engine& E)
     return hipMemcpy(E.tile_order, id.data(), id.size() * 4, hipMemcpyHostToDevice);
 }
 
+// Half-shell tiles (home cell + 13 forward cells): the LDS holds the staged
+// positions (stage_cap entries) and the force sums (acc_cap entries, 24 bytes
+// each); the stage has room for a tile 6 standard deviations above the mean, the
+// sums for any tile that size or the stage's, whichever is larger, and a tile is
+// never split over several workgroups.
+static void half_geometry(Engine& E, int64_t n)
+{
+    const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
+    const double cellv = E.cell[0] * E.cell[1] * E.cell[2];
+    const double mean = 14.0 * (double)n / V * cellv;
+    const int per = (int)sizeof(double4) + (E.pp.ntypes > 1 ? 4 : 0);
+    const int want = ((int)(mean + 6.0 * std::sqrt(mean) + 64.0) + 63) / 64 * 64;
+    int cap = want;
+    if (E.tune_stage_scale > 0.0) cap = ((int)(E.tune_stage_scale * mean + 64.0) + 63) / 64 * 64;
+    if (E.stage_override > 0) cap = E.stage_override;
+    E.stage_cap = std::max(64, cap);
+    E.acc_cap = std::max(E.stage_cap, want) + 64;
+    const int64_t budget = (int64_t)E.lds_max - 4096;
+    auto bytes = [&]() {
+        return (int64_t)E.stage_cap * per + (int64_t)E.acc_cap * 24;
+    };
+    while (E.stage_cap > 64 && (bytes() > budget ||
+                                (int64_t)list_lds_bytes(E.stage_cap, E.cap) > budget))
+        E.stage_cap -= 64;
+    E.split = 1;
+}
+
 static void tile_geometry(Engine& E, int64_t n)
 {
+    if (E.half) {
+        half_geometry(E, n);
+        return;
+    }
     const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
     const double cellv = E.cell[0] * E.cell[1] * E.cell[2];
     const double mean = 27.0 * (double)n / V * cellv;
@@ -289,18 +321,44 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     E.ewald_tol = p->ewald_tol > 0 ? p->ewald_tol : 1e-16;
     E.r_list = std::max(p->rc_lj, p->rc_coul) + p->skin;
     E.r_in = p->rc_lj + p->skin;
+    if (p->neighbor_type != RBMD_NEIGHBOR_VERLET && p->neighbor_type != RBMD_NEIGHBOR_RBL) {
+        g_create_err = "neighbor_type must be RBMD_NEIGHBOR_VERLET or RBMD_NEIGHBOR_RBL";
+        delete h;
+        return RBMD_ERR_INVALID;
+    }
+    if (p->neighbor_type == RBMD_NEIGHBOR_RBL) {
+        if (!(p->r_core > 0) || !(p->r_core < std::max(p->rc_lj, p->rc_coul)) ||
+            p->rbl_pice < 1 || p->rbl_pice > 65536) {
+            g_create_err = "RBL needs 0 < r_core < max(rc_lj, rc_coul) and 1 <= rbl_pice <= 65536";
+            delete h;
+            return RBMD_ERR_INVALID;
+        }
+        E.rbl = true;
+        E.r_core = p->r_core;
+        E.rbl_pice = p->rbl_pice;
+        E.rbl_seed = p->rbl_seed;
+    }
     E.n_max = n_max;
     E.stage_override = p->tile_stage;
     for (int d = 0; d < 3; ++d) {
         E.dims[d] = (int)(E.box.L[d] / E.r_list);
         E.cell[d] = E.box.L[d] / (E.dims[d] > 0 ? E.dims[d] : 1);
     }
-    if (E.dims[0] < 3 || E.dims[1] < 3 || E.dims[2] < 3) {
-        g_create_err = "box must be at least 3 list radii (max cutoff + skin) on every axis";
+    // cells at least one list radius wide: boxes narrower than three cells take
+    // the same 27-cell stencil, whose wrapped cells then repeat with different
+    // periodic shifts, so every image within the list radius is a candidate (the
+    // reference's locator does the same, ContPointLocator.h:44-53,
+    // ExecPointLocator.h:303-319)
+    if (E.dims[0] < 1 || E.dims[1] < 1 || E.dims[2] < 1) {
+        g_create_err = "box must be at least one list radius (max cutoff + skin) on every axis";
         delete h;
         return RBMD_ERR_BOX;
     }
     E.ncell = E.dims[0] * E.dims[1] * E.dims[2];
+    // the half-shell pair pass: FP32, full Verlet list, every neighbouring cell pair
+    // distinct (three or more cells per axis), unless the caller asks for full rows
+    E.half = p->full_list == 0 && p->precision == 32 && !E.rbl && E.dims[0] >= 3 &&
+             E.dims[1] >= 3 && E.dims[2] >= 3;
     // pair parameters
     PairParams& pp = E.pp;
     pp.rcl2 = p->rc_lj * p->rc_lj;
@@ -323,7 +381,9 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     // capacity: 1.35x the mean row at the density of a full handle, >= 128
     const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
     const double mean = (double)n_max / V * 4.0 / 3.0 * M_PI * pow(E.r_list, 3);
-    int cap = (int)ceil(1.35 * mean + 32.0);
+    // half-shell rows hold half the neighbours, plus up to the home cell's population
+    // more or less depending on the atom's place in its cell
+    int cap = E.half ? (int)ceil(1.35 * 0.5 * mean + 48.0) : (int)ceil(1.35 * mean + 32.0);
     cap = std::max(128, ((cap + 63) / 64) * 64);
     if (p->list_capacity > 0) cap = ((p->list_capacity + 63) / 64) * 64;
 
@@ -393,6 +453,12 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(dalloc(&E.ev, 16 * 8));
     E.ev_part_blocks = E.ncell * 4;  // tile workgroups (split <= 4)
     CR(dalloc(&E.ev_part, (size_t)E.ev_part_blocks * 8 * 8));
+    if (E.half) CR(dalloc(&E.planes, (size_t)kHalfSegs * 3 * N * sizeof(float)));
+    if (E.rbl) {
+        CR(dalloc(&E.rbl_part, (size_t)E.ev_part_blocks * 3 * 8));
+        CR(dalloc(&E.rbl_mean, 4 * 8));
+        CR(hipMemset(E.rbl_mean, 0, 4 * 8));
+    }
     CR(hipHostMalloc((void**)&E.h_stats, 64, hipHostMallocDefault));
     CR(hipStreamCreateWithFlags(&E.own_stream, hipStreamNonBlocking));
     CR(hipEventCreateWithFlags(&E.order_event, hipEventDisableTiming));
@@ -570,7 +636,10 @@ static int force_pass(Engine& E, const double* x, const double* y, const double*
     launch_gather(E, x, y, z, s);
     if (ks) launch_rho(E, x, y, z, s);
     launch_pair(E, fx, fy, fz, ev, s);
-    if (ks) launch_kforce(E, x, y, z, fx, fy, fz, s);
+    // a random batch list's near forces have their mean subtracted by the k-space
+    // force kernel (which runs without k-vectors when there is no k-space)
+    if (E.rbl) launch_rbl_mean(E, s);
+    if (ks || E.rbl) launch_kforce(E, x, y, z, fx, fy, fz, s);
     if (ev) launch_ev_final(E, true, s);
     return RBMD_OK;
 }
@@ -579,6 +648,26 @@ static int force_pass(Engine& E, const double* x, const double* y, const double*
 // still exact through its stencil fallback, only slower).
 static int maintain(Engine& E, hipStream_t s)
 {
+    if (E.half && E.h_stats[7] != 0ull) {
+        // a half tile larger than its LDS force sums was skipped (stats[7], mirrored
+        // after every call): report it once and return the handle to full rows (no
+        // such limit) for the following calls
+        unsigned long long f7 = 0;
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(&f7, E.stats + 7, 8, hipMemcpyDeviceToHost));
+        if (f7) {
+            E.half = false;
+            tile_geometry(E, E.n_max);
+            int rc = alloc_list(E, (int)std::max<int64_t>(2 * E.cap, 128));
+            if (rc != RBMD_OK) return rc;
+            HIP_TRY(hipMemset(E.stats + 7, 0, 8));
+            E.h_stats[7] = 0ull;
+            launch_force_rebuild(E, s);
+            return fail(E, RBMD_ERR_CAPACITY,
+                        "a half-shell tile exceeded its LDS force sums; the last forces are "
+                        "invalid and the handle now uses full rows");
+        }
+    }
     if (E.h_stats[2] == 0ull) return RBMD_OK;
     HIP_TRY(hipStreamSynchronize(s));
     unsigned long long st[8];
@@ -663,6 +752,7 @@ static void md_segment(Engine& E, const RepState& st, double dt, uint64_t seed,
             launch_md_open(E, st, dt, sa_of(k), s, fuse && k > 0);
             launch_md_sort_rho(E, st, s);
             launch_tile_rho(E, st.fx, st.fy, st.fz, s);
+            launch_rbl_mean(E, s);
             launch_md_close(E, st, dt, s);
             if (!fuse || k + 1 == steps) launch_thermo_reduce(E, dt, s);
         }
@@ -672,6 +762,7 @@ static void md_segment(Engine& E, const RepState& st, double dt, uint64_t seed,
     for (int k = 0; k < steps; ++k) {
         launch_md_sort_rho(E, st, s);
         launch_tile_rho(E, st.fx, st.fy, st.fz, s);
+        launch_rbl_mean(E, s);  // random batch list only
         if (k + 1 < steps) launch_md_close_open(E, st, dt, sa_of(k + 1), s);
         else launch_md_close(E, st, dt, s);
     }
@@ -795,27 +886,40 @@ int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int
     HIP_TRY(hipMemcpy(cs.data(), E.cell_start, cs.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(nc.data(), E.ncount, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(rows.data(), E.nlist, rows.size() * 2, hipMemcpyDeviceToHost));
-    // rows hold tile-local indices of the atom's cell tile (stencil order of tile.h)
+    // rows hold tile-local indices of the atom's cell tile (stencil order of tile.h;
+    // half tiles: the home cell, then the forward cells in stencil order, half.h)
+    const int nseg = E.half ? kHalfSegs : 27;
     auto tile = [&](int c, int off[28], int gstart[27]) {
         const int dx = E.dims[0], dy = E.dims[1], dz = E.dims[2];
         const int cz = c / (dx * dy), cy = (c / dx) % dy, cx = c % dx;
+        int order[27], k = 0;
+        if (E.half) {
+            order[k++] = kHomeSeg;
+            for (int t = 0; t < 27; ++t) {
+                const int a = t / 9 - 1, b = (t / 3) % 3 - 1, g = t % 3 - 1;
+                if (g > 0 || (g == 0 && (b > 0 || (b == 0 && a > 0)))) order[k++] = t;
+            }
+        } else {
+            for (int t = 0; t < 27; ++t) order[k++] = t;
+        }
         int o = 0;
-        for (int t = 0; t < 27; ++t) {
+        for (int q = 0; q < nseg; ++q) {
+            const int t = order[q];
             const int ax = (cx + t / 9 - 1 + dx) % dx, ay = (cy + (t / 3) % 3 - 1 + dy) % dy,
                       az = (cz + t % 3 - 1 + dz) % dz;
             const int cc = ax + ay * dx + az * dx * dy;
-            off[t] = o;
-            gstart[t] = cs[(size_t)cc];
+            off[q] = o;
+            gstart[q] = cs[(size_t)cc];
             o += cs[(size_t)cc + 1] - cs[(size_t)cc];
         }
-        off[27] = o;
+        off[nseg] = o;
     };
     std::vector<int64_t> slot_of((size_t)n);
     for (int64_t s = 0; s < n; ++s) slot_of[(size_t)perm[(size_t)s]] = s;
-    int64_t e = 0;
-    offsets[0] = 0;
     bool overflow = false;
     int off[28], gst[27], last_c = -1;
+    // the rows in atom ids; a half-shell row's pair (i, j) also goes to j's row
+    std::vector<std::vector<int32_t>> lists((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
         const int64_t s = slot_of[(size_t)i];
         const int ni = nc[(size_t)s].x, no = nc[(size_t)s].y;
@@ -827,10 +931,19 @@ int rbmd_force_get_neighbors(rbmd_force_t h, int64_t* offsets, int32_t* idx, int
         }
         const uint16_t* row = rows.data() + (size_t)s * E.cap;
         for (int k = 0; k < ni + no && !overflow; ++k) {
-            const int u = row[k];
-            int st = 26;
+            const int u = row[k] & kMaxTile;  // a random batch list's selection tag aside
+            int st = nseg - 1;
             while (off[st] > u) --st;
-            if (idx && e < cap) idx[e] = perm[(size_t)(gst[st] + (u - off[st]))];
+            const int32_t j = perm[(size_t)(gst[st] + (u - off[st]))];
+            lists[(size_t)i].push_back(j);
+            if (E.half) lists[(size_t)j].push_back((int32_t)i);
+        }
+    }
+    int64_t e = 0;
+    offsets[0] = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int32_t j : lists[(size_t)i]) {
+            if (idx && e < cap) idx[e] = j;
             ++e;
         }
         offsets[i + 1] = e;
@@ -938,11 +1051,11 @@ int rbmd_force_get_stats(rbmd_force_t h, int64_t* out)
         HIP_TRY(hipMemcpy(ls, E.stats + 8, sizeof ls, hipMemcpyDeviceToHost));
     }
     out[0] = (int64_t)st[0];
-    out[1] = (int64_t)ls[0];
+    out[1] = (int64_t)ls[0] * (E.half ? 2 : 1);  // full-list entries (a half row holds each pair once)
     out[2] = E.cap;
     out[3] = (int64_t)st[2];
     out[4] = E.ncell;
-    out[5] = (int64_t)ls[1];
+    out[5] = (int64_t)ls[1] * (E.half ? 2 : 1);
     out[6] = E.kspace == RBMD_KSPACE_EWALD ? E.P : 0;
     out[7] = E.n;
     out[8] = (int64_t)st[6];
@@ -1174,6 +1287,21 @@ static int bfail(rbmd_batch_t b, int code, const std::string& msg)
 static void batch_segment(rbmd_batch_t b, int steps, hipStream_t s)
 {
     Batch& B = b->b;
+    if (B.thermo) {
+        // as md_segment with a thermostat: every step is a segment of its own, and
+        // the kinetic temperature of each thermostatted replica is reduced after
+        // its closing kick (one workgroup per replica) before the next opening kick
+        // applies the coefficient; replicas without a thermostat take the same
+        // steps with no reduction (their opening and closing kicks carry none)
+        for (int k = 0; k < steps; ++k) {
+            md_batch_launch(B, BP_KICKDRIFT, s);
+            md_batch_launch(B, BP_SORT_RHO, s);
+            pair_batch_launch(B, BP_PAIR, s);
+            md_batch_launch(B, BP_KFORCE, s);
+            md_batch_launch(B, BP_THERMO_REDUCE, s);
+        }
+        return;
+    }
     md_batch_launch(B, BP_KICKDRIFT, s);
     for (int k = 0; k < steps; ++k) {
         md_batch_launch(B, BP_SORT_RHO, s);
@@ -1192,6 +1320,11 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
     rbmd_batch_t b = new rbmd_batch_s();
     for (int32_t r = 0; r < nrep; ++r) {
         const rbmd_force_params& p = params[r];
+        if (p.neighbor_type == RBMD_NEIGHBOR_RBL) {
+            g_create_err = "batched replicas use the full Verlet list (RBL is single-handle only)";
+            rbmd_batch_destroy(b);
+            return RBMD_ERR_INVALID;
+        }
         if (p.precision != params[0].precision || p.kspace != params[0].kspace ||
             (p.ntypes > 1) != (params[0].ntypes > 1)) {
             g_create_err = "replicas must share precision, k-space mode and single/multi type";
@@ -1199,7 +1332,10 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
             return RBMD_ERR_INVALID;
         }
         rbmd_force_t h = nullptr;
-        const int rc = rbmd_force_create(&p, n_max[r], &h);
+        // replicas share the batched full-row tile pass
+        rbmd_force_params pr = p;
+        pr.full_list = 1;
+        const int rc = rbmd_force_create(&pr, n_max[r], &h);
         if (rc != RBMD_OK) {
             rbmd_batch_destroy(b);
             return rc;
@@ -1265,6 +1401,9 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         if (E.n > 0 && (!q.x || !q.y || !q.z || !q.vx || !q.vy || !q.vz || !q.fx || !q.fy ||
                         !q.fz))
             return bfail(b, RBMD_ERR_INVALID, "null state pointer");
+        if (E.thermo_type != kThermoNone && !(thermo_dof(E) > 0))
+            return bfail(b, RBMD_ERR_STATE, "replica " + std::to_string(r) +
+                                                ": thermostat without degrees of freedom");
     }
     if (nsteps <= 0) return RBMD_OK;
     hipStream_t cs = (hipStream_t)stream;
@@ -1284,6 +1423,8 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     B.st.resize(R);
     B.sa.assign(R, SampleArgs{});
     B.dt = dt;
+    B.thermo = false;
+    for (const Engine* E : B.es) B.thermo = B.thermo || E->thermo_type != kThermoNone;
     for (size_t r = 0; r < R; ++r) {
         const rbmd_replica_state& q = st[r];
         RepState& o = B.st[r];
@@ -1304,6 +1445,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     for (Engine* E : B.es)
         BHIP_TRY(hipMemsetAsync(E->md_count, 0, (size_t)2 * E->ncell * sizeof(int32_t), s));
     nlist_batch_launch(B, BP_MDBIN, s);
+    if (B.thermo) md_batch_launch(B, BP_THERMO_BEGIN, s);
     if (!use_graph) {
         constexpr int64_t kEagerSegment = int64_t(1) << 30;
         for (int64_t k = 0; k < nsteps; k += kEagerSegment)
@@ -1319,6 +1461,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         };
         for (const BatchPhase& ph : B.ph) key_phase(ph);
         for (const BatchPhase& ph : B.pair_more) key_phase(ph);
+        key.push_back((const void*)(uintptr_t)(B.thermo ? 1 : 0));
         for (int g = 0; g < 2; ++g) {
             Engine::StepGraph& G = b->graphs[g];
             const int steps = g == 0 ? kGraphSteps : 1;
@@ -1342,6 +1485,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         for (int64_t k = 0; k < nsteps % kGraphSteps; ++k)
             BHIP_TRY(hipGraphLaunch(b->graphs[1].exec, s));
     }
+    if (B.thermo) md_batch_launch(B, BP_THERMO_END, s);
     BHIP_TRY(hipStreamSynchronize(s));
     for (size_t r = 0; r < R; ++r) {
         Engine& E = *B.es[r];

@@ -87,6 +87,22 @@ RBMD_HD inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_i
     out[3] = c3;
 }
 
+// Random batch list shell selection: whether the shell pair (i, j) -- original atom
+// ids, i the atom whose row holds j -- is kept at list build `epoch`: Philox4x32-10
+// with key = seed and counter (i, j, epoch, 0x52424C53), word 0 divisible by pice
+// (probability 1/pice up to 2^-32 rounding).  The reference keeps every pice-th shell
+// pair of its stencil sweep instead (ComputeRBLNeighboursOnceWorklet,
+// rbmd/framework/run/worklet/RunWorklet.cpp:690-697), an order the cell sort's ties
+// decide; a counter-based draw makes the sample reproducible.
+RBMD_HD inline bool rbl_select(uint64_t seed, uint32_t epoch, int32_t i, int32_t j, uint32_t pice)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)j, epoch, 0x52424C53u};
+    uint32_t r[4];
+    philox4x32_10(ctr, key, r);
+    return r[0] % pice == 0u;
+}
+
 // 53-bit uniform in [0,1) from two 32-bit words (exact in FP64).
 RBMD_HD inline double u53(uint32_t a, uint32_t b)
 {

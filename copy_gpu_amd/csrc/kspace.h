@@ -299,6 +299,7 @@ struct KForceArgs {
     double dt;
     uint32_t* flags;
     ThermoArgs th;  // velocity thermostat of the closing kick (KForce<T, true>)
+    const double* rbl_mean = nullptr;  // random batch list: mean near force to subtract
 };
 
 template <typename T>
@@ -334,6 +335,13 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
         f0 = A.fx[i];
         f1 = A.fy[i];
         f2 = A.fz[i];
+        if (A.rbl_mean) {
+            // the random batch list's mean-force correction (SumRBLCorrForceWorklet,
+            // rbmd/framework/run/worklet/RunWorklet.cpp:1304-1321)
+            f0 = (T)((double)f0 - A.rbl_mean[0]);
+            f1 = (T)((double)f1 - A.rbl_mean[1]);
+            f2 = (T)((double)f2 - A.rbl_mean[2]);
+        }
         if (P > 0) qi = (T)A.q[i];
         pv = pre(i);
     }
@@ -443,6 +451,7 @@ struct ThermoReduce {
     };
     __device__ static void run(const Args& a, int, int)
     {
+        if (a.th.type == kThermoNone) return;  // a batch replica without a thermostat
         const double s = thermo_sum<kThreads>(a.th);
         if (threadIdx.x == 0) thermo_finish(a.th, s, a.dt);
     }
@@ -545,9 +554,11 @@ inline KForceArgs<T> kforce_args(Engine& e, const double* x, const double* y, co
                                  void* fx, void* fy, void* fz, double* vx, double* vy,
                                  double* vz, const double* mass, double dt, bool kspace = true)
 {
-    return KForceArgs<T>{x,        y,  z,  e.q,  (int)e.n, e.box, (const KCoef<T>*)e.kcoef,
-                         kspace ? e.P : 0, (T*)fx, (T*)fy, (T*)fz, vx, vy, vz, mass, dt,
-                         e.flags};
+    KForceArgs<T> a{x,        y,  z,  e.q,  (int)e.n, e.box, (const KCoef<T>*)e.kcoef,
+                    kspace ? e.P : 0, (T*)fx, (T*)fy, (T*)fz, vx, vy, vz, mass, dt,
+                    e.flags};
+    a.rbl_mean = e.rbl ? e.rbl_mean : nullptr;
+    return a;
 }
 
 inline int kforce_blocks(const Engine& e) { return (int)((e.n + kKfAtoms - 1) / kKfAtoms); }

@@ -164,7 +164,27 @@ hipError_t md_prepare_t(Batch& b)
     if ((err = phase_upload(b.ph[BP_KICKDRIFT], kd, nkd, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_SORT_RHO], sr, nsr, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_KFORCE], kc, nkc, 0)) != hipSuccess) return err;
-    return phase_upload(b.ph[BP_KFORCE_DRIFT], ko, nko, 0);
+    if ((err = phase_upload(b.ph[BP_KFORCE_DRIFT], ko, nko, 0)) != hipSuccess) return err;
+    // thermostats: one workgroup per thermostatted replica for the state reset and
+    // the per-step reduction, the atoms of rescaling ones for the final scale
+    std::vector<ThermoBegin::Args> tb(R);
+    std::vector<ThermoReduce::Args> tr(R);
+    std::vector<VScale::Args> te(R);
+    std::vector<int> ntb(R, 0), ntr(R, 0), nte(R, 0);
+    for (size_t r = 0; r < R; ++r) {
+        Engine& e = *b.es[r];
+        const RepState& st = b.st[r];
+        tb[r] = ThermoBegin::Args{e.thermo_st, e.thermo_cnt};
+        tr[r] = ThermoReduce::Args{thermo_args(e, b.dt), b.dt};
+        te[r] = VScale::Args{st.vx, st.vy, st.vz, e.thermo_st, (int)e.n};
+        if (e.thermo_type == kThermoNone) continue;
+        ntb[r] = ntr[r] = 1;
+        if (e.thermo_type == kThermoBerendsen || e.thermo_type == kThermoRescale)
+            nte[r] = std::max(ceil_div(e.n, kMdBlock), 1);
+    }
+    if ((err = phase_upload(b.ph[BP_THERMO_BEGIN], tb, ntb, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_THERMO_REDUCE], tr, ntr, 0)) != hipSuccess) return err;
+    return phase_upload(b.ph[BP_THERMO_END], te, nte, 0);
 }
 
 }  // namespace
@@ -241,6 +261,9 @@ void md_batch_launch(Batch& b, int phase, hipStream_t s)
         if (f32) phase_launch<KForce<float, true>>(b.ph[phase], s);
         else phase_launch<KForce<double, true>>(b.ph[phase], s);
         break;
+    case BP_THERMO_BEGIN: phase_launch<ThermoBegin>(b.ph[phase], s); break;
+    case BP_THERMO_REDUCE: phase_launch<ThermoReduce>(b.ph[phase], s); break;
+    case BP_THERMO_END: phase_launch<VScale>(b.ph[phase], s); break;
     default: break;
     }
 #undef RBMD_MB

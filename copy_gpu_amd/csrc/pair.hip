@@ -14,6 +14,7 @@
 // Pair displacements are FP64 differences of the staged positions (no per-pair
 // minimum image), the force maths runs in T (float or double), and cutoff
 // decisions use the FP64 squared distance.
+#include "half.h"
 #include "kspace.h"
 #include "state.h"
 #include "tile.h"
@@ -138,17 +139,40 @@ __device__ inline T pair_fs(double d2, double qq, int ti, int tj, bool ok, const
 // accumulated in FP64 (one FMA per component, no conversion of dx), and with EV
 // the energies and the virial W_ab += -1/2 fs dx_a dx_b in FP64.  The two
 // variants differ only in fs, so the accumulation follows the branch once.
-template <typename T, bool EV, bool MULTI>
+// Random batch list weight of a pair at squared distance d2 (the reference's core
+// and shell criteria, RunWorklet.cpp:686-703): 1 inside the core, pice for a selected
+// shell pair, 0 otherwise (unselected shell pairs, the 1e-4 bands around r_core^2 and
+// the cutoff, which the reference's two lists also leave out).
+struct RblPair {
+    double rs2, rc2;  // r_core^2, max(rc_lj, rc_coul)^2
+    float pice;
+};
+
+__device__ inline float rbl_weight(const RblPair& r, double d2, bool sel)
+{
+    const bool core = (r.rs2 - d2 > 1e-4) && (d2 > 1e-4);
+    const bool shell = sel && (r.rc2 - d2 > 1e-4) && (d2 - r.rs2 > 1e-4);
+    return core ? 1.0f : (shell ? r.pice : 0.0f);
+}
+
+template <typename T, bool EV, bool MULTI, bool RBL = false>
 __device__ inline void accumulate_pair(bool lj, double dx, double dy, double dz, double qq, int ti,
                                        int tj, bool valid, const PairScalars& ps,
                                        const PairParams* __restrict__ pp, double& fx, double& fy,
-                                       double& fz, double* acc)
+                                       double& fz, double* acc, const RblPair& rb = RblPair{},
+                                       bool sel = false)
 {
     const double d2 = dx * dx + dy * dy + dz * dz;
     const bool ok = valid && d2 > 1e-4;
     T e_lj = 0, e_re = 0;
-    const T fs = lj ? pair_fs<T, EV, MULTI, true>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re)
-                    : pair_fs<T, EV, MULTI, false>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re);
+    T fs = lj ? pair_fs<T, EV, MULTI, true>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re)
+              : pair_fs<T, EV, MULTI, false>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re);
+    if (RBL) {
+        const T w = (T)rbl_weight(rb, d2, sel);
+        fs *= w;
+        e_lj *= w;
+        e_re *= w;
+    }
     const double f = (double)fs;
     fx = fma(f, dx, fx);
     fy = fma(f, dy, fy);
@@ -261,13 +285,26 @@ __device__ inline int wave_max_int(int v)
 // Otherwise (a row that overflowed, or lane groups past the tile's last atom)
 // the round tests each slot, and rows that overflowed scan every tile candidate
 // instead.
-template <typename T, bool EV, bool MULTI, bool STAGED>
+// Random batch list arguments of the pair pass: the pair weights, the shell
+// selection the overflowed rows' candidate scan applies itself, and the per-tile
+// sums of the near forces (for the mean-force correction).
+struct RblArgs {
+    RblPair w;
+    uint64_t seed;
+    uint32_t pice;
+    double near2;
+    double* part;  // [tile workgroups][3]
+};
+
+template <typename T, bool EV, bool MULTI, bool STAGED, bool RBL = false>
 __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
                                  const double4* __restrict__ lds,
                                  const int32_t* __restrict__ ltype, int a0, int a1, int h0, int wv,
                                  int lane, const PairScalars& ps,
                                  const PairParams* __restrict__ pp, T* __restrict__ fx,
-                                 T* __restrict__ fy, T* __restrict__ fz, double* acc)
+                                 T* __restrict__ fy, T* __restrict__ fz, double* acc,
+                                 const RblArgs& rbl = RblArgs{}, uint32_t epoch = 0,
+                                 double* fsum = nullptr)
 {
     const int total = h.off[27];
     const int cap = a.cap;
@@ -315,8 +352,14 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
                 if (base + kGroup * kRound < tmax) load_u(base + kGroup * kRound, un);
                 double4 pj[kRound];
                 int tj[kRound] = {};
+                bool sel[kRound] = {};
 #pragma unroll
                 for (int k = 0; k < kRound; ++k) {
+                    if (RBL) {
+                        // random batch rows: selected shell entries carry kRblSel
+                        sel[k] = u[k] >= 0 && (u[k] & kRblSel) != 0;
+                        if (u[k] >= 0) u[k] &= kMaxTile;
+                    }
                     const int uu = FAST ? u[k] : (u[k] < 0 ? 0 : u[k]);
                     pj[k] = tile_pos<STAGED>(h, lds, a.xs, uu);
                     if (MULTI) tj[k] = tile_type<STAGED>(h, ltype, a.ts, uu);
@@ -324,10 +367,10 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
 #pragma unroll
                 for (int k = 0; k < kRound; ++k) {
                     if (base + kGroup * k < tmax)
-                        accumulate_pair<T, EV, MULTI>(
+                        accumulate_pair<T, EV, MULTI, RBL>(
                             base + kGroup * k < nin_max, pj[k].x - pi.x, pj[k].y - pi.y,
                             pj[k].z - pi.z, qi * pj[k].w, ti, tj[k], FAST || u[k] >= 0, ps, pp,
-                            ax, ay, az, acc);
+                            ax, ay, az, acc, rbl.w, sel[k]);
                 }
             }
         };
@@ -336,12 +379,17 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
         else
             rounds(std::false_type{});
         if (__ballot(live && over) != 0ull) {
+            const int32_t id_i = RBL && live ? a.perm[s] : 0;
             for (int uu = gl; uu < total; uu += kGroup) {
                 const double4 pj = tile_pos<STAGED>(h, lds, a.xs, uu);
                 const int tj = MULTI ? tile_type<STAGED>(h, ltype, a.ts, uu) : 0;
-                accumulate_pair<T, EV, MULTI>(true, pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
-                                              qi * pj.w, ti, tj, live && over, ps, pp, ax, ay,
-                                              az, acc);
+                bool sel = false;
+                if (RBL && live && over)
+                    sel = rbl_select(rbl.seed, epoch, id_i, tile_atom_id(h, a.perm, uu),
+                                     rbl.pice);
+                accumulate_pair<T, EV, MULTI, RBL>(true, pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
+                                                   qi * pj.w, ti, tj, live && over, ps, pp, ax,
+                                                   ay, az, acc, rbl.w, sel);
             }
         }
         ax = group_sum(ax);
@@ -352,6 +400,12 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
             fx[i] = (T)ax;
             fy[i] = (T)ay;
             fz[i] = (T)az;
+            if (RBL) {
+                // the stored forces, summed for the mean-force correction
+                fsum[0] += (double)(T)ax;
+                fsum[1] += (double)(T)ay;
+                fsum[2] += (double)(T)az;
+            }
         }
     }
 }
@@ -373,6 +427,24 @@ __device__ inline void block_ev(const double* acc, int lane, int wv, double* __r
     }
 }
 
+// Per-workgroup FP64 sums of the near forces (random batch list): wave sums, then
+// the 8 waves in order.
+__device__ inline void block_fsum(const double* fs, int lane, int wv, double* __restrict__ out)
+{
+    __shared__ double red[kTileWaves][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double v = wave_sum_dpp(fs[k]);
+        if (lane == 0) red[wv][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = 0.0;
+        for (int w = 0; w < kTileWaves; ++w) v += red[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+}
+
 // Tile pass: one workgroup per (cell tile, part).  On a rebuild step it first
 // builds the Verlet rows of its home atoms (list_wave over the tile's reference
 // positions), then the pair pass: the tile's current positions are staged in
@@ -385,7 +457,7 @@ __device__ inline void block_ev(const double* acc, int lane, int wv, double* __r
 // the workgroup finally bins its home atoms for the next rebuild; that comes
 // last so its atomics' latency overlaps the other workgroups' pair work rather
 // than delaying this one's.
-template <typename T, bool EV, bool MULTI>
+template <typename T, bool EV, bool MULTI, bool RBL = false>
 struct TilePass {
     static constexpr int kThreads = kTileThreads;
     struct Args {
@@ -410,6 +482,7 @@ struct TilePass {
         double3 cell;  // cell edge lengths (binning)
         MdBins bins;
         const int32_t* order;  // workgroup -> cell * split + part (null: identity)
+        RblArgs rbl;           // random batch list (RBL instantiation)
     };
     __device__ static void run(const Args& A, int blk, int)
     {
@@ -433,24 +506,31 @@ struct TilePass {
         const int wv = uniform(threadIdx.x >> 6);
         const int h0 = h.gstart[kHomeSeg], hc = h.cnt[kHomeSeg];
         const int a0 = h0 + (hc * part) / A.split, a1 = h0 + (hc * (part + 1)) / A.split;
+        // list build number (the random batch list's selection epoch); the build
+        // kernels before this one have counted the current build
+        const uint32_t epoch = RBL ? (uint32_t)A.stats[0] : 0u;
         if (rebuild) {
             if (!staged && part == 0 && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
             const RefStage rs(lds, A.stage_cap);
+            int32_t* ids = RBL && staged ? stage_ids(lds, A.stage_cap, a.cap) : nullptr;
             if (staged) {
-                for (int u = threadIdx.x; u < total; u += kTileThreads)
+                for (int u = threadIdx.x; u < total; u += kTileThreads) {
                     rs.put(u, stage_ref(h, A.xs_ref, u));
+                    if (RBL) ids[u] = tile_atom_id(h, a.perm, u);
+                }
                 const int pad = ref_stage_len(total);  // the sweep reads whole chunks
                 for (int u = total + (int)threadIdx.x; u < pad; u += kTileThreads)
                     rs.put(u, h.dummy);
             }
             __syncthreads();
             uint16_t* hb = hit_buffer(lds, A.stage_cap, a.cap, wv);
+            const RblBuild rb{A.rbl.near2, A.rbl.seed, epoch, A.rbl.pice, a.perm, ids};
             if (staged)
-                list_wave<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap, hb,
-                                A.nlist, A.ncount, A.stats);
+                list_wave<true, RBL>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2,
+                                     a.cap, hb, A.nlist, A.ncount, A.stats, rb);
             else
-                list_wave<false>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2, a.cap,
-                                 hb, A.nlist, A.ncount, A.stats);
+                list_wave<false, RBL>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2,
+                                      a.cap, hb, A.nlist, A.ncount, A.stats, rb);
             __syncthreads();  // rows and LDS are reused below
         }
         if (staged) {
@@ -484,13 +564,16 @@ struct TilePass {
         }
         __syncthreads();
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double fsum[3] = {0, 0, 0};
         if (staged)
-            pair_wave<T, EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
-                                          A.fx, A.fy, A.fz, acc);
+            pair_wave<T, EV, MULTI, true, RBL>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps,
+                                               A.pp, A.fx, A.fy, A.fz, acc, A.rbl, epoch, fsum);
         else
-            pair_wave<T, EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
-                                           A.pp, A.fx, A.fy, A.fz, acc);
+            pair_wave<T, EV, MULTI, false, RBL>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane,
+                                                A.ps, A.pp, A.fx, A.fy, A.fz, acc, A.rbl, epoch,
+                                                fsum);
         if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)tb * 8);
+        if (RBL) block_fsum(fsum, lane, wv, A.rbl.part + (size_t)tb * 3);
         if (A.md) {
             // bins of the next rebuild: clear this step's buffer (consumed by the
             // cell sort) for this cell, bin the current home atoms into the other
@@ -514,6 +597,343 @@ struct TilePass {
     }
 };
 
+// ================================================================ half-shell pass
+// Position / type of half-tile candidate u (LDS when staged; the row sentinel
+// u = total reads the dummy).
+template <bool STAGED>
+__device__ inline double4 half_pos(const HalfHdr& h, const double4* __restrict__ lds,
+                                   const double4* __restrict__ xs, int u)
+{
+    if (STAGED) return lds[u];
+    const int total = h.off[kHalfSegs];
+    const int uc = u < total ? u : 0;
+    const int st = half_seg(h, uc);
+    const double4 p = xs[h.gstart[st] + (uc - h.off[st])];
+    return u < total ? make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w)
+                     : h.dummy;
+}
+
+template <bool STAGED>
+__device__ inline int half_type(const HalfHdr& h, const int32_t* __restrict__ ltype,
+                                const int32_t* __restrict__ ts, int u)
+{
+    if (STAGED) return ltype[u];
+    const int total = h.off[kHalfSegs];
+    const int uc = u < total ? u : 0;
+    const int st = half_seg(h, uc);
+    const int t = ts[h.gstart[st] + (uc - h.off[st])];
+    return u < total ? t : 0;
+}
+
+// LDS sums of the half pass: the forces on every candidate of the tile.
+struct HalfAcc {
+    double *x, *y, *z;
+};
+
+// One pair of a half row: the FP32 force factor as in the full pass, the force on
+// i accumulated in FP64 registers, minus the same on j added to the LDS sums on the
+// 2^-30 grid; with EV the pair's whole energy (both directed halves) and virial.
+template <bool EV, bool MULTI>
+__device__ inline void accumulate_half(bool lj, double dx, double dy, double dz, double qq,
+                                       int ti, int tj, bool valid, const PairScalars& ps,
+                                       const PairParams* __restrict__ pp, double& fx,
+                                       double& fy, double& fz, double* acc, const HalfAcc& J,
+                                       int u)
+{
+#pragma clang fp contract(off)
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const bool ok = valid && d2 > 1e-4;
+    float e_lj = 0, e_re = 0;
+    const float fs = lj ? pair_fs<float, EV, MULTI, true>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re)
+                        : pair_fs<float, EV, MULTI, false>(d2, qq, ti, tj, ok, ps, pp, &e_lj,
+                                                           &e_re);
+    const double f = (double)fs;
+    fx = fma(f, dx, fx);
+    fy = fma(f, dy, fy);
+    fz = fma(f, dz, fz);
+    atomicAdd(&J.x[u], -half_grid(f * dx));
+    atomicAdd(&J.y[u], -half_grid(f * dy));
+    atomicAdd(&J.z[u], -half_grid(f * dz));
+    if (EV && ok) {
+        const double h = -f;
+        acc[0] += 2.0 * (double)e_lj;
+        acc[1] += 2.0 * (double)e_re;
+        acc[2] += h * dx * dx;
+        acc[3] += h * dy * dy;
+        acc[4] += h * dz * dz;
+        acc[5] += h * dx * dy;
+        acc[6] += h * dx * dz;
+        acc[7] += h * dy * dz;
+    }
+}
+
+// pair_wave over half rows: the same lane groups, rounds and sentinel handling;
+// each row's force on its home atom goes to the LDS sums at the end of the row.
+// Rows that overflowed scan the candidates after the home atom instead.
+template <bool EV, bool MULTI, bool STAGED>
+__device__ inline void pair_wave_half(const PairArgs& a, const HalfHdr& h,
+                                      const double4* __restrict__ lds,
+                                      const int32_t* __restrict__ ltype, int a0, int a1, int h0,
+                                      int wv, int lane, const PairScalars& ps,
+                                      const PairParams* __restrict__ pp, double* acc,
+                                      const HalfAcc& J)
+{
+    const int total = h.off[kHalfSegs];
+    const int cap = a.cap;
+    constexpr int kPer = 64 / kGroup;
+    const int g = lane / kGroup, gl = lane % kGroup;
+    const uint16_t* __restrict__ rows = a.nlist + (size_t)uniform(h0) * cap;
+    for (int t0 = a0 + kPer * wv; t0 < a1; t0 += kPer * kTileWaves) {
+        const int s = t0 + g;
+        const bool live = s < a1;
+        const int2 nc = live ? a.ncount[s] : make_int2(0, 0);
+        const int n_in = nc.x, tot = nc.x + nc.y;
+        const bool over = tot >= cap;
+        const int tl = over ? 0 : tot;
+        const int tmax = wave_max_int(tl);
+        const int nin_max = wave_max_int(over ? 0 : n_in);
+        const bool fast = __ballot(!live || over) == 0ull;
+        const int ui = live ? s - h0 : 0;
+        const double4 pi = STAGED ? lds[ui] : half_pos<false>(h, lds, a.xs, ui);
+        const int ti = MULTI ? half_type<STAGED>(h, ltype, a.ts, ui) : 0;
+        const double qi = pi.w * ps.qqr2e;
+        const uint32_t rof = (uint32_t)ui * (uint32_t)cap + (uint32_t)gl;
+        double ax = 0, ay = 0, az = 0;
+        auto rounds = [&](auto fast_tag) {
+            constexpr bool FAST = decltype(fast_tag)::value;
+            auto load_u = [&](int base, int* u) {
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) {
+                    const int v = base + kGroup * k;
+                    if (FAST)
+                        u[k] = (int)rows[rof + (uint32_t)min(v, tl - gl)];
+                    else
+                        u[k] = v + gl < tl ? (int)rows[rof + (uint32_t)v] : -1;
+                }
+            };
+            int un[kRound];
+            load_u(0, un);
+            for (int base = 0; base < tmax; base += kGroup * kRound) {
+                int u[kRound];
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) u[k] = un[k];
+                if (base + kGroup * kRound < tmax) load_u(base + kGroup * kRound, un);
+                double4 pj[kRound];
+                int tj[kRound] = {};
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) {
+                    const int uu = FAST ? u[k] : (u[k] < 0 ? total : u[k]);
+                    pj[k] = half_pos<STAGED>(h, lds, a.xs, uu);
+                    if (MULTI) tj[k] = half_type<STAGED>(h, ltype, a.ts, uu);
+                }
+#pragma unroll
+                for (int k = 0; k < kRound; ++k) {
+                    if (base + kGroup * k < tmax) {
+                        const int uu = FAST ? u[k] : (u[k] < 0 ? total : u[k]);
+                        accumulate_half<EV, MULTI>(base + kGroup * k < nin_max, pj[k].x - pi.x,
+                                                   pj[k].y - pi.y, pj[k].z - pi.z, qi * pj[k].w,
+                                                   ti, tj[k], FAST || u[k] >= 0, ps, pp, ax, ay,
+                                                   az, acc, J, uu);
+                    }
+                }
+            }
+        };
+        if (fast)
+            rounds(std::true_type{});
+        else
+            rounds(std::false_type{});
+        if (__ballot(live && over) != 0ull) {
+            for (int uu = ui + 1 + gl; uu < total + kGroup; uu += kGroup) {
+                const int uc = uu < total ? uu : total;
+                const double4 pj = half_pos<STAGED>(h, lds, a.xs, uc);
+                const int tj = MULTI ? half_type<STAGED>(h, ltype, a.ts, uc) : 0;
+                accumulate_half<EV, MULTI>(true, pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
+                                           qi * pj.w, ti, tj, live && over && uu < total, ps, pp,
+                                           ax, ay, az, acc, J, uc);
+            }
+        }
+        ax = group_sum(ax);
+        ay = group_sum(ay);
+        az = group_sum(az);
+        if (live && gl == kSumLane) {
+            atomicAdd(&J.x[ui], half_grid(ax));
+            atomicAdd(&J.y[ui], half_grid(ay));
+            atomicAdd(&J.z[ui], half_grid(az));
+        }
+    }
+}
+
+// Force planes of the half pass: [kHalfSegs][3][stride] floats, plane k holding the
+// sums a tile made for the atoms of its segment k, at their cell-sorted slots.
+struct HalfPlanes {
+    float* p;
+    size_t stride;
+};
+
+// Tile pass over half tiles (FP32): one workgroup per home cell (never split).  On
+// a rebuild step the half rows are built first (list_wave_half), then the tile is
+// staged, every pair of it evaluated once, and the LDS sums written to the planes.
+// acc_cap bounds the tiles the LDS sums hold; a larger tile sets stats[7] and is
+// skipped (the host then reports the error and returns the handle to full rows).
+template <bool EV, bool MULTI>
+struct TilePassHalf {
+    static constexpr int kThreads = kTileThreads;
+    struct Args {
+        typename TilePass<float, EV, MULTI>::Args t;
+        HalfPlanes planes;
+        int acc_cap;
+    };
+    __device__ static void run(const Args& AH, int blk, int)
+    {
+        const auto& A = AH.t;
+        const PairArgs& a = A.a;
+        extern __shared__ double4 lds[];
+        double* jbase = (double*)(lds + A.stage_cap);
+        const HalfAcc J{jbase, jbase + AH.acc_cap, jbase + 2 * (size_t)AH.acc_cap};
+        int32_t* ltype = (int32_t*)(jbase + 3 * (size_t)AH.acc_cap);
+        __shared__ HalfHdr h;
+        const int c = A.order ? A.order[blk] : blk;
+        const bool rebuild = A.flags[2] != 0u;
+        if (threadIdx.x == 0) {
+            if (blk == 0) A.flags[0] = 0u;
+            A.cell_count[c] = 0;
+        }
+        half_setup(c, A.dims, A.box, A.cell_start, h);
+        const int total = h.off[kHalfSegs];
+        const bool staged = total < A.stage_cap;
+        const int lane = threadIdx.x & 63;
+        const int wv = uniform(threadIdx.x >> 6);
+        const int h0 = h.gstart[0], hc = h.cnt[0];
+        const int a0 = h0, a1 = h0 + hc;
+        if (total + 1 > AH.acc_cap) {
+            if (threadIdx.x == 0) A.stats[7] = 1ull;
+            return;
+        }
+        if (rebuild) {
+            if (!staged && threadIdx.x == 0) atomicAdd(&A.stats[6], 1ull);
+            const RefStage rs(lds, A.stage_cap);
+            if (staged) {
+                for (int u = threadIdx.x; u < total; u += kTileThreads)
+                    rs.put(u, half_ref(h, A.xs_ref, u));
+                const int pad = ref_stage_len(total);
+                for (int u = total + (int)threadIdx.x; u < pad; u += kTileThreads)
+                    rs.put(u, h.dummy);
+            }
+            __syncthreads();
+            uint16_t* hb = hit_buffer(lds, A.stage_cap, a.cap, wv);
+            if (staged)
+                list_wave_half<true>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2,
+                                     a.cap, hb, A.nlist, A.ncount, A.stats);
+            else
+                list_wave_half<false>(h, rs, A.xs_ref, a0 + wv, a1, h0, lane, A.rl2, A.rin2,
+                                      a.cap, hb, A.nlist, A.ncount, A.stats);
+            __syncthreads();
+        }
+        if (staged) {
+            for (int u = threadIdx.x; u < total; u += kTileThreads) {
+                const int st = half_seg(h, u);
+                const int j = h.gstart[st] + (u - h.off[st]);
+                const double4 p = a.xs[j];
+                lds[u] = make_double4(p.x - h.sx[st], p.y - h.sy[st], p.z - h.sz[st], p.w);
+                if (MULTI) ltype[u] = a.ts[j];
+            }
+            if (threadIdx.x == 0) {
+                lds[total] = h.dummy;
+                if (MULTI) ltype[total] = 0;
+            }
+        }
+        for (int u = threadIdx.x; u <= total; u += kTileThreads) {
+            J.x[u] = 0.0;
+            J.y[u] = 0.0;
+            J.z[u] = 0.0;
+        }
+        __syncthreads();
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (staged)
+            pair_wave_half<EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
+                                            acc, J);
+        else
+            pair_wave_half<EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
+                                             A.pp, acc, J);
+        __syncthreads();
+        for (int u = threadIdx.x; u < total; u += kTileThreads) {
+            const int st = half_seg(h, u);
+            const size_t slot = (size_t)h.gstart[st] + (size_t)(u - h.off[st]);
+            float* pl = AH.planes.p + (size_t)st * 3 * AH.planes.stride + slot;
+            pl[0] = (float)J.x[u];
+            pl[AH.planes.stride] = (float)J.y[u];
+            pl[2 * AH.planes.stride] = (float)J.z[u];
+        }
+        if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)c * 8);
+        if (A.md) {
+            const int b = (int)(A.flags[3] & 1u);
+            if (threadIdx.x == 0) A.bins.count[b * A.bins.ncell + c] = 0;
+            for (int base = a0; base < a1; base += kTileThreads) {
+                const int s = base + (int)threadIdx.x;
+                const bool act = s < a1;
+                double p[3] = {0.0, 0.0, 0.0};
+                int v = 0;
+                if (act) {
+                    const double4 p4 = a.xs[s];
+                    p[0] = p4.x;
+                    p[1] = p4.y;
+                    p[2] = p4.z;
+                    v = a.perm[s];
+                }
+                md_bin_wave(A.bins, 1 - b, act, s, v, p, A.box, A.dims, A.cell);
+            }
+        }
+    }
+};
+
+// Per-atom sum of the 14 force planes in a fixed order (FP64), written to the
+// force arrays in original atom order.
+struct PlaneSum {
+    static constexpr int kThreads = 256;
+    struct Args {
+        HalfPlanes planes;
+        int n;
+        const int32_t* perm;
+        float *fx, *fy, *fz;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int s = blk * kThreads + (int)threadIdx.x;
+        if (s >= a.n) return;
+        const size_t st = a.planes.stride;
+        double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < kHalfSegs; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] += (double)a.planes.p[((size_t)k * 3 + c) * st + s];
+        const int i = a.perm[s];
+        a.fx[i] = (float)f[0];
+        a.fy[i] = (float)f[1];
+        a.fz[i] = (float)f[2];
+    }
+};
+
+// MD step over half tiles: TileRho with the half tile pass.
+template <bool M>
+struct TileRhoHalf {
+    static constexpr int kThreads = kTileThreads;
+    struct Args {
+        typename RhoFinalT<float, kTileThreads>::Args fin;
+        int nfin, ntile;
+        int fin_first;
+        typename TilePassHalf<false, M>::Args tile;
+    };
+    __device__ static void run(const Args& A, int blk, int)
+    {
+        const int t = A.fin_first ? blk - A.nfin : blk;
+        if (t < 0 || t >= A.ntile) {
+            RhoFinalT<float, kTileThreads>::run(A.fin, A.fin_first ? blk : blk - A.ntile, A.nfin);
+            return;
+        }
+        TilePassHalf<false, M>::run(A.tile, t, A.ntile);
+    }
+};
+
 // MD step: the tile pass and the rho(k) final reduction (nfin workgroups, one
 // k per wave) in one grid.  Both need only the previous kernel's output
 // (cell-sorted positions, rho partials), and the k-space force that follows
@@ -521,14 +941,14 @@ struct TilePass {
 // workgroup t runs on CU t mod ncu as that order assumes, and the short
 // reductions fill the slots that CUs with fewer tiles leave free; with many
 // dispatch rounds the reductions come first and finish early.
-template <typename T, bool EV, bool M>
+template <typename T, bool EV, bool M, bool RBL = false>
 struct TileRho {
     static constexpr int kThreads = kTileThreads;
     struct Args {
         typename RhoFinalT<T, kTileThreads>::Args fin;
         int nfin, ntile;
         int fin_first;
-        typename TilePass<T, EV, M>::Args tile;
+        typename TilePass<T, EV, M, RBL>::Args tile;
     };
     __device__ static void run(const Args& A, int blk, int)
     {
@@ -537,7 +957,40 @@ struct TileRho {
             RhoFinalT<T, kTileThreads>::run(A.fin, A.fin_first ? blk : blk - A.ntile, A.nfin);
             return;
         }
-        TilePass<T, EV, M>::run(A.tile, t, A.ntile);
+        TilePass<T, EV, M, RBL>::run(A.tile, t, A.ntile);
+    }
+};
+
+// Mean near force of a random batch list pass: the per-tile sums in a fixed order,
+// divided by n (ExecutionMD::ComputeRBLNearForce's corr_value,
+// rbmd/framework/run/ExecutionMD.cpp:564-565); the k-space / kick kernel that follows
+// subtracts it from every atom's near force (SumRBLCorrForceWorklet,
+// rbmd/framework/run/worklet/RunWorklet.cpp:1304-1321).
+struct RblMean {
+    static constexpr int kThreads = 256;
+    struct Args {
+        const double* part;
+        int ntile;
+        double inv_n;
+        double* mean;
+    };
+    __device__ static void run(const Args& a, int, int)
+    {
+        __shared__ double red[kThreads / 64][3];
+        const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+        double v[3] = {0, 0, 0};
+        for (int b = t; b < a.ntile; b += kThreads)
+            for (int c = 0; c < 3; ++c) v[c] += a.part[(size_t)b * 3 + c];
+        for (int c = 0; c < 3; ++c) {
+            const double w = wave_sum_dpp(v[c]);
+            if (lane == 0) red[wv][c] = w;
+        }
+        __syncthreads();
+        if (t < 3) {
+            double sum = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) sum += red[w][t];
+            a.mean[t] = sum * a.inv_n;
+        }
     }
 };
 
@@ -595,52 +1048,113 @@ PairScalars pair_scalars(const Engine& e)
     return ps;
 }
 
-template <typename T, bool EV, bool M>
-typename TilePass<T, EV, M>::Args tile_args(Engine& e, void* fx, void* fy, void* fz, bool md)
+template <typename T, bool EV, bool M, bool RBL = false>
+typename TilePass<T, EV, M, RBL>::Args tile_args(Engine& e, void* fx, void* fy, void* fz,
+                                                 bool md)
 {
-    return typename TilePass<T, EV, M>::Args{
+    RblArgs rbl{};
+    if (e.rbl) {
+        const double rc = std::max(e.rc_lj, e.rc_coul), rn = e.r_core + e.skin;
+        rbl.w = RblPair{e.r_core * e.r_core, rc * rc, (float)e.rbl_pice};
+        rbl.seed = e.rbl_seed;
+        rbl.pice = (uint32_t)e.rbl_pice;
+        rbl.near2 = rn * rn;
+        rbl.part = e.rbl_part;
+    }
+    return typename TilePass<T, EV, M, RBL>::Args{
         PairArgs{e.xs, e.ts, e.perm, e.nlist, e.ncount, e.cap},
         e.xs_ref, e.cell_start, e.split, make_int3(e.dims[0], e.dims[1], e.dims[2]), e.box,
         e.stage_cap, e.r_list * e.r_list, e.r_in * e.r_in, pair_scalars(e), e.d_pp, (T*)fx,
         (T*)fy, (T*)fz, e.ev_part, e.flags, e.stats, e.nlist, e.ncount, e.cell_count,
         md ? 1 : 0, make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e),
-        tile_order_ncu(e.ncell, e.split, e.num_cu) > 0 ? e.tile_order : nullptr};
+        tile_order_ncu(e.ncell, e.split, e.num_cu) > 0 ? e.tile_order : nullptr, rbl};
 }
 
 size_t pair_lds(const Engine& e)
 {
     const size_t stage = (size_t)e.stage_cap * (sizeof(double4) + (e.pp.ntypes > 1 ? 4 : 0));
-    const size_t build = list_lds_bytes(e.stage_cap, e.cap);
+    const size_t build = list_lds_bytes(e.stage_cap, e.cap, e.rbl);
     return stage > build ? stage : build;
 }
 
-template <typename T, bool EV, bool M>
-void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+// LDS of a half tile pass: the staged positions (and types) and the force sums, or
+// the list build's stage and hit buffers, whichever is larger.
+size_t half_lds(const Engine& e)
 {
-    k_one<TilePass<T, EV, M>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
-        tile_args<T, EV, M>(e, fx, fy, fz, false));
+    const size_t pass = (size_t)e.stage_cap * sizeof(double4) + (size_t)e.acc_cap * 24 +
+                        (e.pp.ntypes > 1 ? (size_t)e.stage_cap * 4 : 0);
+    const size_t build = list_lds_bytes(e.stage_cap, e.cap);
+    return pass > build ? pass : build;
 }
 
-template <typename T, bool M>
-typename TileRho<T, false, M>::Args tile_rho_args(Engine& e, void* fx, void* fy, void* fz,
-                                                   unsigned long long* step_dev)
+template <bool EV, bool M>
+typename TilePassHalf<EV, M>::Args half_args(Engine& e, bool md)
 {
-    typename TileRho<T, false, M>::Args a;
+    typename TilePassHalf<EV, M>::Args a;
+    a.t = tile_args<float, EV, M, false>(e, nullptr, nullptr, nullptr, md);
+    a.planes = HalfPlanes{e.planes, (size_t)std::max<int64_t>(e.n_max, 1)};
+    a.acc_cap = e.acc_cap;
+    return a;
+}
+
+void plane_sum_launch(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    k_one<PlaneSum><<<(int)((e.n + PlaneSum::kThreads - 1) / PlaneSum::kThreads),
+                      PlaneSum::kThreads, 0, s>>>(PlaneSum::Args{
+        HalfPlanes{e.planes, (size_t)std::max<int64_t>(e.n_max, 1)}, (int)e.n, e.perm,
+        (float*)fx, (float*)fy, (float*)fz});
+}
+
+template <bool EV, bool M>
+void half_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    k_one<TilePassHalf<EV, M>><<<e.ncell, kTileThreads, half_lds(e), s>>>(half_args<EV, M>(e, false));
+    plane_sum_launch(e, fx, fy, fz, s);
+}
+
+template <bool M>
+void half_rho_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    typename TileRhoHalf<M>::Args a;
+    const bool ks = has_kspace(e);
+    if (ks) a.fin = rho_final_args<float>(e, e.capturing ? e.d_step : nullptr);
+    else memset(&a.fin, 0, sizeof a.fin);
+    a.nfin = ks ? rho_final_blocks<float, kTileThreads>(e.P) : 0;
+    a.ntile = e.ncell;
+    a.fin_first = tile_order_ncu(e.ncell, 1, e.num_cu) == 0;
+    a.tile = half_args<false, M>(e, true);
+    k_one<TileRhoHalf<M>><<<a.nfin + e.ncell, kTileThreads, half_lds(e), s>>>(a);
+    plane_sum_launch(e, fx, fy, fz, s);
+}
+
+template <typename T, bool EV, bool M, bool RBL>
+void pair_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
+{
+    k_one<TilePass<T, EV, M, RBL>><<<e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(
+        tile_args<T, EV, M, RBL>(e, fx, fy, fz, false));
+}
+
+template <typename T, bool M, bool RBL = false>
+typename TileRho<T, false, M, RBL>::Args tile_rho_args(Engine& e, void* fx, void* fy, void* fz,
+                                                        unsigned long long* step_dev)
+{
+    typename TileRho<T, false, M, RBL>::Args a;
     const bool ks = has_kspace(e);
     if (ks) a.fin = rho_final_args<T>(e, step_dev);
     else memset(&a.fin, 0, sizeof a.fin);
     a.nfin = ks ? rho_final_blocks<T, kTileThreads>(e.P) : 0;
     a.ntile = e.ncell * e.split;
     a.fin_first = tile_order_ncu(e.ncell, e.split, e.num_cu) == 0;
-    a.tile = tile_args<T, false, M>(e, fx, fy, fz, true);
+    a.tile = tile_args<T, false, M, RBL>(e, fx, fy, fz, true);
     return a;
 }
 
-template <typename T, bool M>
+template <typename T, bool M, bool RBL>
 void tile_rho_launch_t(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
 {
-    const auto a = tile_rho_args<T, M>(e, fx, fy, fz, e.capturing ? e.d_step : nullptr);
-    k_one<TileRho<T, false, M>><<<a.nfin + e.ncell * e.split, kTileThreads, pair_lds(e), s>>>(a);
+    const auto a = tile_rho_args<T, M, RBL>(e, fx, fy, fz, e.capturing ? e.d_step : nullptr);
+    k_one<TileRho<T, false, M, RBL>><<<a.nfin + e.ncell * e.split, kTileThreads, pair_lds(e),
+                                       s>>>(a);
 }
 
 // Tile workgroups of a replica that fit one CU at once with its LDS stage.
@@ -700,10 +1214,26 @@ void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s
     if (e.n == 0) return;
     ScopedTimer t(e, P_PAIR, s);
     const bool multi = e.pp.ntypes > 1;
+    if (e.half) {
+        // the half-shell pass and the per-atom sum of its force planes
+        if (ev) {
+            if (multi) half_launch_t<true, true>(e, fx, fy, fz, s);
+            else half_launch_t<true, false>(e, fx, fy, fz, s);
+        } else {
+            if (multi) half_launch_t<false, true>(e, fx, fy, fz, s);
+            else half_launch_t<false, false>(e, fx, fy, fz, s);
+        }
+        return;
+    }
 #define RBMD_PAIR_M(T, EV)                                                                    \
     do {                                                                                      \
-        if (multi) pair_launch_t<T, EV, true>(e, fx, fy, fz, s);                          \
-        else pair_launch_t<T, EV, false>(e, fx, fy, fz, s);                               \
+        if (e.rbl) {                                                                          \
+            if (multi) pair_launch_t<T, EV, true, true>(e, fx, fy, fz, s);                    \
+            else pair_launch_t<T, EV, false, true>(e, fx, fy, fz, s);                         \
+        } else {                                                                              \
+            if (multi) pair_launch_t<T, EV, true, false>(e, fx, fy, fz, s);                   \
+            else pair_launch_t<T, EV, false, false>(e, fx, fy, fz, s);                        \
+        }                                                                                     \
     } while (0)
     if (e.precision == 32) {
         if (ev) RBMD_PAIR_M(float, true);
@@ -720,19 +1250,38 @@ void launch_tile_rho(Engine& e, void* fx, void* fy, void* fz, hipStream_t s)
     if (e.n == 0) return;
     ScopedTimer t(e, P_PAIR, s);
     const bool multi = e.pp.ntypes > 1;
-    if (e.precision == 32) {
-        if (multi) tile_rho_launch_t<float, true>(e, fx, fy, fz, s);
-        else tile_rho_launch_t<float, false>(e, fx, fy, fz, s);
-    } else {
-        if (multi) tile_rho_launch_t<double, true>(e, fx, fy, fz, s);
-        else tile_rho_launch_t<double, false>(e, fx, fy, fz, s);
+    if (e.half) {
+        if (multi) half_rho_launch_t<true>(e, fx, fy, fz, s);
+        else half_rho_launch_t<false>(e, fx, fy, fz, s);
+        return;
     }
+#define RBMD_TR(T, M)                                                                         \
+    do {                                                                                      \
+        if (e.rbl) tile_rho_launch_t<T, M, true>(e, fx, fy, fz, s);                           \
+        else tile_rho_launch_t<T, M, false>(e, fx, fy, fz, s);                                \
+    } while (0)
+    if (e.precision == 32) {
+        if (multi) RBMD_TR(float, true);
+        else RBMD_TR(float, false);
+    } else {
+        if (multi) RBMD_TR(double, true);
+        else RBMD_TR(double, false);
+    }
+#undef RBMD_TR
+}
+
+void launch_rbl_mean(Engine& e, hipStream_t s)
+{
+    if (!e.rbl || e.n == 0) return;
+    k_one<RblMean><<<1, RblMean::kThreads, 0, s>>>(
+        RblMean::Args{e.rbl_part, e.ncell * e.split, 1.0 / (double)e.n, e.rbl_mean});
 }
 
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s)
 {
     ScopedTimer t(e, P_EV_FINAL, s);
     const int nb = with_pair ? e.ncell * e.split : 0;  // one partial per tile workgroup
+                                                        // (split 1 for half tiles)
     const int P = (e.kspace != 0) ? e.P : 0;
     k_ev_final<<<1, 256, 0, s>>>(e.ev_part, nb, e.kev, P, e.e_self, e.ev);
 }

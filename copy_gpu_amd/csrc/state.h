@@ -47,6 +47,18 @@ struct Engine {
     Box box{};
     double rc_lj = 0, rc_coul = 0, skin = 0, alpha = 0, qqr2e = 1, ewald_tol = 1e-16;
     double r_list = 0, r_in = 0;
+    // random batch list (neighbor type RBL): core radius, shell stride, selection seed
+    bool rbl = false;
+    double r_core = 0;
+    int rbl_pice = 1;
+    uint64_t rbl_seed = 0;
+    double* rbl_part = nullptr;  // [tile workgroups][3] near-force sums of each tile
+    double* rbl_mean = nullptr;  // [4] mean near force over all atoms
+    // half-shell pair pass (FP32, full Verlet list, at least three cells per axis):
+    // every pair evaluated once; force planes [14][3][n_max] summed per atom
+    bool half = false;
+    int acc_cap = 0;             // LDS force sums of a half tile (candidates + sentinel)
+    float* planes = nullptr;
     int precision = 32, kspace = 0, p_max = 0;
     PairParams pp{};
     int64_t n_max = 0, n = 0;
@@ -190,6 +202,9 @@ enum BatchPh {
     BP_PAIR,           // rho final + tile pass (pair.hip)
     BP_KFORCE,         // k-space force + closing kick (md.hip)
     BP_KFORCE_DRIFT,   // ... fused with the next step's opening kick/drift (md.hip)
+    BP_THERMO_BEGIN,   // thermostatted replicas: state reset before the first step
+    BP_THERMO_REDUCE,  // ... temperature and coefficient after each closing kick
+    BP_THERMO_END,     // ... last velocity scale after the last step
     BP_NUM
 };
 
@@ -201,6 +216,7 @@ struct Batch {
     int precision = 32;
     int kspace = 0;
     bool multi = false;
+    bool thermo = false;  // some replica has a thermostat: one segment per step
     BatchPhase ph[BP_NUM];
     // the tile pass runs one launch per LDS class of replicas: ph[BP_PAIR] holds
     // the first class, pair_more the others (pair.hip pair_prepare_t)
@@ -251,6 +267,8 @@ void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s);
 void launch_pair(Engine& e, void* fx, void* fy, void* fz, bool ev, hipStream_t s);
 void launch_tile_rho(Engine& e, void* fx, void* fy, void* fz, hipStream_t s);
 void launch_ev_final(Engine& e, bool with_pair, hipStream_t s);
+// random batch list: the mean near force the next k-space / kick kernel subtracts
+void launch_rbl_mean(Engine& e, hipStream_t s);
 
 // ---- launchers (kspace.hip)
 SampleArgs sample_args(const Engine& e, int P, uint64_t seed, const uint64_t* step_host);

@@ -310,11 +310,12 @@ struct RefStage {
 };
 
 // LDS bytes of the list build: the reference stage plus one hit buffer of cap
-// uint16 per wave.
-inline size_t list_lds_bytes(int stage_cap, int cap)
+// uint16 per wave (and, for a random batch list, the candidates' atom ids).
+inline size_t list_lds_bytes(int stage_cap, int cap, bool rbl = false)
 {
     return 24 * (size_t)ref_stage_len(stage_cap) +
-           (size_t)kTileWaves * (size_t)cap * sizeof(uint16_t);
+           (size_t)kTileWaves * (size_t)cap * sizeof(uint16_t) +
+           (rbl ? 4 * (size_t)ref_stage_len(stage_cap) : 0);
 }
 
 __device__ inline uint16_t* hit_buffer(double4* lds, int stage_cap, int cap, int wv)
@@ -322,12 +323,39 @@ __device__ inline uint16_t* hit_buffer(double4* lds, int stage_cap, int cap, int
     return (uint16_t*)((double*)lds + 3 * (size_t)ref_stage_len(stage_cap)) + (size_t)wv * cap;
 }
 
-template <bool STAGED>
+// Atom ids of the staged candidates (random batch list builds), after the hit buffers.
+__device__ inline int32_t* stage_ids(double4* lds, int stage_cap, int cap)
+{
+    return (int32_t*)(hit_buffer(lds, stage_cap, cap, kTileWaves));
+}
+
+// Random batch list of a build: rows keep every pair closer than r_near = r_core +
+// skin (it may come inside r_core before the next build) and the shell pairs that
+// rbl_select keeps, tagged with kRblSel so the pair pass weights them.
+constexpr int kRblSel = 0x8000;
+struct RblBuild {
+    double near2;      // (r_core + skin)^2
+    uint64_t seed;
+    uint32_t epoch;    // list build number
+    uint32_t pice;
+    const int32_t* perm;  // sorted slot -> atom id
+    const int32_t* ids;   // LDS: atom id of each staged candidate (null: from perm)
+};
+
+// Atom id of tile-local candidate u (random batch list builds and their fallback).
+__device__ inline int32_t tile_atom_id(const TileHdr& h, const int32_t* __restrict__ perm, int u)
+{
+    const int st = tile_seg(h, u);
+    return perm[h.gstart[st] + (u - h.off[st])];
+}
+
+template <bool STAGED, bool RBL = false>
 __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
                                  const double4* __restrict__ xs_ref, int first, int a1, int h0,
                                  int lane, double rl2, double rin2, int cap, uint16_t* hb,
                                  uint16_t* __restrict__ nlist, int2* __restrict__ ncount,
-                                 unsigned long long* __restrict__ stats)
+                                 unsigned long long* __restrict__ stats,
+                                 const RblBuild& rb = RblBuild{})
 {
 #pragma clang fp contract(off)
     const int total = h.off[27];
@@ -378,13 +406,41 @@ __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
                                     __builtin_amdgcn_ballot_w64((rl2 - d2) > 1e-4) &
                                     __builtin_amdgcn_ballot_w64(d2 > 1e-4);
                 const int slot = n + (int)mbcnt64(mh);
-                if (hit && slot < cap) hb[slot] = (uint16_t)(d2 < rin2 ? (u | kInnerFlag) : u);
+                // tag: LJ range (full list) or the always-kept near band (random batch)
+                const double tag2 = RBL ? rb.near2 : rin2;
+                if (hit && slot < cap) hb[slot] = (uint16_t)(d2 < tag2 ? (u | kInnerFlag) : u);
                 n += __popcll(mh);
             }
         }
         uint16_t* __restrict__ row = nlist + (size_t)s * cap;
         int n_in = n;
-        if (n < cap) {
+        if (RBL && n < cap) {
+            // random batch list: the near band, and the shell pairs this build
+            // selects (tagged kRblSel), in sweep order; the pair pass evaluates LJ on
+            // every entry
+            __syncwarp();
+            const int32_t id_i = rb.perm[s];
+            int nk = 0;
+            for (int k0 = 0; k0 < n; k0 += 64) {
+                const int k = k0 + lane;
+                const int e = k < n ? (int)hb[k] : 0;
+                const int u = e & kMaxTile;
+                // every hit draws its selection: a near-band entry kept for being near
+                // counts only if selected should it lie in the shell when evaluated
+                bool sel = false;
+                if (k < n) {
+                    const int32_t id_j = rb.ids ? rb.ids[u] : tile_atom_id(h, rb.perm, u);
+                    sel = rbl_select(rb.seed, rb.epoch, id_i, id_j, rb.pice);
+                }
+                const bool keep = k < n && ((e & kInnerFlag) != 0 || sel);
+                const uint64_t mk = __ballot(keep);
+                if (keep) row[nk + (int)mbcnt64(mk)] = (uint16_t)(sel ? (u | kRblSel) : u);
+                nk += __popcll(mk);
+            }
+            if (lane == 0) row[nk] = (uint16_t)total;
+            __syncwarp();
+            n_in = n = nk;
+        } else if (n < cap) {
             __syncwarp();  // the hit buffer, written by every lane of this wave
             n_in = 0;
             for (int k0 = 0; k0 < n; k0 += 64) {

@@ -18,6 +18,7 @@ librbmd_force.so.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -60,6 +61,12 @@ class ForceConfig:
     sigma: tuple = (1.0,)
     qqr2e: float = 1.0
     ewald_tol: float = 1e-16
+    kmax: int = 2                 # "coulomb.kmax": lattice cube of the thermo far energy
+    neighbor_type: str = "VERLET-LIST"  # "neighbor.type": "VERLET-LIST" | "RBL"
+    r_core: float = 0.0           # "neighbor.r_core" (RBL core radius)
+    rbl_pice: int = 1             # RBL shell stride (rbl_pice_num of "neighbor_sample_num")
+    rbl_seed: int = 0             # RBL shell selection stream
+    full_list: bool = False       # True: every pair from both sides (no half-shell rows)
     n_max: int = 0
     list_capacity: int = 0        # Verlet row capacity; 0 = automatic
     tile_stage: int = 0           # LDS stage of a cell tile (atoms); 0 = automatic
@@ -116,6 +123,12 @@ class LJCutCoulKspaceForce:
         p.ewald_tol = cfg.ewald_tol
         p.list_capacity = cfg.list_capacity
         p.tile_stage = cfg.tile_stage
+        rbl = cfg.neighbor_type.upper() == "RBL"
+        p.neighbor_type = L.NEIGHBOR_RBL if rbl else L.NEIGHBOR_VERLET
+        p.r_core = float(cfg.r_core)
+        p.rbl_pice = int(cfg.rbl_pice)
+        p.rbl_seed = int(cfg.rbl_seed)
+        p.full_list = 1 if cfg.full_list else 0
         return p, (eps, sig)
 
     # ------------------------------------------------------------ lifecycle
@@ -143,6 +156,7 @@ class LJCutCoulKspaceForce:
         if atom_type is not None:
             assert atom_type.is_cuda and atom_type.dtype == torch.int32
         self.n = charge.numel()
+        self.charge = charge  # kept for the handle-free k-space step (thermo far energy)
         self._check(self.lib.rbmd_force_set_atoms(self.h, self.n, _ptr(charge), _ptr(atom_type),
                                                   _stream(stream)), "set_atoms")
 
@@ -273,3 +287,44 @@ class LJCutCoulKspaceForce:
                                          _ptr(vz), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(img),
                                          _ptr(mass), dt, nsteps, seed, step0,
                                          1 if use_graph else 0, st), "md_run")
+
+
+def rbl_pice_num(n: int, box_len, cut_off: float, r_core: float, neighbor_sample_num: float):
+    """The random batch list's shell stride as the reference derives it from
+    "neighbor_sample_num" (ExecutionMD::ComputeRBLNearForce,
+    rbmd/framework/run/ExecutionMD.cpp:476-484): density rho = n / V (FP32 `Real`),
+    coeff = round(1 + 0.05 / rho - 0.05), counts c(r) = trunc(coeff rho ceil(4/3 pi r^3) + 1),
+    pice = ceil(1 / (neighbor_sample_num / (c(cut_off) - c(r_core))))."""
+    V = float(box_len[0]) * float(box_len[1]) * float(box_len[2])
+    rho = float(np.float32(n / V))
+    coeff = float(np.floor(1.0 + (0.05 / rho - 0.05) + 0.5))
+
+    def count(r):
+        return int(coeff * rho * math.ceil(4.0 / 3.0 * float(np.float32(math.pi)) * r ** 3) + 1)
+
+    rate = float(neighbor_sample_num) / float(count(cut_off) - count(r_core))
+    return max(1, int(math.ceil(1.0 / rate)))
+
+
+def kspace_compute(box_len, alpha, x, y, z, q, m, w=None, fx=None, fy=None, fz=None,
+                   box_lo=(0.0, 0.0, 0.0), qqr2e=1.0, precision=64, stream=None):
+    """The k-space step of a given k-list without a force handle (rbmd_kspace_compute):
+    returns (rho[P, 2] FP64, ev[7] = E_k, W_xx, W_yy, W_zz, W_xy, W_xz, W_yz), and adds
+    the k-space force to fx, fy, fz when given (scratch arrays otherwise).
+    m: int32 device tensor (3P); w: FP64 device tensor (P) or None (S/P)."""
+    lib = L.lib()
+    n, P = x.numel(), m.numel() // 3
+    dt = torch.float32 if precision == 32 else torch.float64
+    if fx is None:
+        fx, fy, fz = (torch.zeros(n, device=x.device, dtype=dt) for _ in range(3))
+    rho = torch.zeros(2 * P, device=x.device, dtype=torch.float64)
+    ev = torch.zeros(7, device=x.device, dtype=torch.float64)
+    prm = L.KspaceParams((C.c_double * 3)(*[float(v) for v in box_lo]),
+                         (C.c_double * 3)(*[float(v) for v in box_len]), float(alpha),
+                         float(qqr2e), int(precision))
+    rc = lib.rbmd_kspace_compute(C.byref(prm), n, _ptr(x), _ptr(y), _ptr(z), _ptr(q), P, _ptr(m),
+                                 _ptr(w), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(rho), _ptr(ev),
+                                 _stream(stream))
+    if rc != 0:
+        raise RBMDError(f"rbmd_kspace_compute -> {rc}: {lib.rbmd_kspace_last_error().decode()}")
+    return rho.view(P, 2), ev

@@ -22,7 +22,7 @@ import re
 import numpy as np
 
 from . import _lib as L
-from .force import ForceConfig, RBMDError
+from .force import ForceConfig, RBMDError, rbl_pice_num
 
 
 def _p(a):
@@ -136,9 +136,36 @@ def load_config(path: str, n_max: int = 0, skin: float = 0.3, coulomb_cut: float
         run["inbuild_dims"] = tuple(int(d) for d in ib.get("dims", (10, 10, 10)))
     if box_len is None:
         raise RBMDError("load_config: no init_configuration (read_data or inbuild)")
+    rbl = {}
+    if str(nb.get("type", "")).upper() == "RBL":
+        # random batch list: core radius and the shell stride the reference derives
+        # from neighbor_sample_num at the system's density
+        n_atoms = run["data"]["n"] if "data" in run else int(np.prod(run["inbuild_dims"]))
+        r_core = float(nb.get("r_core", rc / 2))
+        rbl = {"neighbor_type": "RBL", "r_core": r_core,
+               "rbl_pice": rbl_pice_num(n_atoms, box_len, rc, r_core,
+                                        float(nb.get("neighbor_sample_num", 50)))}
+        run["neighbor_sample_num"] = float(nb.get("neighbor_sample_num", 50))
     fc = ForceConfig(box_len=box_len, box_lo=box_lo, cut_off=rc,
                      coulomb_cut=rc if coulomb_cut is None else float(coulomb_cut), skin=skin,
                      alpha=float(cl.get("alpha", 0.5)), kspace=kspace,
                      coulomb_sample_num=int(cl.get("coulomb_sample_num", 100)),
+                     kmax=int(cl.get("kmax", 2)), **rbl,
                      precision=precision, eps=eps, sigma=sigma, n_max=n_max)
     return fc, run, cfg
+
+
+def inbuild_system(run: dict, box_lo, box_len):
+    """The reference's in-built initial configuration ("init_configuration.inbuild"):
+    one atom at the centroid of every cell of a dims grid over the box, x fastest
+    (MeshFreeCondition::InitPosition, rbmd/framework/InitCondition/MeshFreeCondition.cpp:58-80,
+    the cell centroids of a uniform data set), with charge -1 on the first half of the
+    atoms and +1 on the rest (ExecutionMD::InitialCondition,
+    rbmd/framework/run/ExecutionMD.cpp:155-168).  Returns (x, y, z, q)."""
+    dims = run["inbuild_dims"]
+    ax = [box_lo[d] + (np.arange(dims[d]) + 0.5) * (box_len[d] / dims[d]) for d in range(3)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    n = x.size
+    q = np.full(n, -1.0)
+    q[n // 2:] = 1.0
+    return x.ravel().copy(), y.ravel().copy(), z.ravel().copy(), q

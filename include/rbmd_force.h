@@ -29,7 +29,7 @@ extern "C" {
 #define RBMD_OK 0
 #define RBMD_ERR_INVALID (-1)   /* bad argument or parameter                    */
 #define RBMD_ERR_HIP (-2)       /* HIP runtime error                            */
-#define RBMD_ERR_BOX (-3)       /* box shorter than 3 list radii on some axis   */
+#define RBMD_ERR_BOX (-3)       /* box shorter than one list radius on some axis */
 #define RBMD_ERR_PRECISION (-4) /* force dtype does not match handle precision */
 #define RBMD_ERR_STATE (-5)     /* call order (e.g. compute before set_atoms)   */
 #define RBMD_ERR_CAPACITY (-6)  /* n or P above the sizes given at create       */
@@ -39,6 +39,9 @@ extern "C" {
 #define RBMD_KSPACE_EWALD 2 /* full Ewald lattice sum (FP64 physics oracle)   */
 
 #define RBMD_EV 1           /* compute_flags: accumulate energy and virial    */
+
+#define RBMD_NEIGHBOR_VERLET 0 /* full Verlet list (reference "VERLET-LIST")     */
+#define RBMD_NEIGHBOR_RBL 1    /* random batch list: core + sampled shell ("RBL") */
 
 /* Parameters.  Key names follow the reference JSON (res/json_files/LJ_salt/rbmd.json:
  * cut_off, alpha, coulomb_sample_num, ...). */
@@ -60,6 +63,27 @@ typedef struct {
     int32_t list_capacity; /* Verlet row capacity (multiple of 64); 0 = automatic */
     int32_t tile_stage;    /* LDS stage of a cell tile, in atoms (>= 64); 0 = automatic.
                               Tiles with more candidates read global memory instead. */
+    /* Near-field list ("neighbor.type").  RBL (the random batch list of
+     * ComputeRBLNeighboursOnceWorklet / ComputeNearForceRBLWorklet,
+     * rbmd/framework/run/worklet/RunWorklet.cpp:634-708, 957-1048, with the mean-force
+     * correction of ExecutionMD::ComputeRBLNearForce, run/ExecutionMD.cpp:460-566):
+     * pairs closer than r_core count fully; each pair in the shell
+     * r_core < r < max(rc_lj, rc_coul) is kept with probability 1/rbl_pice and then
+     * counts rbl_pice times (the reference keeps every pice_num-th shell pair of its
+     * sweep); then the mean of the near forces over all atoms is subtracted.  The
+     * shell pairs of atom i are chosen by Philox4x32-10 (key rbl_seed, counter
+     * (id_i, id_j, list build number, 0x52424C53)), word 0 mod rbl_pice == 0, drawn
+     * again at every list build. */
+    int32_t neighbor_type; /* RBMD_NEIGHBOR_*                                     */
+    double r_core;         /* RBL core radius ("r_core")                           */
+    int32_t rbl_pice;      /* RBL shell stride (pice_num, >= 1)                    */
+    uint64_t rbl_seed;     /* RBL shell selection stream                           */
+    /* 0: with FP32 maths, the full Verlet list and at least three cells per axis,
+     * the pair pass evaluates every pair once (half-shell rows, Newton's third law)
+     * and adds the force to both atoms; otherwise, and with 1, every row holds all
+     * neighbours and each pair is evaluated from both sides as the reference does.
+     * Forces agree to FP32 rounding; neighbour lists and energies are the same. */
+    int32_t full_list;
 } rbmd_force_params;
 
 /* FP64 energies (LJ units x qqr2e for Coulomb terms) and virial
@@ -178,8 +202,10 @@ int rbmd_md_set_temperature_log(rbmd_force_t h, double* log, int64_t len);
 /* The Langevin noise vector g (3 doubles) of a noise step, on the host. */
 int rbmd_langevin_noise_host(uint64_t seed, uint64_t step, double* g);
 
-/* Current full Verlet list in ORIGINAL atom ids as CSR (HOST arrays):
- * offsets[n+1], idx[cap]; rows unsorted.  Returns the entry count in *count
+/* Current Verlet list in ORIGINAL atom ids as CSR (HOST arrays):
+ * offsets[n+1], idx[cap]; rows unsorted.  An atom j appears once per periodic image
+ * of j within the list radius (several times only in boxes narrower than two list
+ * radii).  A random batch list holds the near band and the selected shell pairs.  Returns the entry count in *count
  * (if larger than cap only the offsets are written).  Synchronising.
  * Replaces the list of FullNeighborListBuilder::Build
  * (rbmd-gpu src/neighbor_list/src/neighbor_list_builder/full_neighbor_list_builder.cpp:31-45). */
@@ -222,6 +248,35 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
  * closing kick (and the next step's kick/drift). */
 int rbmd_force_set_timing(rbmd_force_t h, int32_t enable);
 int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int64_t* launches);
+
+/* ---- the k-space step without a force handle ----
+ *
+ * Structure factor rho(k) = sum_j q_j exp(i k.r_j) over a given k-list and the
+ * k-space force F_i += q_i sum_k c_k k (cos(k.r_i) Im rho_k - sin(k.r_i) Re rho_k),
+ * c_k = -4 pi w_k / (V k^2) qqr2e, with the kernels of rbmd_force_compute.  Replaces
+ * ExecutionMD::ComputeRBEEleForce with ComputeNewChargeStructureFactorRBE
+ * (rbmd/framework/run/ExecutionMD.cpp:389-448) and RunWorklet::ComputeNewRBEForce
+ * (rbmd/framework/run/worklet/RunWorklet.cpp:1508-1548, 1616-1677).
+ * x, y, z, q: FP64 device arrays (n); m: int32 device (3P, m != 0); w: FP64 device
+ * (P) or NULL (the RBE weight S/P); fx, fy, fz: float* (precision 32) or double*
+ * (64) device arrays the k-space force is ADDED to; rho: FP64 device (2P: Re, Im
+ * per k) or NULL; ev: FP64 device (7: E_k, W_xx, W_yy, W_zz, W_xy, W_xz, W_yz) or
+ * NULL.  Error text: rbmd_kspace_last_error(). */
+typedef struct {
+    double box_lo[3];
+    double box_len[3];
+    double alpha;       /* exp(-k^2 / 4 alpha) convention, as rbmd_force_params */
+    double qqr2e;       /* Coulomb unit factor (1 in LJ units)               */
+    int32_t precision;  /* 32: FP32 sines and force sums, 64: FP64          */
+} rbmd_kspace_params;
+
+/* Allocates its scratch (device memory) for the call and synchronises the stream. */
+int rbmd_kspace_compute(const rbmd_kspace_params* p, int64_t n, const double* x, const double* y,
+                        const double* z, const double* q, int32_t P, const int32_t* m,
+                        const double* w, void* fx, void* fy, void* fz, double* rho, double* ev,
+                        void* stream);
+
+const char* rbmd_kspace_last_error(void);
 
 /* Plain device copy of nbytes (multiple of 16), used to calibrate HBM
  * traffic counters against a kernel of known traffic. */

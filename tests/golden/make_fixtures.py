@@ -16,6 +16,9 @@ committed outputs are data only, so the tests never read the reference:
                        sorted by atom id.
   lj_salt_6000.data.gz that example data file itself, gzip-compressed, as the input of
                        the data-file reader's parity test.
+  lj_salt_rbmd.json    the reference's example run input rbmd/res/json_files/LJ_salt/rbmd.json
+                       (in-built lattice, random batch list, RBE, Berendsen NVT), byte for
+                       byte, as the input of the JSON front end's end-to-end test.
 """
 from __future__ import annotations
 
@@ -85,7 +88,17 @@ def lj_salt():
     return len(atoms)
 
 
+def example_json():
+    src = os.path.join(REF, "rbmd", "res", "json_files", "LJ_salt", "rbmd.json")
+    with open(src, "rb") as f:
+        data = f.read()
+    with open(os.path.join(OUT, "lj_salt_rbmd.json"), "wb") as f:
+        f.write(data)
+    return len(data)
+
+
 if __name__ == "__main__":
     print("gnear rows:", gnear())
     print("lj_salt atoms:", lj_salt())
+    print("lj_salt_rbmd.json bytes:", example_json())
     sys.exit(0)

@@ -24,8 +24,11 @@ def _dev(a, dtype=torch.float64):
 def _cfg(L, n, kspace="RBE", P=100, precision=32):
     from copy_gpu_amd import ForceConfig
 
+    # full rows, the pair pass the batched tile pass uses (single handles would
+    # otherwise take the half-shell pass, whose forces differ in the last bits)
     return ForceConfig(box_len=(L, L, L), cut_off=2.5, coulomb_cut=5.0, alpha=0.5, skin=0.3,
-                       kspace=kspace, coulomb_sample_num=P, precision=precision, n_max=n)
+                       kspace=kspace, coulomb_sample_num=P, precision=precision, n_max=n,
+                       full_list=True)
 
 
 def _systems(sizes, seed0=100):
@@ -53,7 +56,12 @@ def _prime(f, st, P, seed, kspace):
     f.Execute(*st[:3], *st[6:9])
 
 
-def _run_single(sysr, seed, nsteps, kspace, P, use_graph, proposal=None):
+def _thermo(f, kind, seed):
+    if kind is not None:
+        f.set_thermostat(kind, (1.2, 1.2, 0.05), seed=seed, gamma=10.0)
+
+
+def _run_single(sysr, seed, nsteps, kspace, P, use_graph, proposal=None, thermostat=None):
     from copy_gpu_amd import LJCutCoulKspaceForce
 
     x, y, z, q, L, vx, vy, vz = sysr
@@ -61,6 +69,7 @@ def _run_single(sysr, seed, nsteps, kspace, P, use_graph, proposal=None):
     f.Init(_dev(q))
     if proposal is not None:
         proposal.attach(f)
+    _thermo(f, thermostat, seed)
     st = _state(sysr, f.fdtype)
     _prime(f, st, P, seed, kspace)
     f.md_run(*st[:9], dt=0.002, nsteps=nsteps, seed=seed, step0=1, img=st[9],
@@ -70,11 +79,14 @@ def _run_single(sysr, seed, nsteps, kspace, P, use_graph, proposal=None):
     if kspace == "RBE":
         m, w, _, _ = f.kspace()
         out += [m, w]
+    if thermostat is not None:
+        th = f.thermostat_state()
+        out += [np.array([th["T"], th["xi"], th["noise_step"]])]
     f.close()
     return out
 
 
-def _run_batch(syss, seeds, nsteps, kspace, P, use_graph, proposals=None):
+def _run_batch(syss, seeds, nsteps, kspace, P, use_graph, proposals=None, thermostats=None):
     from copy_gpu_amd.batch import ReplicaBatch
 
     b = ReplicaBatch([_cfg(s[4], s[0].size, kspace, P) for s in syss])
@@ -83,6 +95,8 @@ def _run_batch(syss, seeds, nsteps, kspace, P, use_graph, proposals=None):
         f.Init(_dev(sysr[3]))
         if proposals and proposals[r] is not None:
             proposals[r].attach(f)
+        if thermostats:
+            _thermo(f, thermostats[r], seeds[r])
         st = _state(sysr, f.fdtype)
         _prime(f, st, P, seeds[r], kspace)
         states.append(st)
@@ -90,11 +104,14 @@ def _run_batch(syss, seeds, nsteps, kspace, P, use_graph, proposals=None):
              use_graph=use_graph)
     torch.cuda.synchronize()
     outs = []
-    for st, f in zip(states, b.replicas):
+    for r, (st, f) in enumerate(zip(states, b.replicas)):
         o = [t.cpu().numpy() for t in st]
         if kspace == "RBE":
             m, w, _, _ = f.kspace()
             o += [m, w]
+        if thermostats and thermostats[r] is not None:
+            th = f.thermostat_state()
+            o += [np.array([th["T"], th["xi"], th["noise_step"]])]
         outs.append(o)
     stats = [f.stats() for f in b.replicas]
     b.close()
@@ -113,6 +130,29 @@ def test_batch_rbe_equals_single_runs(gpu, use_graph):
         ref = _run_single(sysr, seeds[r], 40, "RBE", 100, use_graph=True)
         for a, b_ in zip(outs[r], ref):
             assert np.array_equal(a, b_), f"replica {r} differs"
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_thermostats_equal_single_runs(gpu, use_graph):
+    """Replicas with different NVT thermostats (and one without) in one batch: each
+    takes bit-identical steps to the same handle advanced alone by md_run with its
+    thermostat (ExecutionNVT::Solve + UpdateVelocityByTempConType,
+    rbmd/framework/run/ExecutionNVT.cpp:133-159, 256-291), including the final
+    temperature, Nose-Hoover friction and Langevin noise step."""
+    sizes = [2048, 3500, 2300, 4100, 2700]
+    kinds = ["BERENDSEN", None, "NOSE_HOOVER", "LANGEVIN", "TEMP_RESCALE"]
+    _log(f"case: batch thermostats {kinds} on replicas {sizes}, 20 steps, graph={use_graph}")
+    syss = _systems(sizes, seed0=500)
+    seeds = [SEED + 11 * r for r in range(len(sizes))]
+    outs, _ = _run_batch(syss, seeds, 20, "RBE", 100, use_graph, thermostats=kinds)
+    for r, sysr in enumerate(syss):
+        ref = _run_single(sysr, seeds[r], 20, "RBE", 100, use_graph=True, thermostat=kinds[r])
+        assert len(outs[r]) == len(ref)
+        for a, b_ in zip(outs[r], ref):
+            assert np.array_equal(a, b_), f"replica {r} ({kinds[r]}) differs"
+    # the thermostats acted: velocities of the Berendsen replica differ from NVE
+    nve = _run_single(syss[0], seeds[0], 20, "RBE", 100, use_graph=True)
+    assert not np.array_equal(outs[0][3], nve[3])
 
 
 def test_batch_lj_only_equals_single_runs(gpu):

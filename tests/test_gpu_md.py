@@ -441,13 +441,26 @@ def test_thermo_output_row_matches_oracle(orc, gpu):
     p = [t.cpu().numpy() for t in (X, Y, Z)]
     v = [t.cpu().numpy() for t in (VX, VY, VZ)]
     _, evp = orc.pair_forces(*p, q, np.zeros(3), Lb, 2.5, 5.0, alpha)
-    m, w = _halfspace_lattice(Lb, alpha, tol)
-    _, evk = orc.kspace_forces(*p, q, Lb, alpha, m, w)
     e_self = orc.self_energy(q, alpha)
     assert abs(r["VanderWaalsEnergy"] - evp[0] / n) < 1e-11 * abs(evp[0] / n)
     assert abs(r["NearCoulombicEnergy"] - evp[1] / n) < 1e-11 * abs(evp[1] / n)
+    # far energy: the reference's Kmax cube (ThermoOutput.cpp:218-252), default Kmax 2,
+    # every m != 0 with weight exp(-k^2 / 4 alpha) -- not the handle's own k-space
+    r2 = np.arange(-2, 3)
+    cube = np.stack(np.meshgrid(r2, r2, r2, indexing="ij"), -1).reshape(-1, 3)
+    cube = cube[(cube != 0).any(1)].astype(np.int32)
+    kk = (2 * np.pi * cube / Lb) ** 2
+    _, evk = orc.kspace_forces(*p, q, Lb, alpha, cube, np.exp(-kk.sum(1) / (4 * alpha)))
     far = (evk[0] + e_self) / n
     assert abs(r["FarCoulombicEnergy"] - far) < 1e-10 * abs(far)
+    # an RBE handle reports the same deterministic far energy at these positions
+    g = _make(Lb, n, 32, "RBE", alpha=alpha)
+    g.Init(_dev(q))
+    g.sample_kvectors(100, SEED, 0)
+    g.set_thermostat("MEASURE")
+    rg = thermo.thermo_row(g, X, Y, Z, 20, dt)
+    assert abs(rg["FarCoulombicEnergy"] - far) < 1e-10 * abs(far)
+    assert abs(thermo.far_kspace_energy(f, X, Y, Z, kmax=3) / n - far + e_self / n) > 0
     ke = 0.5 * sum(float(np.dot(mass * c, c)) for c in v) / n
     assert abs(r["KinticEnergy"] - ke) < 1e-12 * ke
     assert abs(r["Temperature"] - orc.temperature(v, mass)) < 1e-12 * r["Temperature"]

@@ -29,14 +29,14 @@ def _dev(a, dtype=torch.float64):
 
 def _make(Lb, n, precision=32, kspace="NONE", P=100, rc_lj=2.5, rc_coul=5.0, alpha=0.5,
           skin=0.3, lo=(0.0, 0.0, 0.0), eps=(1.0,), sigma=(1.0,), list_capacity=0,
-          ewald_tol=1e-16, tile_stage=0):
+          ewald_tol=1e-16, tile_stage=0, full_list=False):
     from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
 
     cfg = ForceConfig(box_len=tuple(float(v) for v in Lb), box_lo=lo, cut_off=rc_lj,
                       coulomb_cut=rc_coul, skin=skin, alpha=alpha, kspace=kspace,
                       coulomb_sample_num=P, precision=precision, eps=eps, sigma=sigma,
                       n_max=n, list_capacity=list_capacity, ewald_tol=ewald_tol,
-                      tile_stage=tile_stage)
+                      tile_stage=tile_stage, full_list=full_list)
     return LJCutCoulKspaceForce(cfg)
 
 
@@ -152,6 +152,46 @@ def test_pair_forces_and_energy(orc, gpu, precision, tol):
     assert abs(ev["e_real"] - evo[1]) < 10 * tol * abs(evo[1])
     W = ev["virial"]
     assert np.linalg.norm(W - evo[2:8]) < 10 * tol * np.linalg.norm(evo[2:8])
+
+
+@pytest.mark.parametrize("types", [False, True])
+def test_half_shell_rows_equal_full_rows(orc, gpu, types):
+    """FP32 handles evaluate each pair once over half-shell tiles (home cell + 13
+    forward cells) and add the force to both atoms; with full rows every pair is
+    evaluated from both sides.  Same neighbour sets, forces within FP32 rounding,
+    energies and virial to 1e-6, both against the oracle at its FP32 bars; and over
+    tiles read from global memory (LDS stage 64) the same bits as staged tiles."""
+    _log(f"case: half-shell vs full rows, FP32, 12000 ions, two types {types}")
+    x, y, z, q, L = _ions(12000, seed=31)
+    Lb = np.array([L] * 3)
+    kw = dict(eps=(1.0, 0.6), sigma=(1.0, 1.3)) if types else {}
+    tt = (np.arange(x.size) % 3 == 0).astype(np.int32) if types else None
+    out = {}
+    for full, stage in ((False, 0), (True, 0), (False, 64)):
+        f = _make(Lb, x.size, **kw, full_list=full, tile_stage=stage)
+        f.Init(_dev(q), None if tt is None else _dev(tt, torch.int32))
+        F = _compute(f, x, y, z, ev=True)
+        out[(full, stage)] = (F, f.energy_virial(), f.neighbors(), f.stats())
+        f.close()
+    Fh, evh, (oh, ih), sh = out[(False, 0)]
+    Ff, evf, (of, if_), sf = out[(True, 0)]
+    assert sh["entries"] == sf["entries"]
+    assert np.array_equal(oh, of)
+    for i in range(0, x.size, 7):
+        assert np.array_equal(np.sort(ih[oh[i]:oh[i + 1]]), np.sort(if_[of[i]:of[i + 1]]))
+    assert _rel(Fh, Ff) < 2e-6, _rel(Fh, Ff)
+    for k in ("e_lj", "e_real"):
+        assert abs(evh[k] - evf[k]) < 1e-6 * abs(evf[k])
+    assert np.linalg.norm(evh["virial"] - evf["virial"]) < 1e-6 * np.linalg.norm(evf["virial"])
+    xw, yw, zw = orc.wrap(x, y, z, np.zeros(3), Lb)
+    Fo, evo = orc.pair_forces(xw, yw, zw, q, np.zeros(3), Lb, 2.5, 5.0, 0.5, types=tt,
+                              eps=kw.get("eps", (1.0,)), sigma=kw.get("sigma", (1.0,)))
+    assert _rel(Fh, Fo) < 1e-5
+    # the global-memory tiles: the same bits (order-independent LDS sums)
+    Fg, evg, _, sg = out[(False, 64)]
+    assert sg["big_tiles"] > 0
+    assert np.array_equal(Fg, Fh)
+    assert evg["e_real"] == evh["e_real"]
 
 
 def test_pair_forces_two_types(orc, gpu):
@@ -443,8 +483,15 @@ def test_errors_and_empty(gpu):
     from copy_gpu_amd import RBMDError
 
     _log("case: error paths")
-    with pytest.raises(RBMDError, match="3 list radii"):
-        _make(np.array([10.0, 10.0, 10.0]), 10)
+    # a box narrower than one list radius (5.3) on some axis: the stencil would miss
+    # images (boxes of one or two list radii are supported, test_gpu_rbl)
+    with pytest.raises(RBMDError, match="one list radius"):
+        _make(np.array([10.0, 5.0, 10.0]), 10)
+    with pytest.raises(RBMDError, match="r_core"):
+        from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
+
+        LJCutCoulKspaceForce(ForceConfig(box_len=(20.0,) * 3, n_max=10, neighbor_type="RBL",
+                                         r_core=6.0, rbl_pice=3))
     f = _make(np.array([20.0] * 3), 100, kspace="RBE", P=10)
     with pytest.raises(RBMDError, match="set_atoms"):
         _compute(f, np.zeros(0), np.zeros(0), np.zeros(0))
