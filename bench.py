@@ -1,22 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: atom-steps/s of the charged-LJ + Random Batch Ewald MD step on one MI355X.
+"""Benchmark: atom-steps/s of the charged-LJ + Random Batch Ewald MD step on one MI355X,
+at fixed force RMS error against FP64 Ewald, learned vs plain RBE.
 
-Workload (default): 32,768 +/-1 ions, uniform random with minimum separation
-0.8, density 0.3 (L = 47.80), LJ cutoff 2.5, erfc real-space cutoff 5.0,
-alpha 0.5, P = 100 k-vectors resampled every step from the RBE Gaussian,
-Verlet skin 0.3, T = 1, dt = 0.002; FP32 pair and k-space maths, FP64
-positions, velocities, integrator.  A step = kick, drift, list check (and
-rebuild when needed), k-sampling, structure factor, pair pass, k-space force,
-kick -- captured once in a HIP graph and replayed.
+Workload (default): 100,000 +/-1 ions, uniform random with minimum separation
+0.8, density 0.3 (L = 69.34), LJ cutoff 2.5, erfc real-space cutoff 5.0,
+alpha 0.5, Verlet skin 0.3, T = 1, dt = 0.002; FP32 pair and k-space maths, FP64
+positions, velocities, integrator.  The k-space proposal network (4-64-64-1 MLP)
+is trained on the GPU on FP64 |rho(k)|^2 of 50 MD snapshots, then learned P = 50
+is compared with the analytic sampler at P = 50/100/200/400: force RMS error
+against FP64 full Ewald (5 held-out configurations x 20 k-lists each) and
+atom-steps/s of each.  The headline value is the learned P = 50 run.  A step =
+kick, drift, list check (and rebuild when needed), k-sampling, structure factor,
+pair pass, k-space force, kick -- captured in HIP graphs (eight steps and one
+step) before the timed region and replayed.
 
 Prints ONE JSON line on stdout (logs go to stderr).  Also reported:
-  * force RMS error vs FP64 full Ewald on the final configuration, over 20
-    independent k-lists (total force and k-space part);
-  * roofline of the pair pass (dominant HBM-bound kernel): algorithmic bytes
-    4 E + 40 N per launch over the mean launch time from HIP events on its
+  * roofline of the pair pass (the dominant kernel): algorithmic bytes
+    4 E + 40 N per launch over its mean launch time from HIP events on its
     stream (E = full-list entries at r_list), against 8 TB/s;
-  * the CPU baseline: the reference-faithful FP32 step of the oracle on the
-    host cores for a bounded sample of steps.
+  * the CPU baseline: the reference-faithful step on the host cores for a bounded
+    sample of steps of the same workload;
+  * beside the headline: the 32,768-ion plain-RBE workload (timed steps, pair-pass
+    roofline, Berendsen NVT) and the pair-pass roofline of the 1,048,576-ion one.
 """
 from __future__ import annotations
 
@@ -38,7 +43,7 @@ SEED = 20261202
 
 WORKLOADS = {
     "c1": dict(n=4000, fcc=10, rho=0.8442, rc_lj=2.5, rc_coul=0.0, alpha=0.0, P=0, dt=0.005,
-               steps=1000, kspace="NONE", precision=64, ewald=False, cpu=False,
+               steps=1000, kspace="NONE", precision=64, ewald=False,
                desc="plain LJ fluid, FP64: 4000 atoms on an FCC lattice at density 0.8442, "
                     "rc 2.5, skin 0.3, T 1.0, dt 0.005, no charges"),
     "c2": dict(n=32768, rho=0.3, rc_lj=2.5, rc_coul=5.0, alpha=0.5, P=100, dt=0.002,
@@ -90,12 +95,17 @@ def cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, target_s):
 
     n, P = wl["n"], wl["P"]
     threads = max(1, min(16, os.cpu_count() or 1))
-    pos = np.ascontiguousarray(np.stack([x, y, z], 1), np.float32)
-    vel = np.ascontiguousarray(np.stack([vx, vy, vz], 1), np.float32)
+    real = np.float64 if wl.get("precision", 32) == 64 else np.float32
+    pos = np.ascontiguousarray(np.stack([x, y, z], 1), real)
+    vel = np.ascontiguousarray(np.stack([vx, vy, vz], 1), real)
     frc = np.zeros_like(pos)
+    if wl.get("kspace", "RBE") == "NONE":
+        q = None
     from copy_gpu_amd import systems
 
     def klist(steps, s0):
+        if P == 0:
+            return None
         return np.concatenate([systems.sample_kvectors_host(wl["alpha"], [L] * 3, P, SEED,
                                                             s0 + s)[0] for s in range(steps)])
 
@@ -119,10 +129,13 @@ def cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, target_s):
     oracle.faithful_run(pos, vel, frc, q, lo, Lb, wl["rc_lj"], wl["rc_coul"], wl["alpha"], P,
                         m1, steps1, wl["dt"], 1)
     el1 = time.perf_counter() - t
+    prec = "FP64" if real == np.float64 else "FP32"
+    what = ("N*P structure-factor arrays, analytic sampler at the same P" if P else
+            "LJ only")
     return {"value": n * steps / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "single_thread_value": n * steps1 / el1, "host_cpus": os.cpu_count(),
-            "sample": f"{steps} reference-faithful FP32 steps (per-step cell sort, list at rc "
-                      f"without skin, N*P structure-factor arrays) of the same {n}-ion "
+            "sample": f"{steps} reference-faithful {prec} steps (per-step counting sort by "
+                      f"cell, list at rc without skin, {what}) of the same {n}-atom "
                       f"workload, OpenMP over atoms, {el:.1f} s; then {steps1} steps on one "
                       f"thread, {el1:.1f} s"}
 
@@ -151,12 +164,20 @@ class State:
                  nsteps=nsteps, seed=SEED, step0=step0, use_graph=use_graph)
 
 
+# MD steps whose HIP graphs the library captures on first use: eight steps and one
+# step (the remainder); a warm-up of at least GRAPH_WARMUP steps captures both
+GRAPH_WARMUP = 9
+
+
 def timed(f, st, wl, W, K, step, use_graph):
-    """W warm-up then K timed MD steps; returns (seconds, list builds, next step)."""
+    """W warm-up steps (at least GRAPH_WARMUP with graphs, so both cached graphs are
+    captured and instantiated before the clock starts), then K timed MD steps;
+    returns (seconds, list builds, next step)."""
     import torch
 
-    st.run(f, wl, W, step, use_graph)
-    step += W
+    Wr = max(W, GRAPH_WARMUP) if use_graph else W
+    st.run(f, wl, Wr, step, use_graph)
+    step += Wr
     torch.cuda.synchronize()
     b0 = f.stats()["builds"]
     t0 = time.perf_counter()
@@ -373,7 +394,7 @@ def learned_eval(wl, L, Q, st, step, args):
         s2 = base.clone()
         g.sample_kvectors(P, SEED, step)
         g.Execute(s2.X, s2.Y, s2.Z, *s2.F)
-        el, rebuilds, _ = timed(g, s2, wl, 20, K, step + 1, not args.no_graph)
+        el, rebuilds, _ = timed(g, s2, wl, args.warmup, K, step + 1, not args.no_graph)
         rate = n * K / el
         log(f"{kind:8s} P={P:4d}: eps_F {eps if eps is None else f'{eps:.4e}'} "
             f"(k-space part {epsk if epsk is None else f'{epsk:.4e}'}), "
@@ -523,20 +544,131 @@ def batch_eval(wl, args):
             "replica_sizes": sizes, "eps_F_samples": errs, "proposal_train_s": t_train}
 
 
+def make_state(wl, dev):
+    """Starting configuration of a workload: positions, charges, velocities."""
+    import torch
+
+    from copy_gpu_amd import systems
+
+    n = wl["n"]
+    if wl.get("fcc"):
+        x, y, z, L = systems.fcc(wl["fcc"], wl["rho"])
+        q = np.zeros(n)
+    else:
+        x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED)
+    vx, vy, vz = systems.velocities(n, 1.0, seed=SEED)
+    t64 = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to(dev, torch.float64)  # noqa
+    fdt = torch.float64 if wl.get("precision", 32) == 64 else torch.float32
+    st = State(*(t64(a) for a in (x, y, z, vx, vy, vz)),
+               [torch.zeros(n, device=dev, dtype=fdt) for _ in range(3)])
+    return st, t64(q), L, (x, y, z, q, vx, vy, vz)
+
+
+def roofline_of(f, st, wl, wname, K, step, measured_limiter):
+    """Pair-pass roofline of a handle's current state plus per-kernel ms per step."""
+    import torch
+
+    per_step_ms, pair_bytes, pair_ms, E, step = kernel_times(f, st, wl, min(K, 100), step)
+    achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
+    valu = read_traffic(f"pair_{wname}", "valu_wave_instructions_per_launch")
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    traffic = read_traffic(f"pair_{wname}")
+    roof = {
+        "kernel": "pair",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "traffic_source": (f"profiles/pair_{wname}_traffic.txt (rocprofv3 --pmc FETCH_SIZE and "
+                           "WRITE_SIZE of the same pair pass, scaled by the counters' reading "
+                           "of a copy kernel of known size)") if traffic else None,
+        "algorithmic_bytes_per_launch": pair_bytes,
+        "mean_launch_ms": pair_ms,
+        "list_entries": E,
+        "measured_limiter": measured_limiter,
+        "valu_issue_frac": (valu / (pair_ms * 1e-3 * n_cu * ENGINE_CLOCK_HZ)) if valu else None,
+        "valu_issue_note": "SQ_INSTS_VALU per launch (profiles/pair_<workload>_traffic.txt) / "
+                           "(mean launch time x CUs x 2.4 GHz x 1 wave64 instruction per "
+                           "CU-cycle); an indicator of VALU issue, not a bound",
+    }
+    return roof, per_step_ms, max(per_step_ms, key=per_step_ms.get), step
+
+
+PAIR_LIMITER = ("VALU issue, LDS gathers and latency together (the bytes moved equal the "
+                "algorithmic bytes, so the pass does not reach the HBM roofline)")
+
+
+def plain_leg(wname, args, W, K, use_graph, with_nvt=False):
+    """A workload with the analytic sampler (or no k-space): timed MD steps, the
+    pair-pass roofline, eps_F against FP64 Ewald on the final configuration."""
+    import torch
+
+    wl = WORKLOADS[wname]
+    n, P = wl["n"], wl["P"]
+    dev = torch.device("cuda:0")
+    precision = wl.get("precision", 32)
+    kspace = wl.get("kspace", "RBE")
+    log(f"workload {wname}: {n} atoms")
+    st, Q, L, host = make_state(wl, dev)
+    f = make_force(wl, L, P, kspace=kspace, precision=precision)
+    f.Init(Q)
+    if kspace == "RBE":
+        f.sample_kvectors(P, SEED, 0)
+    f.Execute(st.X, st.Y, st.Z, *st.F)
+    if args.tune:
+        t = time.perf_counter()
+        shapes = f.autotune(st.X, st.Y, st.Z, *st.F)
+        log(f"autotune ({time.perf_counter() - t:.2f} s): {shapes}")
+    if args.thermostat:
+        f.set_thermostat(args.thermostat, (1.0, 1.0, 0.5))
+    log(f"warmup {max(W, GRAPH_WARMUP)} steps, timed {K} steps (graph={use_graph}, "
+        f"thermostat={args.thermostat or 'none'})")
+    el, rebuilds, step = timed(f, st, wl, W, K, 1, use_graph)
+    value = n * K / el
+    log(f"timed {K} steps: {el * 1e3:.2f} ms -> {value:.4e} atom-steps/s, "
+        f"{rebuilds} list builds")
+    log(f"list/tile stats: {f.stats()}")
+    roof, per_step_ms, dominant, step = roofline_of(f, st, wl, wname, K, step, PAIR_LIMITER)
+    eps_total = eps_k = None
+    if not args.skip_ewald and wl.get("ewald", True):
+        log("FP64 Ewald reference on the final configuration")
+        refs, nkv = ewald_refs(wl, L, Q, [(st.X, st.Y, st.Z)])
+        eps_total, eps_k = force_error(f, P, [(st.X, st.Y, st.Z)], refs, 20)
+        log(f"force RMS error vs FP64 Ewald: total {eps_total:.3e}, relative to k-space "
+            f"part {eps_k:.3e} ({nkv} half-space k-vectors)")
+    out = {"workload": wl["desc"], "value": value, "unit": "atom-steps/s",
+           "ms_per_step": el * 1e3 / K, "steps": K, "list_builds_in_timed_steps": rebuilds,
+           "force_rms_err_vs_fp64_ewald": eps_total,
+           "force_rms_err_vs_fp64_ewald_kspace_part": eps_k,
+           "roofline": roof, "per_kernel_ms_per_step": per_step_ms,
+           "dominant_kernel": dominant}
+    if with_nvt:
+        log("NVT (Berendsen) run of the same steps")
+        out["nvt_berendsen"] = nvt_leg(f, st, wl, W, K, step, use_graph)
+        log(f"NVT: {out['nvt_berendsen']['value']:.4e} atom-steps/s, "
+            f"{out['nvt_berendsen']['ms_per_step'] * 1e3:.1f} us per step")
+    return out, f, st, host, L
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ewald", action="store_true")
     ap.add_argument("--skip-scale", action="store_true",
-                    help="with the default workload, skip the legs reported beside it: the "
-                         "pair-pass roofline of the 1,048,576-ion workload (c3) and the NVT "
-                         "(Berendsen) run")
+                    help="skip the legs reported beside the headline: the 32,768-ion "
+                         "workload (timed steps, roofline, Berendsen NVT) and the pair-pass "
+                         "roofline of the 1,048,576-ion workload")
+    ap.add_argument("--pair-only", action="store_true",
+                    help="only the pair-pass roofline of the workload's starting "
+                         "configuration (20 force calls; for counter collection)")
     ap.add_argument("--thermostat", default=None,
                     choices=["BERENDSEN", "TEMP_RESCALE", "NOSE_HOOVER", "LANGEVIN"],
                     help="time the MD steps under this NVT thermostat (temperature 1.0, "
@@ -555,8 +687,6 @@ def main():
 
     import torch
 
-    from copy_gpu_amd import systems
-
     if args.copy_calibration:
         import ctypes
 
@@ -572,160 +702,101 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 x 256 MiB copied")
 
+    if args.pair_only:
+        print(json.dumps({"pair_pass": pair_pass_at_scale(args.workload)}), flush=True)
+        return
+
     wl = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else wl["steps"]
     W = args.warmup
+    use_graph = not args.no_graph
+    base = {"metric": METRIC, "unit": "atom-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic"}
     if wl.get("batch"):
         res = batch_eval(wl, args)
-        out = {
-            "metric": METRIC,
-            "value": res["value"],
-            "unit": "atom-steps/s",
-            "n_gpus": 1,
-            "steps": K,
-            "warmup": W,
-            "ms_per_step": res["ms_per_batch_step"],
-            "higher_is_better": True,
-            "scaling": "weak",
-            "vs_baseline": None,
-            "dtype": "fp32",
-            "data": "synthetic",
-            "config": {"workload": wl["desc"], "n_atoms_total": res["total_atoms"],
-                       "replicas": len(res["replica_sizes"]), "P": wl["P"],
-                       "kspace": "learned MLP proposal per replica (mixture 0.9/0.1)",
-                       "hip_graph": not args.no_graph, "parallelism": "single GPU"},
-            "batch": {k: v for k, v in res.items() if k not in ("value",)},
-            "roofline": None,
-            "cpu_baseline": None,
-        }
+        out = dict(base, value=res["value"], ms_per_step=res["ms_per_batch_step"],
+                   dtype="fp32",
+                   config={"workload": wl["desc"], "n_atoms_total": res["total_atoms"],
+                           "replicas": len(res["replica_sizes"]), "P": wl["P"],
+                           "kspace": "learned MLP proposal per replica (mixture 0.9/0.1)",
+                           "hip_graph": use_graph, "parallelism": "single GPU"},
+                   batch={k: v for k, v in res.items() if k not in ("value",)},
+                   roofline=None, cpu_baseline=None)
         print(json.dumps(out), flush=True)
         return
-    n, P = wl["n"], wl["P"]
+
     dev = torch.device("cuda:0")
     precision = wl.get("precision", 32)
     kspace = wl.get("kspace", "RBE")
-    if wl.get("fcc"):
-        log(f"workload {args.workload}: FCC lattice of {n} atoms")
-        x, y, z, L = systems.fcc(wl["fcc"], wl["rho"])
-        q = np.zeros(n)
-    else:
-        log(f"workload {args.workload}: generating {n} ions")
-        x, y, z, q, L = systems.random_ions(n, wl["rho"], 0.8, seed=SEED)
-    vx, vy, vz = systems.velocities(n, 1.0, seed=SEED)
-    t64 = lambda a: torch.as_tensor(np.ascontiguousarray(a)).to(dev, torch.float64)  # noqa
-    Q = t64(q)
-    fdt = torch.float64 if precision == 64 else torch.float32
-    st = State(*(t64(a) for a in (x, y, z, vx, vy, vz)),
-               [torch.zeros(n, device=dev, dtype=fdt) for _ in range(3)])
-    use_graph = not args.no_graph
-    learned = None
     if wl.get("learned"):
+        st, Q, L, host = make_state(wl, dev)
         learned, (f, st), prop, step = learned_eval(wl, L, Q, st, 1, args)
         lr = learned["runs"][0]
-        value, el, rebuilds = lr["atom_steps_per_s"], lr["ms_per_step"] * K / 1e3, \
-            lr["list_builds"]
+        value, ms_step, rebuilds = lr["atom_steps_per_s"], lr["ms_per_step"], lr["list_builds"]
         eps_total, eps_k = lr["eps_F"], lr["eps_F_kspace_part"]
+        log(f"list/tile stats: {f.stats()}")
+        roof, per_step_ms, dominant, step = roofline_of(f, st, wl, args.workload, K, step,
+                                                        PAIR_LIMITER)
+        leg = None
     else:
-        f = make_force(wl, L, P, kspace=kspace, precision=precision)
-        f.Init(Q)
-        if kspace == "RBE":
-            f.sample_kvectors(P, SEED, 0)
-        f.Execute(st.X, st.Y, st.Z, *st.F)
-        if args.tune:
-            t = time.perf_counter()
-            shapes = f.autotune(st.X, st.Y, st.Z, *st.F)
-            log(f"autotune ({time.perf_counter() - t:.2f} s): {shapes}")
-        if args.thermostat:
-            f.set_thermostat(args.thermostat, (1.0, 1.0, 0.5))
-        log(f"warmup {W} steps, timed {K} steps (graph={use_graph}, "
-            f"thermostat={args.thermostat or 'none'})")
-        el, rebuilds, step = timed(f, st, wl, W, K, 1, use_graph)
-        value = n * K / el
-        log(f"timed {K} steps: {el * 1e3:.2f} ms -> {value:.4e} atom-steps/s, "
-            f"{rebuilds} list builds")
-    log(f"list/tile stats: {f.stats()}")
-
-    per_step_ms, pair_bytes, pair_ms, E, step = kernel_times(f, st, wl, min(K, 100), step)
-    achieved = pair_bytes / (pair_ms * 1e-3) / 1e9
-    dominant = max(per_step_ms, key=per_step_ms.get)
-
-    if not learned:
-        eps_total = eps_k = None
-        if not args.skip_ewald and wl.get("ewald", True):
-            log("FP64 Ewald reference on the final configuration")
-            refs, nkv = ewald_refs(wl, L, Q, [(st.X, st.Y, st.Z)])
-            eps_total, eps_k = force_error(f, P, [(st.X, st.Y, st.Z)], refs, 20)
-            log(f"force RMS error vs FP64 Ewald: total {eps_total:.3e}, relative to k-space "
-                f"part {eps_k:.3e} ({nkv} half-space k-vectors)")
-
-    valu = read_traffic(f"pair_{args.workload}", "valu_wave_instructions_per_launch")
-    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
-    valu_frac = valu / (pair_ms * 1e-3 * n_cu * ENGINE_CLOCK_HZ) if valu else None
+        leg, f, st, host, L = plain_leg(args.workload, args, W, K, use_graph,
+                                        with_nvt=(args.workload == "c2" and not args.skip_scale
+                                                  and not args.thermostat))
+        learned = None
+        value, ms_step = leg["value"], leg["ms_per_step"]
+        rebuilds = leg["list_builds_in_timed_steps"]
+        eps_total = leg["force_rms_err_vs_fp64_ewald"]
+        eps_k = leg["force_rms_err_vs_fp64_ewald_kspace_part"]
+        roof, per_step_ms, dominant = (leg["roofline"], leg["per_kernel_ms_per_step"],
+                                       leg["dominant_kernel"])
+    n, P = wl["n"], wl["P"]
     cpu = None
     if not args.skip_cpu and wl.get("cpu", True):
         log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
+        x, y, z, q, vx, vy, vz = host
         cpu = cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, args.cpu_seconds)
         log(f"CPU baseline: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} threads")
+    f.close()
+    del st
+    torch.cuda.synchronize()
 
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "atom-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": el * 1e3 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp64" if precision == 64 else "fp32",
-        "data": "synthetic",
-        "config": {
-            "workload": wl["desc"],
-            "n_atoms": n,
-            "kspace": ("learned MLP proposal (mixture 0.9 learned / 0.1 analytic)"
-                       if learned else "plain RBE (analytic Gaussian sampler)"
-                       if kspace == "RBE" else "none (LJ only)"),
-            "P": P,
-            "precision": ("FP64 throughout" if precision == 64 else
-                          "FP32 pair and k-space maths; FP64 positions, integrator, energy"),
-            "hip_graph": use_graph,
-            "thermostat": args.thermostat,
-            "list_builds_in_timed_steps": rebuilds,
-            "force_rms_err_vs_fp64_ewald": eps_total,
-            "force_rms_err_vs_fp64_ewald_kspace_part": eps_k,
-            "parallelism": "single GPU",
-        },
-        "roofline": {
-            "kernel": "pair",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(f"pair_{args.workload}"),
-            "algorithmic_bytes_per_launch": pair_bytes,
-            "mean_launch_ms": pair_ms,
-            "list_entries": E,
-            "valu_issue_frac": valu_frac,
-            "valu_issue_note": "SQ_INSTS_VALU per launch (profiles/pair_<workload>_traffic.txt) "
-                               "/ (mean launch time x CUs x 2.4 GHz x 1 wave64 instruction per "
-                               "CU-cycle, the FP64 and unpacked FP32 rate); some instruction "
-                               "kinds issue faster, so this is an indicator, not a bound; the "
-                               "pass is bound by VALU issue, LDS gathers and latency",
-        },
-        "per_kernel_ms_per_step": per_step_ms,
-        "dominant_kernel": dominant,
-        "cpu_baseline": cpu,
+    out = dict(base, value=value, ms_per_step=ms_step,
+               dtype="fp64" if precision == 64 else "fp32")
+    out["config"] = {
+        "workload": wl["desc"],
+        "n_atoms": n,
+        "kspace": ("learned MLP proposal (mixture 0.9 learned / 0.1 analytic)"
+                   if learned else "plain RBE (analytic Gaussian sampler)"
+                   if kspace == "RBE" else "none (LJ only)"),
+        "P": P,
+        "precision": ("FP64 throughout" if precision == 64 else
+                      "FP32 pair and k-space maths; FP64 positions, integrator, energy"),
+        "hip_graph": use_graph,
+        "graph_capture_before_timing": use_graph,
+        "warmup_steps_run": max(W, GRAPH_WARMUP) if use_graph else W,
+        "thermostat": args.thermostat,
+        "list_builds_in_timed_steps": rebuilds,
+        "force_rms_err_vs_fp64_ewald": eps_total,
+        "force_rms_err_vs_fp64_ewald_kspace_part": eps_k,
+        "parallelism": "single GPU",
     }
+    out["roofline"] = roof
+    out["per_kernel_ms_per_step"] = per_step_ms
+    out["dominant_kernel"] = dominant
+    out["cpu_baseline"] = cpu
     if learned:
         out["learned_vs_plain"] = learned
-    if args.workload == "c2" and not args.skip_scale and not args.thermostat:
-        log("NVT (Berendsen) run of the same steps")
-        out["nvt_berendsen"] = nvt_leg(f, st, wl, W, K, step, use_graph)
-        log(f"NVT: {out['nvt_berendsen']['value']:.4e} atom-steps/s, "
-            f"{out['nvt_berendsen']['ms_per_step'] * 1e3:.1f} us per step")
-        # the pair pass at scale, where launch latency no longer dominates it
+    if leg and "nvt_berendsen" in leg:
+        out["nvt_berendsen"] = leg["nvt_berendsen"]
+    if args.workload == "c4" and not args.skip_scale:
+        # beside the headline: the 32,768-ion plain-RBE workload (with its Berendsen NVT
+        # leg) and the pair pass of the 1,048,576-ion workload, where the pass is long
+        # enough that launch and staging latency no longer dominate it
+        c2, fc, _, _, _ = plain_leg("c2", args, W, WORKLOADS["c2"]["steps"], use_graph,
+                                    with_nvt=True)
+        fc.close()
+        out["c2_plain_rbe"] = c2
         out["roofline_c3_pair_pass"] = pair_pass_at_scale("c3")
     print(json.dumps(out), flush=True)
 
