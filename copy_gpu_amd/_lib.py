@@ -54,7 +54,7 @@ class ForceParams(C.Structure):
         ("r_core", C.c_double),
         ("rbl_pice", C.c_int32),
         ("rbl_seed", C.c_uint64),
-        ("full_list", C.c_int32),
+        ("half_shell", C.c_int32),
     ]
 
 

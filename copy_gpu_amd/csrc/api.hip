@@ -355,9 +355,9 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
         return RBMD_ERR_BOX;
     }
     E.ncell = E.dims[0] * E.dims[1] * E.dims[2];
-    // the half-shell pair pass: FP32, full Verlet list, every neighbouring cell pair
-    // distinct (three or more cells per axis), unless the caller asks for full rows
-    E.half = p->full_list == 0 && p->precision == 32 && !E.rbl && E.dims[0] >= 3 &&
+    // the half-shell pair pass, when the caller asks for it: FP32, full Verlet list,
+    // every neighbouring cell pair distinct (three or more cells per axis)
+    E.half = p->half_shell != 0 && p->precision == 32 && !E.rbl && E.dims[0] >= 3 &&
              E.dims[1] >= 3 && E.dims[2] >= 3;
     // pair parameters
     PairParams& pp = E.pp;
@@ -381,9 +381,11 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     // capacity: 1.35x the mean row at the density of a full handle, >= 128
     const double V = E.box.L[0] * E.box.L[1] * E.box.L[2];
     const double mean = (double)n_max / V * 4.0 / 3.0 * M_PI * pow(E.r_list, 3);
-    // half-shell rows hold half the neighbours, plus up to the home cell's population
-    // more or less depending on the atom's place in its cell
-    int cap = E.half ? (int)ceil(1.35 * 0.5 * mean + 48.0) : (int)ceil(1.35 * mean + 32.0);
+    // half-shell rows hold half the neighbours on average, but not at most: the row of
+    // an atom at the forward corner of its cell holds the 7/8 of its sphere that lies in
+    // forward cells plus the later home atoms of the last 1/8, so they get the same
+    // capacity as full rows
+    int cap = (int)ceil(1.35 * mean + 32.0);
     cap = std::max(128, ((cap + 63) / 64) * 64);
     if (p->list_capacity > 0) cap = ((p->list_capacity + 63) / 64) * 64;
 
@@ -429,7 +431,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
         // (a cell beyond it is gathered by a scan instead)
         const double mean = (double)N / (double)E.ncell;
         E.bucket_cap = std::min(2048, std::max(64, ((int)(4.0 * mean) + 32 + 63) / 64 * 64));
-        CR(dalloc(&E.md_count, (size_t)2 * E.ncell * 4));
+        CR(dalloc(&E.md_count, md_count_bytes(E.ncell)));
         CR(dalloc(&E.md_bucket, (size_t)2 * E.ncell * E.bucket_cap * 4));
         CR(dalloc(&E.md_cell, N * 4));
         CR(dalloc(&E.md_vid, N * 4));
@@ -469,7 +471,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     CR(hipMemset(E.ev, 0, 16 * 8));
     CR(hipMemset(E.cell_start, 0, ((size_t)E.ncell + 1) * 4));
     CR(hipMemset(E.cell_count, 0, (size_t)E.ncell * 4));
-    CR(hipMemset(E.md_count, 0, (size_t)2 * E.ncell * 4));
+    CR(hipMemset(E.md_count, 0, md_count_bytes(E.ncell)));
     CR(hipMemset(E.xr, 0, N * 8));
     CR(hipMemset(E.yr, 0, N * 8));
     CR(hipMemset(E.zr, 0, N * 8));
@@ -1334,7 +1336,7 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
         rbmd_force_t h = nullptr;
         // replicas share the batched full-row tile pass
         rbmd_force_params pr = p;
-        pr.full_list = 1;
+        pr.half_shell = 0;
         const int rc = rbmd_force_create(&pr, n_max[r], &h);
         if (rc != RBMD_OK) {
             rbmd_batch_destroy(b);
@@ -1443,7 +1445,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     BHIP_TRY(md_batch_prepare(B));
     // bins of the current positions for a rebuild in the first step
     for (Engine* E : B.es)
-        BHIP_TRY(hipMemsetAsync(E->md_count, 0, (size_t)2 * E->ncell * sizeof(int32_t), s));
+        BHIP_TRY(hipMemsetAsync(E->md_count, 0, md_count_bytes(E->ncell), s));
     nlist_batch_launch(B, BP_MDBIN, s);
     if (B.thermo) md_batch_launch(B, BP_THERMO_BEGIN, s);
     if (!use_graph) {

@@ -51,13 +51,16 @@ struct PairParams {
 // MD-step bins: an atom is binned under its cell-sorted slot at the binning, so
 // the per-atom records are written in slot order (whole cache lines from a tile's
 // consecutive home atoms) rather than scattered by atom id.
+constexpr int kBinChunk = 256;  // cells per chunk count (the cell sort's prefix sums)
+
 struct MdBins {
     int32_t* count;        // [2][ncell]
+    int32_t* chunk;        // [2][nchunk]: sums of count over kBinChunk consecutive cells
     int32_t* bucket;       // [2][ncell][cap] slots at the binning
     int32_t* cell;         // [n] by slot: cell at the binning
     int32_t* vid;          // [n] by slot: atom id
     double *xw, *yw, *zw;  // [n] by slot: wrapped positions at the binning
-    int ncell, cap;
+    int ncell, cap, nchunk;
 };
 
 // ---------------------------------------------------------------- Philox4x32-10

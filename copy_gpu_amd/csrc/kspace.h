@@ -11,9 +11,11 @@
 //   Ewald force / energy   ExecForceFunction.h:226-245, output/ThermoOutput.cpp:209-255
 //
 // Phases are formed from FP64 fractional coordinates, phi = frac(m . s) with
-// s = (x - lo)/L, so FP32 sines keep full accuracy at any |k.r|.  FP32 mode
-// accumulates rho with Kahan compensation per lane and reduces the per-block
-// partials in FP64, deterministically.
+// s = (x - lo)/L, so FP32 sines keep full accuracy at any |k.r|.  FP32 mode sums
+// each lane's products q cos, q sin over blocks of kRhoFlush atoms in FP32 and
+// adds every block sum into an FP64 accumulator (the compensation: no rounding
+// error grows past one block), and reduces the per-workgroup partials in FP64,
+// deterministically.
 #pragma once
 #include "state.h"
 
@@ -49,27 +51,26 @@ __device__ inline void sincos2pi<double>(double phi, double* s, double* c)
 __device__ inline float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ inline double fma_t(double a, double b, double c) { return fma(a, b, c); }
 
-// Kahan accumulator in T (used for float; a no-op compensation in double).
-// add_prod(a, b) adds a*b, the product and the compensation in one FMA.
-template <typename T>
-struct Kahan {
-    T s = 0, c = 0;
-    __device__ inline void add_prod(T a, T b)
-    {
-        const T y = fmaf(a, b, -c);
-        const T t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ inline double value() const { return (double)s - (double)c; }
-};
+// Atoms per block of a lane's T-precision rho sums before the block is added into
+// FP64.  In FP32 a block of 16 unit-size products carries an error of a few 1e-7 at
+// most; the FP64 sum of the blocks adds none that matters.
+constexpr int kRhoFlush = 16;
 
-template <>
-struct Kahan<double> {
-    double s = 0;
-    __device__ inline void add_prod(double a, double b) { s = fma(a, b, s); }
-    __device__ inline double value() const { return s; }
-};
+// Charge in the staged atom record of the rho partials: FP64, or in FP32 mode the
+// FP32 charge's bits in the low word (read back without a conversion per k).
+template <typename T>
+__device__ inline double rho_stage_q(double q)
+{
+    if constexpr (sizeof(T) == 4) return __hiloint2double(0, __float_as_int((float)q));
+    return q;
+}
+
+template <typename T>
+__device__ inline T rho_staged_q(double w)
+{
+    if constexpr (sizeof(T) == 4) return __int_as_float(__double2loint(w));
+    return w;
+}
 
 // ---------------------------------------------------------------- rho partials
 // Workgroups = (atom tiles) x (k tiles of 64), atom tile fastest.  Lane <-> k
@@ -112,7 +113,7 @@ struct RhoPartial {
             my = (double)A.km[3 * kidx + 1];
             mz = (double)A.km[3 * kidx + 2];
         }
-        Kahan<T> re, im;
+        double re = 0.0, im = 0.0;
         const int a0 = bx * A.atoms_per_block;
         int a1 = a0 + A.atoms_per_block;
         a1 = a1 < A.n ? a1 : A.n;
@@ -124,26 +125,34 @@ struct RhoPartial {
                 if (i < a1)
                     st[threadIdx.x] = make_double4((A.x[i] - box.lo[0]) * box.invL[0],
                                                    (A.y[i] - box.lo[1]) * box.invL[1],
-                                                   (A.z[i] - box.lo[2]) * box.invL[2], A.q[i]);
+                                                   (A.z[i] - box.lo[2]) * box.invL[2],
+                                                   rho_stage_q<T>(A.q[i]));
                 else
                     st[threadIdx.x] = make_double4(0, 0, 0, 0);
             }
             __syncthreads();
             const int cnt = (a1 - base) < kRhoStage ? (a1 - base) : kRhoStage;
             if (kok) {
-                for (int a = wv; a < cnt; a += kRhoBlock / 64) {
-                    const double4 p = st[a];
-                    const double phi = fma(mz, p.z, fma(my, p.y, mx * p.x));
-                    T sn, cs;
-                    sincos2pi<T>(phi, &sn, &cs);
-                    const T qa = (T)p.w;
-                    re.add_prod(qa, cs);
-                    im.add_prod(qa, sn);
+                constexpr int kStep = kRhoBlock / 64;
+                for (int b0 = wv; b0 < cnt; b0 += kStep * kRhoFlush) {
+                    const int b1 = (b0 + kStep * kRhoFlush) < cnt ? (b0 + kStep * kRhoFlush) : cnt;
+                    T br = 0, bi = 0;
+                    for (int a = b0; a < b1; a += kStep) {
+                        const double4 p = st[a];
+                        const double phi = fma(mz, p.z, fma(my, p.y, mx * p.x));
+                        T sn, cs;
+                        sincos2pi<T>(phi, &sn, &cs);
+                        const T qa = rho_staged_q<T>(p.w);
+                        br = fma_t(qa, cs, br);
+                        bi = fma_t(qa, sn, bi);
+                    }
+                    re += (double)br;
+                    im += (double)bi;
                 }
             }
         }
-        red[wv][lane][0] = re.value();
-        red[wv][lane][1] = im.value();
+        red[wv][lane][0] = re;
+        red[wv][lane][1] = im;
         __syncthreads();
         if (wv == 0 && kok) {
             double r = 0.0, m = 0.0;

@@ -338,8 +338,13 @@ __device__ inline void cellsort_md_cell(const CellSortMDArgs& a, int c, int lane
     if (req == 0u || c >= a.ncell) return;
     const int b = (int)(a.flags[3] & 1u);
     const int32_t* count = a.bins.count + (size_t)b * a.ncell;
+    const int32_t* chunk = a.bins.chunk + (size_t)b * a.bins.nchunk;
+    // start = the counts of the whole chunks before c, then of the cells of c's own
+    // chunk before c: at most ncell / kBinChunk + kBinChunk loads per cell
+    const int c0 = (c / kBinChunk) * kBinChunk;
     int part = 0;
-    for (int k = lane; k < c; k += 64) part += count[k];
+    for (int k = lane; k < c / kBinChunk; k += 64) part += chunk[k];
+    for (int k = c0 + lane; k < c; k += 64) part += count[k];
     const int s0 = wave_sum(part);
     const int cnt = count[c];
     if (lane == 0) {

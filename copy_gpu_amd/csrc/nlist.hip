@@ -232,10 +232,14 @@ struct MdBin {
     __device__ static void run(const Args& a, int blk, int)
     {
         const int i = blk * kBlock + threadIdx.x;
-        if (i >= a.n) return;
-        const double p[3] = {a.x[i], a.y[i], a.z[i]};
-        // any one-to-one slot numbering serves the cell sort: atom i bins in slot i
-        md_bin_atom(a.bins, (int)(a.flags[3] & 1u), i, i, p, a.box, a.dims, a.cell);
+        const int buf = (int)(a.flags[3] & 1u);
+        int c = -1;
+        if (i < a.n) {
+            const double p[3] = {a.x[i], a.y[i], a.z[i]};
+            // any one-to-one slot numbering serves the cell sort: atom i bins in slot i
+            c = md_bin_atom(a.bins, buf, i, i, p, a.box, a.dims, a.cell);
+        }
+        md_bin_chunks_wave(a.bins, buf, i < a.n, c);
     }
 };
 
@@ -397,7 +401,7 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
 
 void launch_md_bin(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
 {
-    (void)hipMemsetAsync(e.md_count, 0, (size_t)2 * e.ncell * sizeof(int32_t), s);
+    (void)hipMemsetAsync(e.md_count, 0, md_count_bytes(e.ncell), s);
     k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z));
 }
 

@@ -578,7 +578,10 @@ struct TilePass {
             // bins of the next rebuild: clear this step's buffer (consumed by the
             // cell sort) for this cell, bin the current home atoms into the other
             const int b = (int)(A.flags[3] & 1u);
-            if (part == 0 && threadIdx.x == 0) A.bins.count[b * A.bins.ncell + c] = 0;
+            if (part == 0 && threadIdx.x == 0) {
+                A.bins.count[b * A.bins.ncell + c] = 0;
+                if (c % kBinChunk == 0) A.bins.chunk[b * A.bins.nchunk + c / kBinChunk] = 0;
+            }
             for (int base = a0; base < a1; base += kTileThreads) {
                 const int s = base + (int)threadIdx.x;
                 const bool act = s < a1;
@@ -867,7 +870,10 @@ struct TilePassHalf {
         if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)c * 8);
         if (A.md) {
             const int b = (int)(A.flags[3] & 1u);
-            if (threadIdx.x == 0) A.bins.count[b * A.bins.ncell + c] = 0;
+            if (threadIdx.x == 0) {
+                A.bins.count[b * A.bins.ncell + c] = 0;
+                if (c % kBinChunk == 0) A.bins.chunk[b * A.bins.nchunk + c / kBinChunk] = 0;
+            }
             for (int base = a0; base < a1; base += kTileThreads) {
                 const int s = base + (int)threadIdx.x;
                 const bool act = s < a1;

@@ -92,7 +92,7 @@ struct Engine {
     // one dispatch round (rebuilt with the cells; identity until then)
     int32_t* tile_order = nullptr;
     // bins of the MD step's next rebuild (double-buffered by the epoch flags[3])
-    int32_t* md_count = nullptr;   // [2][ncell]
+    int32_t* md_count = nullptr;   // [2][ncell] per-cell counts, then [2][nchunk] chunk counts
     int32_t* md_bucket = nullptr;  // [2][ncell][bucket_cap]
     int32_t* md_cell = nullptr;    // [n] by slot at binning
     int32_t* md_vid = nullptr;     // [n] by slot at binning: atom id
@@ -247,11 +247,20 @@ inline double thermo_dof(const Engine& e)
     return e.thermo_dof > 0 ? e.thermo_dof : 3.0 * (double)e.n - 3.0;
 }
 
+inline int md_nchunk(int ncell) { return (ncell + kBinChunk - 1) / kBinChunk; }
+
 inline MdBins md_bins(const Engine& e)
 {
     const size_t n = (size_t)std::max<int64_t>(e.n_max, 1);
-    return MdBins{e.md_count, e.md_bucket, e.md_cell,   e.md_vid,    e.md_w,
-                  e.md_w + n,  e.md_w + 2 * n, e.ncell, e.bucket_cap};
+    return MdBins{e.md_count, e.md_count + 2 * (size_t)e.ncell, e.md_bucket, e.md_cell,
+                  e.md_vid,   e.md_w, e.md_w + n, e.md_w + 2 * n, e.ncell, e.bucket_cap,
+                  md_nchunk(e.ncell)};
+}
+
+// Bytes of the MD bin counts: per-cell and per-chunk, both double-buffered.
+inline size_t md_count_bytes(int ncell)
+{
+    return (size_t)2 * (size_t)(ncell + md_nchunk(ncell)) * sizeof(int32_t);
 }
 
 // ---- launchers (nlist.hip)

@@ -66,7 +66,7 @@ class ForceConfig:
     r_core: float = 0.0           # "neighbor.r_core" (RBL core radius)
     rbl_pice: int = 1             # RBL shell stride (rbl_pice_num of "neighbor_sample_num")
     rbl_seed: int = 0             # RBL shell selection stream
-    full_list: bool = False       # True: every pair from both sides (no half-shell rows)
+    half_shell: bool = False      # True: FP32 pair pass over half-shell rows (each pair once)
     n_max: int = 0
     list_capacity: int = 0        # Verlet row capacity; 0 = automatic
     tile_stage: int = 0           # LDS stage of a cell tile (atoms); 0 = automatic
@@ -128,7 +128,7 @@ class LJCutCoulKspaceForce:
         p.r_core = float(cfg.r_core)
         p.rbl_pice = int(cfg.rbl_pice)
         p.rbl_seed = int(cfg.rbl_seed)
-        p.full_list = 1 if cfg.full_list else 0
+        p.half_shell = 1 if cfg.half_shell else 0
         return p, (eps, sig)
 
     # ------------------------------------------------------------ lifecycle

@@ -78,12 +78,16 @@ typedef struct {
     double r_core;         /* RBL core radius ("r_core")                           */
     int32_t rbl_pice;      /* RBL shell stride (pice_num, >= 1)                    */
     uint64_t rbl_seed;     /* RBL shell selection stream                           */
-    /* 0: with FP32 maths, the full Verlet list and at least three cells per axis,
-     * the pair pass evaluates every pair once (half-shell rows, Newton's third law)
-     * and adds the force to both atoms; otherwise, and with 1, every row holds all
-     * neighbours and each pair is evaluated from both sides as the reference does.
-     * Forces agree to FP32 rounding; neighbour lists and energies are the same. */
-    int32_t full_list;
+    /* 0 (default): every row holds all neighbours and each pair is evaluated from
+     * both sides, as the reference does.  1: with FP32 maths, the full Verlet list
+     * and at least three cells per axis, the pair pass evaluates every pair once
+     * (half-shell rows, Newton's third law) and adds the force to both atoms (LDS
+     * FP64 sums on a 2^-30 grid, order independent); otherwise as 0.  Forces agree
+     * to FP32 rounding; neighbour lists and energies are the same.  The half-shell
+     * pass is slower on MI355X than full rows (its rows differ in length by the
+     * atom's place in its cell, and every pair adds to LDS three times), so it is
+     * opt-in. */
+    int32_t half_shell;
 } rbmd_force_params;
 
 /* FP64 energies (LJ units x qqr2e for Coulomb terms) and virial

@@ -714,7 +714,10 @@ double orc_nose_hoover_xi(double xi, double T, double T0, double dt)
  * Gaussian 3-vector g per step shared by every atom (UnderdampedLangevinWorklet,
  * run/worklet/RunWorklet.cpp:1700-1728; g drawn in ExecutionNVT::ComputeAllForce,
  * run/ExecutionNVT.cpp:347-352; applied after the force by TempConTypeForce,
- * :735-747).  v is the velocity at the force evaluation (after the opening kick). */
+ * :735-747).  v is the velocity at the force evaluation (after the opening kick).
+ * m is the atom's own mass (as the library's Langevin term); the reference passes a
+ * unit-mass array (ExecutionNVT.cpp:737-738), the same thing for unit masses, its only
+ * case.  With other masses this restates the library's variant, not the reference. */
 void orc_langevin_force(int64_t n, double* fx, double* fy, double* fz, const double* vx,
                         const double* vy, const double* vz, const double* mass, double gamma,
                         double kbt, double dt, const double* g)

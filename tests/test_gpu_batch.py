@@ -24,11 +24,9 @@ def _dev(a, dtype=torch.float64):
 def _cfg(L, n, kspace="RBE", P=100, precision=32):
     from copy_gpu_amd import ForceConfig
 
-    # full rows, the pair pass the batched tile pass uses (single handles would
-    # otherwise take the half-shell pass, whose forces differ in the last bits)
+    # full rows (the default), the pair pass the batched tile pass uses
     return ForceConfig(box_len=(L, L, L), cut_off=2.5, coulomb_cut=5.0, alpha=0.5, skin=0.3,
-                       kspace=kspace, coulomb_sample_num=P, precision=precision, n_max=n,
-                       full_list=True)
+                       kspace=kspace, coulomb_sample_num=P, precision=precision, n_max=n)
 
 
 def _systems(sizes, seed0=100):

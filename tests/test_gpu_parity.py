@@ -29,14 +29,14 @@ def _dev(a, dtype=torch.float64):
 
 def _make(Lb, n, precision=32, kspace="NONE", P=100, rc_lj=2.5, rc_coul=5.0, alpha=0.5,
           skin=0.3, lo=(0.0, 0.0, 0.0), eps=(1.0,), sigma=(1.0,), list_capacity=0,
-          ewald_tol=1e-16, tile_stage=0, full_list=False):
+          ewald_tol=1e-16, tile_stage=0, half_shell=False):
     from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
 
     cfg = ForceConfig(box_len=tuple(float(v) for v in Lb), box_lo=lo, cut_off=rc_lj,
                       coulomb_cut=rc_coul, skin=skin, alpha=alpha, kspace=kspace,
                       coulomb_sample_num=P, precision=precision, eps=eps, sigma=sigma,
                       n_max=n, list_capacity=list_capacity, ewald_tol=ewald_tol,
-                      tile_stage=tile_stage, full_list=full_list)
+                      tile_stage=tile_stage, half_shell=half_shell)
     return LJCutCoulKspaceForce(cfg)
 
 
@@ -156,8 +156,8 @@ def test_pair_forces_and_energy(orc, gpu, precision, tol):
 
 @pytest.mark.parametrize("types", [False, True])
 def test_half_shell_rows_equal_full_rows(orc, gpu, types):
-    """FP32 handles evaluate each pair once over half-shell tiles (home cell + 13
-    forward cells) and add the force to both atoms; with full rows every pair is
+    """FP32 handles with half_shell evaluate each pair once over half-shell tiles (home
+    cell + 13 forward cells) and add the force to both atoms; with full rows every pair is
     evaluated from both sides.  Same neighbour sets, forces within FP32 rounding,
     energies and virial to 1e-6, both against the oracle at its FP32 bars; and over
     tiles read from global memory (LDS stage 64) the same bits as staged tiles."""
@@ -168,7 +168,7 @@ def test_half_shell_rows_equal_full_rows(orc, gpu, types):
     tt = (np.arange(x.size) % 3 == 0).astype(np.int32) if types else None
     out = {}
     for full, stage in ((False, 0), (True, 0), (False, 64)):
-        f = _make(Lb, x.size, **kw, full_list=full, tile_stage=stage)
+        f = _make(Lb, x.size, **kw, half_shell=not full, tile_stage=stage)
         f.Init(_dev(q), None if tt is None else _dev(tt, torch.int32))
         F = _compute(f, x, y, z, ev=True)
         out[(full, stage)] = (F, f.energy_virial(), f.neighbors(), f.stats())
