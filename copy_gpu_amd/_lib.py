@@ -149,6 +149,8 @@ _SIGS = {
     "rbmd_util_copy": (C.c_int, [_P, _P, C.c_int64, _P]),
     "rbmd_kspace_compute": (C.c_int, [C.POINTER(KspaceParams), C.c_int64] + [_P] * 4 +
                             [C.c_int32] + [_P] * 8),
+    "rbmd_kspace_compute_noalloc": (C.c_int, [C.POINTER(KspaceParams), C.c_int64] + [_P] * 4 +
+                                    [C.c_int32] + [_P] * 8),
     "rbmd_kspace_last_error": (C.c_char_p, []),
     "rbmd_batch_create": (C.c_int, [C.c_int32, _P, _P, C.POINTER(C.c_void_p)]),
     "rbmd_batch_destroy": (None, [_P]),

@@ -11,7 +11,11 @@
 //   RBE force worklet                    RunWorklet.cpp:1508-1548, ExecForceFunction.h:247-269
 //
 // rbmd_kspace_compute allocates its scratch (partials, rho, per-k coefficients and
-// terms, weights) for the call.
+// terms, weights) for the call.  rbmd_kspace_compute_noalloc runs the same kernels
+// on a fixed block of device memory (`scratch` below, 4 MiB per device) and
+// allocates nothing, for callers that cannot call hipMalloc.
+
+#include <mutex>
 
 #include "kspace.h"
 #include "state.h"
@@ -221,6 +225,46 @@ void ks_dispatch(const KsCall& c, char* buf, hipStream_t s)
 }  // namespace
 }  // namespace rbmd
 
+// Fixed scratch of rbmd_kspace_compute_noalloc, one copy per device, held until the
+// process ends.  4 MiB covers every P <= 8192 at any n: the partials hold at most
+// 2048 x 64 x 2 doubles (2 MiB, ks_geometry) and the per-k arrays 120 bytes per k.
+// In host code `scratch` is only the symbol's host-side stand-in; kernels take the
+// device address hipGetSymbolAddress returns (ks_scratch_ptr).
+constexpr size_t kKsScratchBytes = (size_t)4 << 20;
+constexpr int kKsScratchMaxP = 8192;
+static __device__ __align__(256) unsigned char scratch[kKsScratchBytes];
+
+namespace rbmd {
+namespace {
+
+constexpr int kKsMaxDevices = 64;
+std::mutex g_ks_scratch_mu;
+char* g_ks_scratch[kKsMaxDevices] = {};
+
+// Device address of the current device's `scratch`, looked up once per device.
+int ks_scratch_ptr(char** out)
+{
+    int dev = -1;
+    hipError_t st = hipGetDevice(&dev);
+    if (st != hipSuccess)
+        return ks_fail(RBMD_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(st));
+    if (dev < 0 || dev >= kKsMaxDevices) return ks_fail(RBMD_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_ks_scratch_mu);
+    if (!g_ks_scratch[dev]) {
+        void* p = nullptr;
+        st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (st != hipSuccess || !p)
+            return ks_fail(RBMD_ERR_HIP,
+                           std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(st));
+        g_ks_scratch[dev] = (char*)p;
+    }
+    *out = g_ks_scratch[dev];
+    return RBMD_OK;
+}
+
+}  // namespace
+}  // namespace rbmd
+
 using namespace rbmd;
 
 extern "C" {
@@ -248,6 +292,30 @@ int rbmd_kspace_compute(const rbmd_kspace_params* p, int64_t n, const double* x,
     if (st != hipSuccess) return ks_fail(RBMD_ERR_HIP, std::string("launch: ") + hipGetErrorString(st));
     if (st2 != hipSuccess)
         return ks_fail(RBMD_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(st2));
+    return RBMD_OK;
+}
+
+int rbmd_kspace_compute_noalloc(const rbmd_kspace_params* p, int64_t n, const double* x,
+                                const double* y, const double* z, const double* q, int32_t P,
+                                const int32_t* m, const double* w, void* fx, void* fy, void* fz,
+                                double* rho, double* ev, void* stream)
+{
+    g_ks_err.clear();
+    if (!p) return ks_fail(RBMD_ERR_INVALID, "null parameters");
+    const KsCall c = ks_call(p, n, x, y, z, q, P, m, w, fx, fy, fz, rho, ev);
+    int rc = ks_check(c);
+    if (rc != RBMD_OK) return rc;
+    if (P > kKsScratchMaxP || ks_layout(n, P).total > kKsScratchBytes)
+        return ks_fail(RBMD_ERR_INVALID,
+                       "rbmd_kspace_compute_noalloc: P = " + std::to_string(P) +
+                           " exceeds the fixed scratch (P <= " +
+                           std::to_string(kKsScratchMaxP) + ")");
+    char* buf = nullptr;
+    rc = ks_scratch_ptr(&buf);
+    if (rc != RBMD_OK) return rc;
+    ks_dispatch(c, buf, (hipStream_t)stream);
+    const hipError_t st = hipGetLastError();
+    if (st != hipSuccess) return ks_fail(RBMD_ERR_HIP, std::string("launch: ") + hipGetErrorString(st));
     return RBMD_OK;
 }
 

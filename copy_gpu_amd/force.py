@@ -307,11 +307,14 @@ def rbl_pice_num(n: int, box_len, cut_off: float, r_core: float, neighbor_sample
 
 
 def kspace_compute(box_len, alpha, x, y, z, q, m, w=None, fx=None, fy=None, fz=None,
-                   box_lo=(0.0, 0.0, 0.0), qqr2e=1.0, precision=64, stream=None):
+                   box_lo=(0.0, 0.0, 0.0), qqr2e=1.0, precision=64, stream=None,
+                   noalloc=False):
     """The k-space step of a given k-list without a force handle (rbmd_kspace_compute):
     returns (rho[P, 2] FP64, ev[7] = E_k, W_xx, W_yy, W_zz, W_xy, W_xz, W_yz), and adds
     the k-space force to fx, fy, fz when given (scratch arrays otherwise).
-    m: int32 device tensor (3P); w: FP64 device tensor (P) or None (S/P)."""
+    m: int32 device tensor (3P); w: FP64 device tensor (P) or None (S/P).
+    noalloc: run rbmd_kspace_compute_noalloc (the library's fixed per-device scratch,
+    P <= 8192, no synchronisation; the output tensors here are still allocated by torch)."""
     lib = L.lib()
     n, P = x.numel(), m.numel() // 3
     dt = torch.float32 if precision == 32 else torch.float64
@@ -322,9 +325,10 @@ def kspace_compute(box_len, alpha, x, y, z, q, m, w=None, fx=None, fy=None, fz=N
     prm = L.KspaceParams((C.c_double * 3)(*[float(v) for v in box_lo]),
                          (C.c_double * 3)(*[float(v) for v in box_len]), float(alpha),
                          float(qqr2e), int(precision))
-    rc = lib.rbmd_kspace_compute(C.byref(prm), n, _ptr(x), _ptr(y), _ptr(z), _ptr(q), P, _ptr(m),
-                                 _ptr(w), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(rho), _ptr(ev),
-                                 _stream(stream))
+    fn = lib.rbmd_kspace_compute_noalloc if noalloc else lib.rbmd_kspace_compute
+    rc = fn(C.byref(prm), n, _ptr(x), _ptr(y), _ptr(z), _ptr(q), P, _ptr(m), _ptr(w), _ptr(fx),
+            _ptr(fy), _ptr(fz), _ptr(rho), _ptr(ev), _stream(stream))
     if rc != 0:
-        raise RBMDError(f"rbmd_kspace_compute -> {rc}: {lib.rbmd_kspace_last_error().decode()}")
+        name = "rbmd_kspace_compute_noalloc" if noalloc else "rbmd_kspace_compute"
+        raise RBMDError(f"{name} -> {rc}: {lib.rbmd_kspace_last_error().decode()}")
     return rho.view(P, 2), ev

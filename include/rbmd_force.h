@@ -280,6 +280,19 @@ int rbmd_kspace_compute(const rbmd_kspace_params* p, int64_t n, const double* x,
                         const double* w, void* fx, void* fy, void* fz, double* rho, double* ev,
                         void* stream);
 
+/* Same computation and outputs as rbmd_kspace_compute (bit for bit), with its
+ * scratch in a fixed 4 MiB block of device memory per GPU instead of a hipMalloc:
+ * it calls no hipMalloc / hipFree and does not synchronise (the work is enqueued on
+ * `stream`).  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  P <= 8192 (larger
+ * P returns RBMD_ERR_INVALID).  Every call on one GPU shares that block, so calls
+ * that overlap on one GPU (other streams, other threads) race: order them on one
+ * stream or synchronise between them. */
+int rbmd_kspace_compute_noalloc(const rbmd_kspace_params* p, int64_t n, const double* x,
+                                const double* y, const double* z, const double* q, int32_t P,
+                                const int32_t* m, const double* w, void* fx, void* fy, void* fz,
+                                double* rho, double* ev, void* stream);
+
 const char* rbmd_kspace_last_error(void);
 
 /* Plain device copy of nbytes (multiple of 16), used to calibrate HBM
