@@ -11,8 +11,8 @@ is compared with the analytic sampler at P = 50/100/200/400: force RMS error
 against FP64 full Ewald (5 held-out configurations x 20 k-lists each) and
 atom-steps/s of each.  The headline value is the learned P = 50 run.  A step =
 kick, drift, list check (and rebuild when needed), k-sampling, structure factor,
-pair pass, k-space force, kick -- captured in HIP graphs (eight steps and one
-step) before the timed region and replayed.
+pair pass, k-space force, kick -- captured in HIP graphs (sixteen steps and the
+run's remainder) before the timed region and replayed.
 
 Prints ONE JSON line on stdout (logs go to stderr).  Also reported:
   * roofline of the pair pass (the dominant kernel): algorithmic bytes
@@ -140,13 +140,17 @@ def cpu_baseline(wl, x, y, z, q, vx, vy, vz, L, target_s):
                       f"thread, {el1:.1f} s"}
 
 
+# --half-shell: FP32 handles use the opt-in half-shell pair pass (each pair once)
+HALF_SHELL = False
+
+
 def make_force(wl, L, P, kspace="RBE", precision=32, tol=1e-14):
     from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
 
     return LJCutCoulKspaceForce(ForceConfig(
         box_len=(L, L, L), cut_off=wl["rc_lj"], coulomb_cut=wl["rc_coul"], skin=0.3,
         alpha=wl["alpha"], kspace=kspace, coulomb_sample_num=max(P, 1), precision=precision,
-        n_max=wl["n"], ewald_tol=tol))
+        n_max=wl["n"], ewald_tol=tol, half_shell=HALF_SHELL))
 
 
 class State:
@@ -164,18 +168,29 @@ class State:
                  nsteps=nsteps, seed=SEED, step0=step0, use_graph=use_graph)
 
 
-# MD steps whose HIP graphs the library captures on first use: eight steps and one
-# step (the remainder); a warm-up of at least GRAPH_WARMUP steps captures both
-GRAPH_WARMUP = 9
+# Steps per cached HIP graph in the library: a K-step run replays the GRAPH_STEPS-step
+# graph K // GRAPH_STEPS times and the graph of the K % GRAPH_STEPS remainder once,
+# each captured on first use.
+GRAPH_STEPS = 16
+
+
+def graph_warmup(W, K):
+    """At least W warm-up steps whose run captures every graph a K-step run replays:
+    the GRAPH_STEPS-step graph (when K >= GRAPH_STEPS) and the remainder's graph,
+    so no capture happens inside the timed region."""
+    r = K % GRAPH_STEPS
+    need = (GRAPH_STEPS if K >= GRAPH_STEPS else 0) + r
+    extra = max(0, W - need)
+    return need + GRAPH_STEPS * ((extra + GRAPH_STEPS - 1) // GRAPH_STEPS)
 
 
 def timed(f, st, wl, W, K, step, use_graph):
-    """W warm-up steps (at least GRAPH_WARMUP with graphs, so both cached graphs are
-    captured and instantiated before the clock starts), then K timed MD steps;
-    returns (seconds, list builds, next step)."""
+    """Warm-up steps (with graphs, graph_warmup(W, K) of them, so every graph the
+    timed run replays is captured and instantiated before the clock starts), then K
+    timed MD steps; returns (seconds, list builds, next step)."""
     import torch
 
-    Wr = max(W, GRAPH_WARMUP) if use_graph else W
+    Wr = graph_warmup(W, K) if use_graph else W
     st.run(f, wl, Wr, step, use_graph)
     step += Wr
     torch.cuda.synchronize()
@@ -497,8 +512,8 @@ def batch_eval(wl, args):
     seeds = [SEED + r for r in range(len(syss))]
     states = [(s.X, s.Y, s.Z, s.VX, s.VY, s.VZ, *s.F) for (_, _, _, s) in syss]
     K = args.steps if args.steps is not None else wl["steps"]
-    W = args.warmup
     use_graph = not args.no_graph
+    W = graph_warmup(args.warmup, K) if use_graph else args.warmup
     b.md_run(states, dt=wl["dt"], nsteps=W, seeds=seeds, step0=1, use_graph=use_graph)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -623,7 +638,8 @@ def plain_leg(wname, args, W, K, use_graph, with_nvt=False):
         log(f"autotune ({time.perf_counter() - t:.2f} s): {shapes}")
     if args.thermostat:
         f.set_thermostat(args.thermostat, (1.0, 1.0, 0.5))
-    log(f"warmup {max(W, GRAPH_WARMUP)} steps, timed {K} steps (graph={use_graph}, "
+    log(f"warmup {graph_warmup(W, K) if use_graph else W} steps, timed {K} steps "
+        f"(graph={use_graph}, "
         f"thermostat={args.thermostat or 'none'})")
     el, rebuilds, step = timed(f, st, wl, W, K, 1, use_graph)
     value = n * K / el
@@ -676,10 +692,15 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="run the library's tile-shape autotuner on the starting configuration "
                          "before the timed steps")
+    ap.add_argument("--half-shell", action="store_true",
+                    help="FP32 handles use the opt-in half-shell pair pass (each pair "
+                         "evaluated once, Newton's third law) instead of full rows")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB three times with the library's plain copy kernel "
                          "(known HBM traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    global HALF_SHELL
+    HALF_SHELL = args.half_shell
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
               "decomposition is not part of this build", file=sys.stderr)
@@ -774,7 +795,7 @@ def main():
                       "FP32 pair and k-space maths; FP64 positions, integrator, energy"),
         "hip_graph": use_graph,
         "graph_capture_before_timing": use_graph,
-        "warmup_steps_run": max(W, GRAPH_WARMUP) if use_graph else W,
+        "warmup_steps_run": graph_warmup(W, K) if use_graph else W,
         "thermostat": args.thermostat,
         "list_builds_in_timed_steps": rebuilds,
         "force_rms_err_vs_fp64_ewald": eps_total,

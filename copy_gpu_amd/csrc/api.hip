@@ -833,12 +833,14 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         for (int64_t v : {(int64_t)E.split, (int64_t)E.stage_cap, (int64_t)E.ncell,
                           (int64_t)E.rho_apb, (int64_t)E.kpart_cap})
             key.push_back((const void*)(uintptr_t)v);
-        // two cached graphs: kGraphSteps steps (amortises the launch of a graph)
-        // and one step (the remainder)
-        for (int g = 0; g < 2; ++g) {
+        // cached graphs: kGraphSteps steps (graphs[0]) and the remainder r < kGraphSteps
+        // (graphs[r]), captured on first use
+        const int rem = (int)(nsteps % kGraphSteps);
+        for (int g : {0, rem}) {
             Engine::StepGraph& G = E.graphs[g];
-            const int steps = g == 0 ? kGraphSteps : 1;
+            const int steps = g == 0 ? kGraphSteps : g;
             if (g == 0 && nsteps < kGraphSteps) continue;
+            if (g != 0 && rem == 0) continue;
             if (G.exec && key == G.key) continue;
             if (G.exec) hipGraphExecDestroy(G.exec);
             if (G.graph) hipGraphDestroy(G.graph);
@@ -861,8 +863,7 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
         HIP_TRY(hipMemcpyAsync(E.d_step, &s0, 8, hipMemcpyHostToDevice, s));
         for (int64_t k = 0; k + kGraphSteps <= nsteps; k += kGraphSteps)
             HIP_TRY(hipGraphLaunch(E.graphs[0].exec, s));
-        for (int64_t k = 0; k < nsteps % kGraphSteps; ++k)
-            HIP_TRY(hipGraphLaunch(E.graphs[1].exec, s));
+        if (rem > 0) HIP_TRY(hipGraphLaunch(E.graphs[rem].exec, s));
     }
     if (E.thermo_type != kThermoNone) launch_thermo_end(E, rs, s);
     HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
@@ -1266,7 +1267,7 @@ struct rbmd_batch_s {
     Batch b;
     hipStream_t own_stream = nullptr;
     hipEvent_t order_event = nullptr;
-    Engine::StepGraph graphs[2];  // [0] kGraphSteps steps, [1] one step
+    Engine::StepGraph graphs[kGraphSteps];  // [0] kGraphSteps steps, [r] r steps
     std::string err;
 };
 
@@ -1447,6 +1448,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     for (Engine* E : B.es)
         BHIP_TRY(hipMemsetAsync(E->md_count, 0, md_count_bytes(E->ncell), s));
     nlist_batch_launch(B, BP_MDBIN, s);
+    nlist_batch_launch(B, BP_CHUNKSUM, s);
     if (B.thermo) md_batch_launch(B, BP_THERMO_BEGIN, s);
     if (!use_graph) {
         constexpr int64_t kEagerSegment = int64_t(1) << 30;
@@ -1464,10 +1466,12 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         for (const BatchPhase& ph : B.ph) key_phase(ph);
         for (const BatchPhase& ph : B.pair_more) key_phase(ph);
         key.push_back((const void*)(uintptr_t)(B.thermo ? 1 : 0));
-        for (int g = 0; g < 2; ++g) {
+        const int rem = (int)(nsteps % kGraphSteps);
+        for (int g : {0, rem}) {
             Engine::StepGraph& G = b->graphs[g];
-            const int steps = g == 0 ? kGraphSteps : 1;
+            const int steps = g == 0 ? kGraphSteps : g;
             if (g == 0 && nsteps < kGraphSteps) continue;
+            if (g != 0 && rem == 0) continue;
             if (G.exec && key == G.key) continue;
             if (G.exec) hipGraphExecDestroy(G.exec);
             if (G.graph) hipGraphDestroy(G.graph);
@@ -1484,8 +1488,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         }
         for (int64_t k = 0; k + kGraphSteps <= nsteps; k += kGraphSteps)
             BHIP_TRY(hipGraphLaunch(b->graphs[0].exec, s));
-        for (int64_t k = 0; k < nsteps % kGraphSteps; ++k)
-            BHIP_TRY(hipGraphLaunch(b->graphs[1].exec, s));
+        if (rem > 0) BHIP_TRY(hipGraphLaunch(b->graphs[rem].exec, s));
     }
     if (B.thermo) md_batch_launch(B, BP_THERMO_END, s);
     BHIP_TRY(hipStreamSynchronize(s));

@@ -158,9 +158,8 @@ void ks_launch(const KsCall& c, char* buf, hipStream_t s)
     ka.fx = (T*)c.fx;
     ka.fy = (T*)c.fy;
     ka.fz = (T*)c.fz;
-    if (c.n > 0)
-        k_one<KForce<T, false>><<<(int)((c.n + kKfAtoms - 1) / kKfAtoms), kKfThreads, 0, s>>>(
-            ka);
+    ka.slices = kf_slices(c.n, c.P);
+    if (c.n > 0) k_one<KForce<T, false>><<<kf_blocks(c.n, ka.slices), kKfThreads, 0, s>>>(ka);
     if (c.rho_out) k_copy_rho<<<(2 * c.P + 255) / 256, 256, 0, s>>>(rho, 2 * c.P, c.rho_out);
     if (c.ev_out) k_kev_sum<<<1, 256, 0, s>>>(kev, c.P, c.ev_out);
 }

@@ -286,14 +286,35 @@ struct RhoFinal {
 };
 
 // ---------------------------------------------------------------- k-space force
-// Workgroup = 64 atoms (lanes, original order) x 8 k-slices (waves): wave w takes
-// k = w, w+8, ... of the coefficients staged in LDS (every lane reads the same
-// k: broadcast), then the 8 partial forces of each atom are summed in a fixed
-// order.  F_i += q_i sum_k k (cos(phi) A_k - sin(phi) B_k).
+// Workgroup = 8 waves = 8 / s atom groups of 64 atoms (lanes, original order) x s
+// k-slices: the waves of a group take k = slice, slice + s, ... of the coefficients
+// staged in LDS (every lane reads the same k: broadcast), then the s partial forces
+// of each atom are summed in a fixed order; slice 0 completes and kicks the atoms.
+// s = 8 (64 atoms per workgroup) unless P <= kKfWideP and that would give more
+// than kKfTargetBlocks workgroups: then the fewest slices that still give that many
+// (at 100,000 atoms and P = 50, s = 4), so fewer waves repeat the per-atom work.
+// F_i += q_i sum_k k (cos(phi) A_k - sin(phi) B_k).
 constexpr int kKfAtoms = 64;
 constexpr int kKfSlices = 8;
 constexpr int kKfThreads = kKfAtoms * kKfSlices;
 constexpr int kKfStage = 512;
+constexpr int kKfWideP = 128;
+constexpr int kKfTargetBlocks = 512;  // two per CU on the 256 CUs of an MI355X
+
+// k-slices per atom group of a k-space force launch over n atoms and P k-vectors
+// (P = 0: the kernel only completes the forces / kicks), and its workgroups
+inline int kf_slices(int64_t n, int P)
+{
+    if (P > kKfWideP) return kKfSlices;
+    int s = 1;
+    while (s < kKfSlices && (n * s + kKfThreads - 1) / kKfThreads < kKfTargetBlocks) s *= 2;
+    return s;
+}
+inline int kf_blocks(int64_t n, int slices)
+{
+    const int64_t apb = (int64_t)kKfAtoms * (kKfSlices / slices);
+    return (int)((n + apb - 1) / apb);
+}
 
 template <typename T>
 struct KForceArgs {
@@ -309,6 +330,7 @@ struct KForceArgs {
     uint32_t* flags;
     ThermoArgs th;  // velocity thermostat of the closing kick (KForce<T, true>)
     const double* rbl_mean = nullptr;  // random batch list: mean near force to subtract
+    int slices = kKfSlices;            // k-slices per atom group (kf_slices)
 };
 
 template <typename T>
@@ -317,10 +339,11 @@ struct KForceLds {
     T red[kKfSlices][3][kKfAtoms];
 };
 
-// The k-space force of the 64 atoms of workgroup blk.  Each lane of wave 0
-// first calls pre(i) for its atom i < n (loads that the completion needs, issued
-// before the k-sum so that their latency overlaps it), then, after the k-sum,
-// fin(i, f0, f1, f2, pre_value) with the complete force (pair force + k-space).
+// The k-space force of the atoms of workgroup blk.  Each lane of an atom-owning
+// wave (slice 0 of its atom group) first calls pre(i) for its atom i < n (loads
+// that the completion needs, issued before the k-sum so that their latency
+// overlaps it), then, after the k-sum, fin(i, f0, f1, f2, pre_value) with the
+// complete force (pair force + k-space).
 // fin runs with FP contraction off in the caller's body.
 template <typename T, class Pre, class Fin>
 __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T>& L, Pre pre,
@@ -328,10 +351,12 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
 {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = blk * kKfAtoms + lane;
     const int n = A.n, P = A.P;
+    const int slices = uniform(A.slices);
+    const int slice = wv % slices;
+    const int i = blk * (kKfAtoms * (kKfSlices / slices)) + (wv / slices) * kKfAtoms + lane;
     const Box& box = A.box;
-    const bool own = wv == 0 && i < n;
+    const bool own = slice == 0 && i < n;
     double sx = 0, sy = 0, sz = 0;
     if (i < n && P > 0) {
         sx = (A.x[i] - box.lo[0]) * box.invL[0];
@@ -361,7 +386,7 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
             L.st[k] = A.coef[base + k];
         __syncthreads();
         const int cnt = (P - base) < kKfStage ? (P - base) : kKfStage;
-        for (int k = wv; k < cnt; k += kKfSlices) {
+        for (int k = slice; k < cnt; k += slices) {
             const KCoef<T>& c = L.st[k];
             const double phi = fma(c.m[2], sz, fma(c.m[1], sy, c.m[0] * sx));
             T sn, cs;
@@ -372,7 +397,7 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
             az = fma_t(g, c.k[2], az);
         }
     }
-    if (P > 0) {
+    if (P > 0 && slices > 1) {
         L.red[wv][0][lane] = ax;
         L.red[wv][1][lane] = ay;
         L.red[wv][2][lane] = az;
@@ -381,10 +406,17 @@ __device__ inline void kforce_block(const KForceArgs<T>& A, int blk, KForceLds<T
     if (own) {
         if (P > 0) {
             T s0 = 0, s1 = 0, s2 = 0;
-            for (int w = 0; w < kKfSlices; ++w) {
-                s0 += L.red[w][0][lane];
-                s1 += L.red[w][1][lane];
-                s2 += L.red[w][2][lane];
+            if (slices > 1) {
+                // the slices of this wave's atom group: waves wv .. wv + slices - 1
+                for (int w = wv; w < wv + slices; ++w) {
+                    s0 += L.red[w][0][lane];
+                    s1 += L.red[w][1][lane];
+                    s2 += L.red[w][2][lane];
+                }
+            } else {
+                s0 += ax;
+                s1 += ay;
+                s2 += az;
             }
             f0 = f0 + qi * s0;
             f1 = f1 + qi * s1;
@@ -529,10 +561,18 @@ struct KForce {
                     A.vz[i] = v2;
                 }
             });
-        if (KICK && th.type != kThermoNone && (threadIdx.x >> 6) == 0) {
-            // wave 0 holds the workgroup's atoms: one partial per workgroup
+        if (KICK && th.type != kThermoNone) {
+            // one partial per workgroup: the waves holding atoms (wave 0 alone, or all
+            // eight with few k-vectors; the others add zero) in wave order
+            __shared__ double kes[kKfThreads / 64];
             const double s = wave_sum(ke);
-            if ((threadIdx.x & 63) == 0) th.part[blk] = s;
+            if ((threadIdx.x & 63) == 0) kes[threadIdx.x >> 6] = s;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = 0.0;
+                for (int w = 0; w < kKfThreads / 64; ++w) t += kes[w];
+                th.part[blk] = t;
+            }
         }
     }
 };
@@ -558,18 +598,23 @@ inline RhoFinalArgs<T> rho_final_args(Engine& e, unsigned long long* step_dev)
                            e.rho,   (KCoef<T>*)e.kcoef, e.kev, step_dev};
 }
 
+// k-vectors a k-space force launch of the handle carries (kforce_args) and its
+// workgroups
+inline int kforce_P(const Engine& e) { return has_kspace(e) ? e.P : 0; }
+inline int kforce_blocks(const Engine& e) { return kf_blocks(e.n, kf_slices(e.n, kforce_P(e))); }
+
 template <typename T>
 inline KForceArgs<T> kforce_args(Engine& e, const double* x, const double* y, const double* z,
                                  void* fx, void* fy, void* fz, double* vx, double* vy,
                                  double* vz, const double* mass, double dt, bool kspace = true)
 {
     KForceArgs<T> a{x,        y,  z,  e.q,  (int)e.n, e.box, (const KCoef<T>*)e.kcoef,
-                    kspace ? e.P : 0, (T*)fx, (T*)fy, (T*)fz, vx, vy, vz, mass, dt,
+                    kspace ? kforce_P(e) : 0, (T*)fx, (T*)fy, (T*)fz, vx, vy, vz, mass, dt,
                     e.flags};
     a.rbl_mean = e.rbl ? e.rbl_mean : nullptr;
+    a.slices = kf_slices(e.n, a.P);
     return a;
 }
 
-inline int kforce_blocks(const Engine& e) { return (int)((e.n + kKfAtoms - 1) / kKfAtoms); }
 
 }  // namespace rbmd

@@ -232,14 +232,36 @@ struct MdBin {
     __device__ static void run(const Args& a, int blk, int)
     {
         const int i = blk * kBlock + threadIdx.x;
-        const int buf = (int)(a.flags[3] & 1u);
-        int c = -1;
-        if (i < a.n) {
-            const double p[3] = {a.x[i], a.y[i], a.z[i]};
-            // any one-to-one slot numbering serves the cell sort: atom i bins in slot i
-            c = md_bin_atom(a.bins, buf, i, i, p, a.box, a.dims, a.cell);
+        if (i >= a.n) return;
+        const double p[3] = {a.x[i], a.y[i], a.z[i]};
+        // any one-to-one slot numbering serves the cell sort: atom i bins in slot i
+        md_bin_atom(a.bins, (int)(a.flags[3] & 1u), i, i, p, a.box, a.dims, a.cell);
+    }
+};
+
+// Chunk counts of MdBin's bins: block k sums the counts of cells [256 k, 256 k + 256)
+// of the buffer MdBin filled and writes chunk k (no atomics: atoms in id order land
+// in every chunk, so per-atom or per-wave chunk atomics serialise on few counters).
+struct ChunkSum {
+    static constexpr int kThreads = kBinChunk;
+    struct Args {
+        MdBins bins;
+        const uint32_t* flags;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        __shared__ int red[kThreads / 64];
+        const int b = (int)(a.flags[3] & 1u);
+        const int c = blk * kBinChunk + (int)threadIdx.x;
+        int v = c < a.bins.ncell ? a.bins.count[b * a.bins.ncell + c] : 0;
+        v = wave_sum(v);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+            a.bins.chunk[b * a.bins.nchunk + blk] = s;
         }
-        md_bin_chunks_wave(a.bins, buf, i < a.n, c);
     }
 };
 
@@ -403,6 +425,8 @@ void launch_md_bin(Engine& e, const double* x, const double* y, const double* z,
 {
     (void)hipMemsetAsync(e.md_count, 0, md_count_bytes(e.ncell), s);
     k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z));
+    k_one<ChunkSum><<<md_nchunk(e.ncell), ChunkSum::kThreads, 0, s>>>(
+        ChunkSum::Args{md_bins(e), e.flags});
 }
 
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s)
@@ -436,8 +460,9 @@ hipError_t nlist_batch_prepare(Batch& b)
     std::vector<Scatter::Args> sp(R);
     std::vector<CellSort::Args> cs(R);
     std::vector<MdBin::Args> mb(R);
+    std::vector<ChunkSum::Args> cks(R);
     std::vector<Gather::Args> ga(R);
-    std::vector<int> nat(R), none(R, 1), ncell(R);
+    std::vector<int> nat(R), none(R, 1), ncell(R), nch(R);
     for (size_t r = 0; r < R; ++r) {
         Engine& e = *b.es[r];
         const RepState& st = b.st[r];
@@ -446,9 +471,11 @@ hipError_t nlist_batch_prepare(Batch& b)
         sp[r] = scatter_args(e);
         cs[r] = cellsort_args(e);
         mb[r] = mdbin_args(e, st.x, st.y, st.z);
+        cks[r] = ChunkSum::Args{md_bins(e), e.flags};
         ga[r] = gather_args(e, st.x, st.y, st.z);
         nat[r] = nblk(e.n > 0 ? e.n : 1);
         ncell[r] = e.ncell;
+        nch[r] = md_nchunk(e.ncell);
     }
     hipError_t err;
     if ((err = phase_upload(b.ph[BP_BIN], bi, nat, 0)) != hipSuccess) return err;
@@ -456,6 +483,7 @@ hipError_t nlist_batch_prepare(Batch& b)
     if ((err = phase_upload(b.ph[BP_SCATTER], sp, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_CELLSORT], cs, ncell, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_MDBIN], mb, nat, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_CHUNKSUM], cks, nch, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_GATHER], ga, nat, 0)) != hipSuccess) return err;
     return hipSuccess;
 }
@@ -468,6 +496,7 @@ void nlist_batch_launch(Batch& b, int phase, hipStream_t s)
     case BP_SCATTER: phase_launch<Scatter>(b.ph[phase], s); break;
     case BP_CELLSORT: phase_launch<CellSort>(b.ph[phase], s); break;
     case BP_MDBIN: phase_launch<MdBin>(b.ph[phase], s); break;
+    case BP_CHUNKSUM: phase_launch<ChunkSum>(b.ph[phase], s); break;
     case BP_GATHER: phase_launch<Gather>(b.ph[phase], s); break;
     default: break;
     }

@@ -636,12 +636,15 @@ struct HalfAcc {
 // One pair of a half row: the FP32 force factor as in the full pass, the force on
 // i accumulated in FP64 registers, minus the same on j added to the LDS sums on the
 // 2^-30 grid; with EV the pair's whole energy (both directed halves) and virial.
+// Lanes on the row sentinel (u = total, past the row's end) and pairs beyond both
+// cutoffs add nothing and skip the atomics (the sentinel lanes would all hit one LDS
+// address and serialise); the sums are exact, so skipping a zero changes no bit.
 template <bool EV, bool MULTI>
 __device__ inline void accumulate_half(bool lj, double dx, double dy, double dz, double qq,
                                        int ti, int tj, bool valid, const PairScalars& ps,
                                        const PairParams* __restrict__ pp, double& fx,
                                        double& fy, double& fz, double* acc, const HalfAcc& J,
-                                       int u)
+                                       int u, int total)
 {
 #pragma clang fp contract(off)
     const double d2 = dx * dx + dy * dy + dz * dz;
@@ -654,9 +657,11 @@ __device__ inline void accumulate_half(bool lj, double dx, double dy, double dz,
     fx = fma(f, dx, fx);
     fy = fma(f, dy, fy);
     fz = fma(f, dz, fz);
-    atomicAdd(&J.x[u], -half_grid(f * dx));
-    atomicAdd(&J.y[u], -half_grid(f * dy));
-    atomicAdd(&J.z[u], -half_grid(f * dz));
+    if (valid && u < total && f != 0.0) {
+        atomicAdd(&J.x[u], -half_grid(f * dx));
+        atomicAdd(&J.y[u], -half_grid(f * dy));
+        atomicAdd(&J.z[u], -half_grid(f * dz));
+    }
     if (EV && ok) {
         const double h = -f;
         acc[0] += 2.0 * (double)e_lj;
@@ -672,23 +677,26 @@ __device__ inline void accumulate_half(bool lj, double dx, double dy, double dz,
 
 // pair_wave over half rows: the same lane groups, rounds and sentinel handling;
 // each row's force on its home atom goes to the LDS sums at the end of the row.
-// Rows that overflowed scan the candidates after the home atom instead.
+// Rows that overflowed scan the candidates after the home atom instead.  Rows are
+// dealt to the lane groups in the order ord (home indices by decreasing row length,
+// so a wave's four rows have about the same length; null: home order).
 template <bool EV, bool MULTI, bool STAGED>
 __device__ inline void pair_wave_half(const PairArgs& a, const HalfHdr& h,
                                       const double4* __restrict__ lds,
                                       const int32_t* __restrict__ ltype, int a0, int a1, int h0,
                                       int wv, int lane, const PairScalars& ps,
                                       const PairParams* __restrict__ pp, double* acc,
-                                      const HalfAcc& J)
+                                      const HalfAcc& J, const int32_t* __restrict__ ord)
 {
     const int total = h.off[kHalfSegs];
     const int cap = a.cap;
     constexpr int kPer = 64 / kGroup;
     const int g = lane / kGroup, gl = lane % kGroup;
     const uint16_t* __restrict__ rows = a.nlist + (size_t)uniform(h0) * cap;
-    for (int t0 = a0 + kPer * wv; t0 < a1; t0 += kPer * kTileWaves) {
-        const int s = t0 + g;
-        const bool live = s < a1;
+    for (int t0 = kPer * wv; t0 < a1 - a0; t0 += kPer * kTileWaves) {
+        const int pos = t0 + g;
+        const bool live = pos < a1 - a0;
+        const int s = a0 + (live && ord ? ord[pos] : pos);
         const int2 nc = live ? a.ncount[s] : make_int2(0, 0);
         const int n_in = nc.x, tot = nc.x + nc.y;
         const bool over = tot >= cap;
@@ -736,7 +744,7 @@ __device__ inline void pair_wave_half(const PairArgs& a, const HalfHdr& h,
                         accumulate_half<EV, MULTI>(base + kGroup * k < nin_max, pj[k].x - pi.x,
                                                    pj[k].y - pi.y, pj[k].z - pi.z, qi * pj[k].w,
                                                    ti, tj[k], FAST || u[k] >= 0, ps, pp, ax, ay,
-                                                   az, acc, J, uu);
+                                                   az, acc, J, uu, total);
                     }
                 }
             }
@@ -752,7 +760,7 @@ __device__ inline void pair_wave_half(const PairArgs& a, const HalfHdr& h,
                 const int tj = MULTI ? half_type<STAGED>(h, ltype, a.ts, uc) : 0;
                 accumulate_half<EV, MULTI>(true, pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
                                            qi * pj.w, ti, tj, live && over && uu < total, ps, pp,
-                                           ax, ay, az, acc, J, uc);
+                                           ax, ay, az, acc, J, uc, total);
             }
         }
         ax = group_sum(ax);
@@ -765,6 +773,9 @@ __device__ inline void pair_wave_half(const PairArgs& a, const HalfHdr& h,
         }
     }
 }
+
+// Largest cell whose home rows the half pass orders by length (LDS ints).
+constexpr int kHalfOrderMax = 256;
 
 // Force planes of the half pass: [kHalfSegs][3][stride] floats, plane k holding the
 // sums a tile made for the atoms of its segment k, at their cell-sorted slots.
@@ -850,14 +861,37 @@ struct TilePassHalf {
             J.y[u] = 0.0;
             J.z[u] = 0.0;
         }
+        // home atoms by decreasing half-row length (ties by index; an overflowed row
+        // counts as longest), for cells of at most kHalfOrderMax atoms
+        __shared__ int32_t ord[kHalfOrderMax], len[kHalfOrderMax];
+        const bool sorted = hc <= kHalfOrderMax;
+        if (sorted) {
+            for (int t = threadIdx.x; t < hc; t += kTileThreads) {
+                const int2 nc = a.ncount[a0 + t];
+                len[t] = nc.x + nc.y;
+            }
+        }
+        __syncthreads();
+        if (sorted) {
+            for (int t = threadIdx.x; t < hc; t += kTileThreads) {
+                const int lt = len[t];
+                int r = 0;
+                for (int j = 0; j < hc; ++j) {
+                    const int lj = len[j];
+                    r += (lj > lt || (lj == lt && j < t)) ? 1 : 0;
+                }
+                ord[r] = t;
+            }
+        }
         __syncthreads();
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int32_t* po = sorted ? ord : nullptr;
         if (staged)
             pair_wave_half<EV, MULTI, true>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps, A.pp,
-                                            acc, J);
+                                            acc, J, po);
         else
             pair_wave_half<EV, MULTI, false>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane, A.ps,
-                                             A.pp, acc, J);
+                                             A.pp, acc, J, po);
         __syncthreads();
         for (int u = threadIdx.x; u < total; u += kTileThreads) {
             const int st = half_seg(h, u);

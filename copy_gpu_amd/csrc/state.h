@@ -39,8 +39,12 @@ struct SampleArgs {
     double* kw = nullptr;
 };
 
-// MD steps captured in one graph (amortises the cost of launching a graph).
-constexpr int kGraphSteps = 8;
+// MD steps captured in one graph.  Launching a graph leaves a gap of about 8 us
+// on the device and every segment opens and closes with its own kick kernels
+// (about 4 us more than a fused step), so segments are long: a run of K steps
+// replays the kGraphSteps-step graph K / kGraphSteps times and one graph of the
+// remaining K % kGraphSteps steps.
+constexpr int kGraphSteps = 16;
 
 struct Engine {
     // ---- parameters
@@ -167,13 +171,13 @@ struct Engine {
     int64_t t_cnt[P_NUM] = {};
     std::vector<hipEvent_t> event_pool;
 
-    // MD graph cache: [0] kGraphSteps steps, [1] one step
+    // MD graph cache: [0] kGraphSteps steps, [r] r steps (a run's remainder)
     struct StepGraph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         std::vector<const void*> key;
     };
-    StepGraph graphs[2];
+    StepGraph graphs[kGraphSteps];
     bool capturing = false;
 
     std::string err;
@@ -196,6 +200,7 @@ enum BatchPh {
     BP_SCATTER,
     BP_CELLSORT,
     BP_MDBIN,
+    BP_CHUNKSUM,       // chunk counts of the MD bins after BP_MDBIN (nlist.hip)
     BP_GATHER,
     BP_RHO,            // rho partials of the force pass (kspace.hip)
     BP_SORT_RHO,       // MD step: rho partials + cell sort (md.hip)

@@ -130,8 +130,8 @@ __device__ inline void md_bin_wave(const MdBins& b, int buf, bool active, int s,
     }
 }
 
-__device__ inline int md_bin_atom(const MdBins& b, int buf, int s, int v, const double p[3],
-                                  const Box& box, int3 dims, double3 cell)
+__device__ inline void md_bin_atom(const MdBins& b, int buf, int s, int v, const double p[3],
+                                   const Box& box, int3 dims, double3 cell)
 {
     double w[3];
     const int c = wrap_cell(p, box, dims, cell, w);
@@ -143,24 +143,6 @@ __device__ inline int md_bin_atom(const MdBins& b, int buf, int s, int v, const 
     const int bc = buf * b.ncell + c;
     const int rank = atomicAdd(&b.count[bc], 1);
     if (rank < b.cap) b.bucket[(size_t)bc * b.cap + rank] = s;
-    return c;
-}
-
-// Chunk counts of one wave's binned atoms (md_bin_atom leaves them out): one atomic
-// per distinct chunk of the wave, since a wave of atoms in id order spans few chunks
-// and per-atom atomics on a handful of chunk counters would serialise.
-__device__ inline void md_bin_chunks_wave(const MdBins& b, int buf, bool active, int c)
-{
-    const int ch = active ? c / kBinChunk : -1;
-    const int lane = (int)__lane_id();
-    uint64_t pending = __ballot(active);
-    while (pending != 0ull) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int cl = __shfl(ch, leader, 64);
-        const uint64_t grp = __ballot(ch == cl) & pending;
-        if (lane == leader) atomicAdd(&b.chunk[buf * b.nchunk + cl], __popcll(grp));
-        pending &= ~grp;
-    }
 }
 
 // ---------------------------------------------------------------- tile order

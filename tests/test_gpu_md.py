@@ -267,8 +267,8 @@ def _oracle_thermo_run(orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt, kforce
 
 @pytest.mark.parametrize("kind", ["BERENDSEN", "TEMP_RESCALE", "NOSE_HOOVER", "LANGEVIN"])
 def test_md_thermostat_matches_oracle(orc, gpu, kind):
-    """FP64 Ewald NVT run of 30 steps (three 8-step graph replays and six single
-    steps) with per-atom masses and list rebuilds, against the oracle's loop in the
+    """FP64 Ewald NVT run of 30 steps (a 16-step graph and a 14-step remainder
+    graph) with per-atom masses and list rebuilds, against the oracle's loop in the
     reference's order: positions, velocities, image flags, the temperature of every
     step, and the Nose-Hoover friction."""
     from copy_gpu_amd import systems

@@ -308,6 +308,32 @@ def test_rbe_forces_given_klist(orc, gpu, precision, tol):
     assert np.linalg.norm(W - Wo) < 10 * tol * np.linalg.norm(Wo)
 
 
+@pytest.mark.parametrize("n,slices", [(20000, 8), (70000, 4), (140000, 2), (270000, 1)])
+def test_kspace_force_work_splits(orc, gpu, n, slices):
+    """The k-space force kernel splits a workgroup's 8 waves into atom groups of 64 x k
+    slices by the launch size (8 slices below 65,409 atoms at P <= 128, fewer above, so
+    it still has 512 workgroups): every split gives the oracle's forces on the same k-list
+    (FP32, the k-space part's bar 1e-4 relative L2; a wrong slice sum is off by O(1))."""
+    from copy_gpu_amd import force as F
+    from copy_gpu_amd import systems
+
+    P, alpha = 64, 0.5
+    _log(f"case: k-space force, {n} ions, P={P}, expected {slices} slices per atom group")
+    x, y, z, q, L = _ions(n, seed=SEED + n)
+    Lb = np.array([L] * 3)
+    m, S = systems.sample_kvectors_host(alpha, Lb, P, seed=SEED, step=3)
+    m = np.asarray(m, np.int32).reshape(-1, 3)
+    fx, fy, fz = (torch.zeros(n, device="cuda", dtype=torch.float32) for _ in range(3))
+    F.kspace_compute(Lb, alpha, _dev(x), _dev(y), _dev(z), _dev(q),
+                     _dev(m.reshape(-1), torch.int32), None, fx, fy, fz, precision=32)
+    torch.cuda.synchronize()
+    Fg = torch.stack([fx, fy, fz], 1).double().cpu().numpy()
+    Fo, _ = orc.kspace_forces(x, y, z, q, Lb, alpha, m, np.full(P, S / P))
+    err = _rel(Fg, Fo)
+    _log(f"  rel L2 error {err:.3e}")
+    assert err < 1e-4
+
+
 def test_autotuned_shapes_keep_parity(orc, gpu):
     """The autotuner only changes launch shapes (workgroups per tile, LDS stage, rho(k)
     workgroup size): forces with the tuned shapes still meet the FP32 bar."""
