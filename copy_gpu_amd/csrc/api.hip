@@ -27,6 +27,13 @@ namespace rbmd {
 void build_kcdf(double alpha, const double* L, std::vector<double>& cdf, int M[3]);
 double rbe_S(double alpha, const double* L);
 
+// An MD run in which some atom moved more than half the Verlet skin in one step: a
+// rebuild references the positions binned one step earlier, so such a step can miss
+// pairs (flags[4], set by the drift).
+static const char* kStepTooLong =
+    "an atom moved more than half the Verlet skin in one step: the list rebuilds may "
+    "have missed pairs; use a larger skin or a smaller dt";
+
 static const char* kPassNames[P_NUM] = {"check", "bin",  "scan",   "scatter",   "cellsort",
                                         "list",  "gather", "rho",  "rho_final", "pair",
                                         "kforce", "kick", "ev_final", "sample"};
@@ -795,6 +802,7 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     if (rc != RBMD_OK) return rc;
     // bin the current positions for a rebuild in the first step (later steps'
     // bins come from the tile pass of the step before)
+    HIP_TRY(hipMemsetAsync(E.flags + 4, 0, sizeof(uint32_t), s));
     launch_md_bin(E, x, y, z, s);
     if (E.thermo_type != kThermoNone) {
         if (!(thermo_dof(E) > 0))
@@ -868,9 +876,11 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     if (E.thermo_type != kThermoNone) launch_thermo_end(E, rs, s);
     HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    uint32_t fl[2];
-    HIP_TRY(hipMemcpy(fl, E.flags, 8, hipMemcpyDeviceToHost));
+    uint32_t fl[5];
+    HIP_TRY(hipMemcpy(fl, E.flags, sizeof(fl), hipMemcpyDeviceToHost));
     if (fl[1]) return fail(E, RBMD_ERR_INVALID, "non-finite positions during the run");
+    if (fl[4])
+        return fail(E, RBMD_ERR_STATE, kStepTooLong);
     return RBMD_OK;
 }
 
@@ -1447,6 +1457,7 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     // bins of the current positions for a rebuild in the first step
     for (Engine* E : B.es)
         BHIP_TRY(hipMemsetAsync(E->md_count, 0, md_count_bytes(E->ncell), s));
+    for (Engine* E : B.es) BHIP_TRY(hipMemsetAsync(E->flags + 4, 0, sizeof(uint32_t), s));
     nlist_batch_launch(B, BP_MDBIN, s);
     nlist_batch_launch(B, BP_CHUNKSUM, s);
     if (B.thermo) md_batch_launch(B, BP_THERMO_BEGIN, s);
@@ -1494,12 +1505,15 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
     BHIP_TRY(hipStreamSynchronize(s));
     for (size_t r = 0; r < R; ++r) {
         Engine& E = *B.es[r];
-        uint32_t fl[2];
+        uint32_t fl[5];
         BHIP_TRY(hipMemcpy(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost));
-        BHIP_TRY(hipMemcpy(fl, E.flags, 8, hipMemcpyDeviceToHost));
+        BHIP_TRY(hipMemcpy(fl, E.flags, sizeof(fl), hipMemcpyDeviceToHost));
         if (fl[1])
             return bfail(b, RBMD_ERR_INVALID,
                          "non-finite positions in replica " + std::to_string(r));
+        if (fl[4])
+            return bfail(b, RBMD_ERR_STATE, std::string(kStepTooLong) + " (replica " +
+                                                std::to_string(r) + ")");
     }
     return RBMD_OK;
 }

@@ -52,10 +52,17 @@ struct PairParams {
 // the per-atom records are written in slot order (whole cache lines from a tile's
 // consecutive home atoms) rather than scattered by atom id.
 constexpr int kBinChunk = 256;  // cells per chunk count (the cell sort's prefix sums)
+// Chunk counts are kept only above this many cells.  Below it each cell's start is
+// summed over every earlier cell (a few us per rebuild at 13,824 cells), which is
+// cheaper than the chunk atomics' cost on every step: the tile pass's wave-aggregated
+// chunk additions land on ncell / 256 counters and serialise there (about 4 us per
+// step at 729 and at 2,197 cells).
+constexpr int kBinChunkMinCells = 16384;
 
 struct MdBins {
     int32_t* count;        // [2][ncell]
     int32_t* chunk;        // [2][nchunk]: sums of count over kBinChunk consecutive cells
+                           // (nchunk = 0: not kept, below kBinChunkMinCells cells)
     int32_t* bucket;       // [2][ncell][cap] slots at the binning
     int32_t* cell;         // [n] by slot: cell at the binning
     int32_t* vid;          // [n] by slot: atom id

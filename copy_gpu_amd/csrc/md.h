@@ -148,6 +148,11 @@ __device__ inline void kick_drift_atom(const KickDriftArgs<T>& a, int i, const K
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (d2 > a.thr2) a.flags[0] = 1u;
     if (d2 != d2) a.flags[1] = 1u;
+    // A rebuild takes its reference positions from the binning one step earlier, so
+    // the list stays exact only while no atom moves more than half the skin in one
+    // step (flags[4] reports a step that did; the run then returns an error).
+    const double s0 = v[0] * dt, s1 = v[1] * dt, s2 = v[2] * dt;
+    if (s0 * s0 + s1 * s1 + s2 * s2 > a.thr2) a.flags[4] = 1u;
     a.xs[in.slot] = make_double4(in.xr[0] + dx, in.xr[1] + dy, in.xr[2] + dz, in.q);
 }
 
@@ -340,10 +345,12 @@ __device__ inline void cellsort_md_cell(const CellSortMDArgs& a, int c, int lane
     const int32_t* count = a.bins.count + (size_t)b * a.ncell;
     const int32_t* chunk = a.bins.chunk + (size_t)b * a.bins.nchunk;
     // start = the counts of the whole chunks before c, then of the cells of c's own
-    // chunk before c: at most ncell / kBinChunk + kBinChunk loads per cell
-    const int c0 = (c / kBinChunk) * kBinChunk;
+    // chunk before c: at most ncell / kBinChunk + kBinChunk loads per cell (without
+    // chunk counts, every earlier cell)
+    const int c0 = a.bins.nchunk > 0 ? (c / kBinChunk) * kBinChunk : 0;
     int part = 0;
-    for (int k = lane; k < c / kBinChunk; k += 64) part += chunk[k];
+    if (a.bins.nchunk > 0)
+        for (int k = lane; k < c / kBinChunk; k += 64) part += chunk[k];
     for (int k = c0 + lane; k < c; k += 64) part += count[k];
     const int s0 = wave_sum(part);
     const int cnt = count[c];

@@ -425,8 +425,9 @@ void launch_md_bin(Engine& e, const double* x, const double* y, const double* z,
 {
     (void)hipMemsetAsync(e.md_count, 0, md_count_bytes(e.ncell), s);
     k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z));
-    k_one<ChunkSum><<<md_nchunk(e.ncell), ChunkSum::kThreads, 0, s>>>(
-        ChunkSum::Args{md_bins(e), e.flags});
+    if (md_nchunk(e.ncell) > 0)
+        k_one<ChunkSum><<<md_nchunk(e.ncell), ChunkSum::kThreads, 0, s>>>(
+            ChunkSum::Args{md_bins(e), e.flags});
 }
 
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s)

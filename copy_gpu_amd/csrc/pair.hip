@@ -580,7 +580,8 @@ struct TilePass {
             const int b = (int)(A.flags[3] & 1u);
             if (part == 0 && threadIdx.x == 0) {
                 A.bins.count[b * A.bins.ncell + c] = 0;
-                if (c % kBinChunk == 0) A.bins.chunk[b * A.bins.nchunk + c / kBinChunk] = 0;
+                if (A.bins.nchunk > 0 && c % kBinChunk == 0)
+                    A.bins.chunk[b * A.bins.nchunk + c / kBinChunk] = 0;
             }
             for (int base = a0; base < a1; base += kTileThreads) {
                 const int s = base + (int)threadIdx.x;
@@ -906,7 +907,8 @@ struct TilePassHalf {
             const int b = (int)(A.flags[3] & 1u);
             if (threadIdx.x == 0) {
                 A.bins.count[b * A.bins.ncell + c] = 0;
-                if (c % kBinChunk == 0) A.bins.chunk[b * A.bins.nchunk + c / kBinChunk] = 0;
+                if (A.bins.nchunk > 0 && c % kBinChunk == 0)
+                    A.bins.chunk[b * A.bins.nchunk + c / kBinChunk] = 0;
             }
             for (int base = a0; base < a1; base += kTileThreads) {
                 const int s = base + (int)threadIdx.x;

@@ -252,7 +252,10 @@ inline double thermo_dof(const Engine& e)
     return e.thermo_dof > 0 ? e.thermo_dof : 3.0 * (double)e.n - 3.0;
 }
 
-inline int md_nchunk(int ncell) { return (ncell + kBinChunk - 1) / kBinChunk; }
+inline int md_nchunk(int ncell)
+{
+    return ncell > kBinChunkMinCells ? (ncell + kBinChunk - 1) / kBinChunk : 0;
+}
 
 inline MdBins md_bins(const Engine& e)
 {

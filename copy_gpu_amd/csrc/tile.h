@@ -119,7 +119,7 @@ __device__ inline void md_bin_wave(const MdBins& b, int buf, bool active, int s,
         int base = 0;
         if (lane == leader) {
             base = atomicAdd(&b.count[buf * b.ncell + cl], __popcll(grp));
-            atomicAdd(&b.chunk[buf * b.nchunk + cl / kBinChunk], __popcll(grp));
+            if (b.nchunk > 0) atomicAdd(&b.chunk[buf * b.nchunk + cl / kBinChunk], __popcll(grp));
         }
         base = __shfl(base, leader, 64);
         if ((grp >> lane) & 1ull) {

@@ -154,7 +154,10 @@ int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* 
  * mass FP64 (n) or NULL (=1).  With RBE, k-vectors are resampled every step
  * from (seed, step0 + s).  use_graph != 0 captures sixteen steps (and the
  * nsteps % 16 remainder) in HIP graphs, cached per handle, and replays them;
- * graph and eager runs give the same bits.  Positions/velocities/forces are updated in place. */
+ * graph and eager runs give the same bits.  A list rebuild takes its reference
+ * positions from the binning one step earlier, so the lists are exact while no
+ * atom moves more than skin/2 in one step; a run in which one did returns
+ * RBMD_ERR_STATE (use a larger skin or a smaller dt).  Positions/velocities/forces are updated in place. */
 int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, double* vy,
                 double* vz, void* fx, void* fy, void* fz, int32_t* img, const double* mass,
                 double dt, int64_t nsteps, uint64_t seed, uint64_t step0, int32_t use_graph,

@@ -489,6 +489,72 @@ def test_md_matches_oracle_integrator(orc, gpu):
     assert np.abs(Vg - np.stack(v, 1)).max() < 1e-8
 
 
+@pytest.mark.parametrize("n,chunked", [(4000, False), (40000, True)])
+def test_md_many_cells_chunked_cell_sort(orc, gpu, n, chunked):
+    """Cells of about one atom (cutoffs 1.0, skin 0.1): 9,261 cells at 4,000 ions and
+    97,336 at 40,000; above 16,384 cells the MD cell sort takes each cell's start from
+    256-cell chunk counts kept beside the cell counts.  FP64 LJ + real-space Coulomb MD
+    with several rebuilds against the oracle loop (the hot start moves atoms up to
+    0.023 per step, under half the skin)."""
+    dt, steps = 0.002, 12
+    x, y, z, q, L, vx, vy, vz = _state(n, seed=47)
+    Lb = np.array([L] * 3)
+    lo = np.zeros(3)
+    _log(f"case: {n} ions, cutoffs 1.0, skin 0.1, {steps} FP64 MD steps with rebuilds")
+    f = _make(Lb, n, precision=64, kspace="NONE", rc_lj=1.0, rc_coul=1.0, skin=0.1)
+    f.Init(_dev(q))
+    assert (f.stats()["cells"] > 16384) == chunked
+    X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+    F = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
+    f.Execute(X, Y, Z, *F)
+    b0 = f.stats()["builds"]
+    f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=dt, nsteps=steps, seed=SEED, step0=1, use_graph=True)
+    torch.cuda.synchronize()
+    assert f.stats()["builds"] - b0 >= 2
+
+    def forces(px, py, pz):
+        Fp, _ = orc.pair_forces(px, py, pz, q, lo, Lb, 1.0, 1.0, 0.5)
+        return Fp
+
+    p = list(orc.wrap(x, y, z, lo, Lb))
+    v = [vx, vy, vz]
+    Fo = forces(*p)
+    for s in range(1, steps + 1):
+        v = orc.kick(v, [Fo[:, 0], Fo[:, 1], Fo[:, 2]], dt)
+        p, _ = orc.drift(p, v, dt, lo, Lb)
+        Fo = forces(*p)
+        v = orc.kick(v, [Fo[:, 0], Fo[:, 1], Fo[:, 2]], dt)
+    Pg = np.stack([X.cpu().numpy(), Y.cpu().numpy(), Z.cpu().numpy()], 1)
+    dpos = Pg - np.stack(p, 1)
+    dpos -= Lb * np.rint(dpos / Lb)
+    bad = np.nonzero(np.abs(dpos).max(1) > 1e-9)[0]
+    _log(f"  builds {f.stats()['builds'] - b0}, atoms off by > 1e-9: {bad.size} {bad[:8]}")
+    assert np.abs(dpos).max() < 1e-9, np.abs(dpos).max()
+    Vg = np.stack([VX.cpu().numpy(), VY.cpu().numpy(), VZ.cpu().numpy()], 1)
+    assert np.abs(Vg - np.stack(v, 1)).max() < 1e-8
+
+
+def test_md_step_longer_than_half_skin_is_an_error(gpu):
+    """A rebuild takes its reference positions from the binning one step earlier, so
+    an MD step that moves an atom more than half the skin can leave a pair out of the
+    list: the run reports it.  The hot start of 4,000 ions moves an atom 0.0195 in one
+    step (oracle loop), past half of a 0.02 skin."""
+    from copy_gpu_amd import RBMDError
+
+    n = 4000
+    x, y, z, q, L, vx, vy, vz = _state(n, seed=47)
+    Lb = np.array([L] * 3)
+    _log("case: MD with a skin shorter than twice the largest step (error expected)")
+    f = _make(Lb, n, precision=64, kspace="NONE", rc_lj=1.0, rc_coul=1.0, skin=0.02)
+    f.Init(_dev(q))
+    X, Y, Z, VX, VY, VZ = (_dev(a) for a in (x, y, z, vx, vy, vz))
+    F = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
+    f.Execute(X, Y, Z, *F)
+    with pytest.raises(RBMDError, match="half the Verlet skin"):
+        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=0.002, nsteps=12, seed=SEED, step0=1)
+    torch.cuda.synchronize()
+
+
 def test_rebuild_trigger(gpu):
     _log("case: list rebuild only after half-skin displacement")
     x, y, z, q, L = _ions(2000, seed=41)
