@@ -62,7 +62,7 @@ struct PairMath;
 template <>
 struct PairMath<float> {
     template <bool LJ>
-    __device__ static inline float eval(double d2, double qq_d, float eps24, float sig2,
+    __device__ static inline float eval(double d2, float qq, float eps24, float sig2,
                                         const PairScalars& ps, bool ok, float* e_lj,
                                         float* e_re, bool ev)
     {
@@ -74,11 +74,11 @@ struct PairMath<float> {
             const float sr2 = sig2 * r2inv;
             const float sr6 = sr2 * sr2 * sr2;
             const bool in = ok && d2 < ps.rcl2;
-            fs = in ? -eps24 * (2.0f * sr6 - 1.0f) * sr6 * r2inv : 0.0f;
+            // -eps24 (2 sr6 - 1) sr6 / r^2, as (sr6 / r^2) (eps24 - 2 eps24 sr6)
+            fs = in ? (sr6 * r2inv) * fmaf(-2.0f * eps24, sr6, eps24) : 0.0f;
             if (ev && in) *e_lj += 0.5f * (eps24 * (1.0f / 6.0f)) * (sr6 * sr6 - sr6);
         }
         // G(r) / r = e2 r2inv (P rinv + tsp), P = erfc(sqrt(alpha) r) e^{alpha r^2}
-        const float qq = (float)qq_d;
         const float r = r2 * rinv;
         const float e2 = __builtin_amdgcn_exp2f(ps.nal2e * r2);
         const float P = erfc_scaled_as(r, ps.pa);
@@ -119,7 +119,7 @@ struct PairMath<double> {
 // Force factor fs of one pair (force on i = fs * r_ij, r_ij = r_j - r_i) and,
 // with EV, its two energies.  qq already carries the Coulomb unit factor.
 template <typename T, bool EV, bool MULTI, bool LJ>
-__device__ inline T pair_fs(double d2, double qq, int ti, int tj, bool ok, const PairScalars& ps,
+__device__ inline T pair_fs(double d2, T qq, int ti, int tj, bool ok, const PairScalars& ps,
                             const PairParams* __restrict__ pp, T* e_lj, T* e_re)
 {
     T eps24 = 0, sig2 = 0;
@@ -156,7 +156,7 @@ __device__ inline float rbl_weight(const RblPair& r, double d2, bool sel)
 }
 
 template <typename T, bool EV, bool MULTI, bool RBL = false>
-__device__ inline void accumulate_pair(bool lj, double dx, double dy, double dz, double qq, int ti,
+__device__ inline void accumulate_pair(bool lj, double dx, double dy, double dz, T qq, int ti,
                                        int tj, bool valid, const PairScalars& ps,
                                        const PairParams* __restrict__ pp, double& fx, double& fy,
                                        double& fz, double* acc, const RblPair& rb = RblPair{},
@@ -188,6 +188,16 @@ __device__ inline void accumulate_pair(bool lj, double dx, double dy, double dz,
         acc[6] += h * dx * dz;
         acc[7] += h * dy * dz;
     }
+}
+
+// Charge in the staged record of the full-row pass: FP64, or in FP32 passes the
+// FP32 charge's bits in the low word, so a pair's charge product is one FP32
+// multiply (records read from global memory carry the FP64 charge).
+template <typename T, bool STAGED>
+__device__ inline T rec_q(double w)
+{
+    if constexpr (STAGED) return rho_staged_q<T>(w);
+    return (T)w;
 }
 
 // Position of tile-local candidate u, already moved to the image next to the
@@ -311,6 +321,7 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
     constexpr int kPer = 64 / kGroup;  // atoms per wave
     const int g = lane / kGroup, gl = lane % kGroup;
     const uint16_t* __restrict__ rows = a.nlist + (size_t)uniform(h0) * cap;  // the tile's rows
+    const char* __restrict__ rowb = (const char*)rows;
     for (int t0 = a0 + kPer * wv; t0 < a1; t0 += kPer * kTileWaves) {
         const int s = t0 + g;
         const bool live = s < a1;
@@ -324,21 +335,24 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
         const int ui = h.off[kHomeSeg] + (live ? s - h0 : 0);
         const double4 pi = STAGED ? lds[ui] : a.xs[live ? s : h0];
         const int ti = MULTI ? (STAGED ? ltype[ui] : a.ts[live ? s : h0]) : 0;
-        const double qi = pi.w * ps.qqr2e;
+        const T qi = rec_q<T, STAGED>(pi.w) * (T)ps.qqr2e;
         const uint32_t rof = (uint32_t)(live ? s - h0 : 0) * (uint32_t)cap + (uint32_t)gl;
         double ax = 0, ay = 0, az = 0;
         auto rounds = [&](auto fast_tag) {
             constexpr bool FAST = decltype(fast_tag)::value;
             // row indices of round base: tile-local, the sentinel past the end
             // (FAST), -1 past the end otherwise
+            // (32-bit byte offsets from the workgroup-uniform row base: one
+            // address operation per load)
             auto load_u = [&](int base, int* u) {
 #pragma unroll
                 for (int k = 0; k < kRound; ++k) {
                     const int v = base + kGroup * k;
+                    const uint32_t e = FAST ? rof + (uint32_t)min(v, tl - gl) : rof + (uint32_t)v;
                     if (FAST)
-                        u[k] = (int)rows[rof + (uint32_t)min(v, tl - gl)];
+                        u[k] = (int)*(const uint16_t*)(rowb + 2u * e);
                     else
-                        u[k] = v + gl < tl ? (int)rows[rof + (uint32_t)v] : -1;
+                        u[k] = v + gl < tl ? (int)*(const uint16_t*)(rowb + 2u * e) : -1;
                 }
             };
             int un[kRound];
@@ -369,7 +383,8 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
                     if (base + kGroup * k < tmax)
                         accumulate_pair<T, EV, MULTI, RBL>(
                             base + kGroup * k < nin_max, pj[k].x - pi.x, pj[k].y - pi.y,
-                            pj[k].z - pi.z, qi * pj[k].w, ti, tj[k], FAST || u[k] >= 0, ps, pp,
+                            pj[k].z - pi.z, qi * rec_q<T, STAGED>(pj[k].w), ti, tj[k],
+                            FAST || u[k] >= 0, ps, pp,
                             ax, ay, az, acc, rbl.w, sel[k]);
                 }
             }
@@ -388,7 +403,8 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
                     sel = rbl_select(rbl.seed, epoch, id_i, tile_atom_id(h, a.perm, uu),
                                      rbl.pice);
                 accumulate_pair<T, EV, MULTI, RBL>(true, pj.x - pi.x, pj.y - pi.y, pj.z - pi.z,
-                                                   qi * pj.w, ti, tj, live && over, ps, pp, ax,
+                                                   qi * rec_q<T, STAGED>(pj.w), ti, tj,
+                                                   live && over, ps, pp, ax,
                                                    ay, az, acc, rbl.w, sel);
             }
         }
@@ -448,7 +464,8 @@ __device__ inline void block_fsum(const double* fs, int lane, int wv, double* __
 // Tile pass: one workgroup per (cell tile, part).  On a rebuild step it first
 // builds the Verlet rows of its home atoms (list_wave over the tile's reference
 // positions), then the pair pass: the tile's current positions are staged in
-// LDS shifted to the home cell's image (and types when there are several);
+// LDS shifted to the home cell's image, with the charge (rec_q) and the types
+// when there are several;
 // each group of kGroup lanes walks one home atom's uint16 row -- LJ-range
 // entries first, so later rounds skip the LJ code -- then reduces across its
 // lanes.  Tiles larger than the LDS stage take the same steps from global
@@ -552,7 +569,7 @@ struct TilePass {
                     const int u = u0 + k * kTileThreads;
                     if (u < total) {
                         lds[u] = make_double4(p[k].x - h.sx[st[k]], p[k].y - h.sy[st[k]],
-                                              p[k].z - h.sz[st[k]], p[k].w);
+                                              p[k].z - h.sz[st[k]], rho_stage_q<T>(p[k].w));
                         if (MULTI) ltype[u] = t[k];
                     }
                 }
@@ -651,8 +668,9 @@ __device__ inline void accumulate_half(bool lj, double dx, double dy, double dz,
     const double d2 = dx * dx + dy * dy + dz * dz;
     const bool ok = valid && d2 > 1e-4;
     float e_lj = 0, e_re = 0;
-    const float fs = lj ? pair_fs<float, EV, MULTI, true>(d2, qq, ti, tj, ok, ps, pp, &e_lj, &e_re)
-                        : pair_fs<float, EV, MULTI, false>(d2, qq, ti, tj, ok, ps, pp, &e_lj,
+    const float qf = (float)qq;
+    const float fs = lj ? pair_fs<float, EV, MULTI, true>(d2, qf, ti, tj, ok, ps, pp, &e_lj, &e_re)
+                        : pair_fs<float, EV, MULTI, false>(d2, qf, ti, tj, ok, ps, pp, &e_lj,
                                                            &e_re);
     const double f = (double)fs;
     fx = fma(f, dx, fx);
