@@ -200,14 +200,35 @@ __device__ inline T rec_q(double w)
     return (T)w;
 }
 
+// LDS stage of the full-row pass: record u is (x, y) at xy[u] and (z, charge word)
+// at zq[u], two arrays of 16-byte entries rather than one of 32-byte records, so a
+// wave's random gathers of either half spread over all the LDS banks instead of the
+// half that 32-byte strides reach.
+struct FullStage {
+    char* xy;       // LDS base of the (x, y) array
+    uint32_t zoff;  // bytes from xy[u] to zq[u]
+    __device__ double4 get(int u) const
+    {
+        const char* p = xy + 16u * (uint32_t)u;
+        const double2 a = *(const double2*)p, b = *(const double2*)(p + zoff);
+        return make_double4(a.x, a.y, b.x, b.y);
+    }
+    __device__ void put(int u, const double4& v) const
+    {
+        char* p = xy + 16u * (uint32_t)u;
+        *(double2*)p = make_double2(v.x, v.y);
+        *(double2*)(p + zoff) = make_double2(v.z, v.w);
+    }
+};
+
 // Position of tile-local candidate u, already moved to the image next to the
 // home cell (from LDS when the tile is staged, else from global memory).
-// The row sentinel u = total reads the tile's dummy entry (staged at lds[total]).
+// The row sentinel u = total reads the tile's dummy entry (staged at record total).
 template <bool STAGED>
-__device__ inline double4 tile_pos(const TileHdr& h, const double4* __restrict__ lds,
+__device__ inline double4 tile_pos(const TileHdr& h, const FullStage& lds,
                                    const double4* __restrict__ xs, int u)
 {
-    if (STAGED) return lds[u];
+    if (STAGED) return lds.get(u);
     const int total = h.off[27];
     const int uc = u < total ? u : 0;
     const int st = tile_seg(h, uc);
@@ -307,8 +328,7 @@ struct RblArgs {
 };
 
 template <typename T, bool EV, bool MULTI, bool STAGED, bool RBL = false>
-__device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
-                                 const double4* __restrict__ lds,
+__device__ inline void pair_wave(const PairArgs& a, const TileHdr& h, const FullStage& lds,
                                  const int32_t* __restrict__ ltype, int a0, int a1, int h0, int wv,
                                  int lane, const PairScalars& ps,
                                  const PairParams* __restrict__ pp, T* __restrict__ fx,
@@ -333,7 +353,7 @@ __device__ inline void pair_wave(const PairArgs& a, const TileHdr& h,
         const int nin_max = wave_max_int(over ? 0 : n_in);
         const bool fast = __ballot(!live || over) == 0ull;
         const int ui = h.off[kHomeSeg] + (live ? s - h0 : 0);
-        const double4 pi = STAGED ? lds[ui] : a.xs[live ? s : h0];
+        const double4 pi = STAGED ? lds.get(ui) : a.xs[live ? s : h0];
         const int ti = MULTI ? (STAGED ? ltype[ui] : a.ts[live ? s : h0]) : 0;
         const T qi = rec_q<T, STAGED>(pi.w) * (T)ps.qqr2e;
         const uint32_t rof = (uint32_t)(live ? s - h0 : 0) * (uint32_t)cap + (uint32_t)gl;
@@ -506,6 +526,7 @@ struct TilePass {
         const PairArgs& a = A.a;
         extern __shared__ double4 lds[];
         int32_t* ltype = (int32_t*)(lds + A.stage_cap);
+        const FullStage fst{(char*)lds, 16u * (uint32_t)A.stage_cap};
         __shared__ TileHdr h;
         const int tb = A.order ? A.order[blk] : blk;
         const int c = tb / A.split, part = tb % A.split;
@@ -568,14 +589,14 @@ struct TilePass {
                 for (int k = 0; k < 2; ++k) {
                     const int u = u0 + k * kTileThreads;
                     if (u < total) {
-                        lds[u] = make_double4(p[k].x - h.sx[st[k]], p[k].y - h.sy[st[k]],
-                                              p[k].z - h.sz[st[k]], rho_stage_q<T>(p[k].w));
+                        fst.put(u, make_double4(p[k].x - h.sx[st[k]], p[k].y - h.sy[st[k]],
+                                                p[k].z - h.sz[st[k]], rho_stage_q<T>(p[k].w)));
                         if (MULTI) ltype[u] = t[k];
                     }
                 }
             }
             if (threadIdx.x == 0) {
-                lds[total] = h.dummy;
+                fst.put(total, h.dummy);
                 if (MULTI) ltype[total] = 0;
             }
         }
@@ -583,10 +604,10 @@ struct TilePass {
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         double fsum[3] = {0, 0, 0};
         if (staged)
-            pair_wave<T, EV, MULTI, true, RBL>(a, h, lds, ltype, a0, a1, h0, wv, lane, A.ps,
+            pair_wave<T, EV, MULTI, true, RBL>(a, h, fst, ltype, a0, a1, h0, wv, lane, A.ps,
                                                A.pp, A.fx, A.fy, A.fz, acc, A.rbl, epoch, fsum);
         else
-            pair_wave<T, EV, MULTI, false, RBL>(a, h, nullptr, nullptr, a0, a1, h0, wv, lane,
+            pair_wave<T, EV, MULTI, false, RBL>(a, h, FullStage{}, nullptr, a0, a1, h0, wv, lane,
                                                 A.ps, A.pp, A.fx, A.fy, A.fz, acc, A.rbl, epoch,
                                                 fsum);
         if (EV) block_ev(acc, lane, wv, A.ev_part + (size_t)tb * 8);
