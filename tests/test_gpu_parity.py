@@ -129,6 +129,65 @@ def test_neighbor_sets_ragged_box_offset_origin(orc, gpu):
     _check_sets(orc, f, x, y, z, lo, Lb, 4.3)
 
 
+def _list_boundary_pairs(rl2):
+    """Atom pairs whose FP64 squared distance, in the reference's operation order, is
+    each double from 3 below to 3 above the largest d2 that (rl2 - d2) > 1e-4 accepts.
+    A is at (18, y0, z0) with y0, z0 on a grid of 12 (whole numbers below 64), B at
+    dx ~ sqrt(rl2) along x plus dy, dz of a few 2^-24: xB - 18 is exact (Sterbenz), dx*dx
+    is a multiple of 2^-48 in [16, 32), y0 + dy and z0 + dz are exact and so are their
+    differences, and dy^2, dz^2 are small multiples of 2^-48, so both FMAs of
+    d2 = fma(dz, dz, fma(dy, dy, dx dx)) are exact sums and every d2 near the boundary is
+    reachable."""
+    t = rl2 - 1e-4
+    while not (rl2 - t) > 1e-4:
+        t = np.nextafter(t, -np.inf)
+    while (rl2 - np.nextafter(t, np.inf)) > 1e-4:
+        t = np.nextafter(t, np.inf)
+    want = [t]
+    for _ in range(3):
+        want.insert(0, np.nextafter(want[0], -np.inf))
+        want.append(np.nextafter(want[-1], np.inf))
+    found = {}
+    xb = 18.0 + np.sqrt(want[0]) - 1e-13
+    for _ in range(80):
+        dx = xb - 18.0
+        base = dx * dx
+        for a in range(6):
+            for b in range(6):
+                d2 = base + (a * a + b * b) * 2.0 ** -48
+                if d2 in want and d2 not in found:
+                    found[d2] = (xb, a * 2.0 ** -24, b * 2.0 ** -24)
+        xb = np.nextafter(xb, np.inf)
+    assert sorted(found) == sorted(want), "boundary distances not all reachable"
+    return t, [found[d] for d in want]
+
+
+@pytest.mark.parametrize("half_shell", [False, True])
+def test_list_criterion_at_its_exact_boundary(orc, gpu, half_shell):
+    """The reference's list criterion (r_list^2 - d2) > 1e-4 at its exact edge: pairs whose
+    squared distance is each double from t - 3 ulp to t + 3 ulp (t the largest d2 it
+    accepts), in the full-row and the half-shell builder: the rows equal the oracle's,
+    and exactly the four pairs at or below t are kept."""
+    _log(f"case: list criterion at its boundary, half_shell={half_shell}")
+    rl2 = 5.3 * 5.3
+    t, pairs = _list_boundary_pairs(rl2)
+    L = 72.0
+    pos = []
+    for k, (xb, dy, dz) in enumerate(pairs):
+        gy, gz = 12.0 * (k % 4), 12.0 * (k // 4)
+        pos.append((18.0, 18.0 + gy, 18.0 + gz))
+        pos.append((xb, 18.0 + gy + dy, 18.0 + gz + dz))
+    x, y, z = (np.array([p[d] for p in pos]) for d in range(3))
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, rc_coul=5.0, half_shell=half_shell)
+    f.Init(_dev(np.zeros(x.size)))
+    _compute(f, x, y, z)
+    _check_sets(orc, f, x, y, z, np.zeros(3), Lb, 5.3)
+    og, ig = f.neighbors()
+    kept = [bool(og[2 * k + 1] > og[2 * k]) for k in range(len(pairs))]
+    assert kept == [True] * 4 + [False] * 3, kept
+
+
 # ----------------------------------------------------------------------------- pair pass
 
 @pytest.mark.parametrize("precision,tol", [(32, 1e-5), (64, 1e-12)])
