@@ -16,8 +16,9 @@ run's remainder) before the timed region and replayed.
 
 Prints ONE JSON line on stdout (logs go to stderr).  Also reported:
   * roofline of the pair pass (the dominant kernel): algorithmic bytes
-    4 E + 40 N per launch over its mean launch time from HIP events on its
-    stream (E = full-list entries at r_list), against 8 TB/s;
+    4 E + 40 N per launch over its mean launch time (20 launches back to back
+    between one pair of HIP events on its stream; E = full-list entries at
+    r_list), against 8 TB/s;
   * the CPU baseline: the reference-faithful step on the host cores for a bounded
     sample of steps of the same workload;
   * beside the headline: the 32,768-ion plain-RBE workload (timed steps, pair-pass
@@ -239,15 +240,13 @@ def kernel_times(f, st, wl, Kt, step):
     for nm in names:
         ms, cnt = f.timing(nm)
         per[nm] = {"mean_ms": ms, "launches": cnt}
-    # the pair pass alone: force evaluations at the final positions reuse the
-    # list (no rebuild), so the tile pass times only the pair forces
+    # the pair pass alone at the final positions: 20 launches back to back between
+    # one pair of HIP events on the launch stream (the library's time_pair), the
+    # time a kernel trace of the same launches gives
     f.set_timing(False)
-    f.set_timing(True)
-    for _ in range(20):
-        f.Execute(st.X, st.Y, st.Z, *st.F)
+    pair_cnt = 20
+    pair_ms = f.time_pair(st.X, st.Y, st.Z, *st.F, reps=pair_cnt)
     torch.cuda.synchronize()
-    pair_ms, pair_cnt = f.timing("pair")
-    f.set_timing(False)
     E = f.stats()["entries"]
     n = wl["n"]
     pair_bytes = 4.0 * E + 40.0 * n
@@ -259,7 +258,8 @@ def kernel_times(f, st, wl, Kt, step):
 
 def pair_pass_at_scale(wname, reps=20):
     """Pair-pass roofline of another workload at fixed positions (its starting
-    configuration): list built once, then reps force calls timed with HIP events."""
+    configuration): list built once, then reps pair-pass launches back to back
+    between one pair of HIP events."""
     import torch
 
     from copy_gpu_amd import systems
@@ -273,15 +273,9 @@ def pair_pass_at_scale(wname, reps=20):
     f = make_force(wl, L, wl["P"])
     f.Init(Q)
     f.sample_kvectors(wl["P"], SEED, 0)
-    f.Execute(X, Y, Z, *F)
-    f.Execute(X, Y, Z, *F)
+    pair_ms = f.time_pair(X, Y, Z, *F, reps=reps)
+    cnt = reps
     torch.cuda.synchronize()
-    f.set_timing(True)
-    for _ in range(reps):
-        f.Execute(X, Y, Z, *F)
-    torch.cuda.synchronize()
-    pair_ms, cnt = f.timing("pair")
-    f.set_timing(False)
     E = f.stats()["entries"]
     f.close()
     nbytes = 4.0 * E + 40.0 * n

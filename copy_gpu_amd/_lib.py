@@ -146,6 +146,7 @@ _SIGS = {
     "rbmd_force_set_timing": (C.c_int, [_P, C.c_int32]),
     "rbmd_force_get_timing": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64)]),
+    "rbmd_force_time_pair": (C.c_int, [_P] + [_P] * 6 + [C.c_int32, C.POINTER(C.c_double), _P]),
     "rbmd_util_copy": (C.c_int, [_P, _P, C.c_int64, _P]),
     "rbmd_kspace_compute": (C.c_int, [C.POINTER(KspaceParams), C.c_int64] + [_P] * 4 +
                             [C.c_int32] + [_P] * 8),

@@ -116,7 +116,7 @@ static void free_all(Engine& E)
                     E.ts,       E.cell_count, E.cell_start, E.atom_cell, E.atom_rank, E.nlist,
                     E.tile_order,
                     E.md_count, E.md_bucket, E.md_cell, E.md_vid, E.md_w,
-                    E.ncount,   E.flags, E.stats,  E.q,      E.type,      E.d_pp,     E.km,
+                    E.ncount,   E.stats, E.q,      E.type,      E.d_pp,     E.km,
                     E.kw,       E.kpart, E.rho,    E.kcoef,  E.kev,       E.kcdf,     E.d_step,
                     E.ev_part,  E.ev,    E.gp_cdf,  E.gp_ratio, E.gp_guide, E.thermo_part,
                     E.thermo_st, E.rbl_part, E.rbl_mean, E.planes};
@@ -446,8 +446,10 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
     }
     CR(dalloc(&E.cell_start, ((size_t)E.ncell + 1) * 4));
     CR(dalloc(&E.ncount, N * sizeof(int2)));
-    CR(dalloc(&E.flags, 64));
-    CR(dalloc(&E.stats, 128));
+    // the build statistics (128 bytes) and the flags (64) share one block, so a run
+    // reads both back with one copy
+    CR(dalloc(&E.stats, 192));
+    E.flags = (uint32_t*)((char*)E.stats + 128);
     CR(dalloc(&E.q, N * 8));
     CR(dalloc(&E.type, N * 4));
     CR(dalloc(&E.d_pp, sizeof(PairParams)));
@@ -468,7 +470,7 @@ int rbmd_force_create(const rbmd_force_params* p, int64_t n_max, rbmd_force_t* o
         CR(dalloc(&E.rbl_mean, 4 * 8));
         CR(hipMemset(E.rbl_mean, 0, 4 * 8));
     }
-    CR(hipHostMalloc((void**)&E.h_stats, 64, hipHostMallocDefault));
+    CR(hipHostMalloc((void**)&E.h_stats, 192, hipHostMallocDefault));
     CR(hipStreamCreateWithFlags(&E.own_stream, hipStreamNonBlocking));
     CR(hipEventCreateWithFlags(&E.order_event, hipEventDisableTiming));
     CR(hipMemset(E.flags, 0, 64));
@@ -714,6 +716,41 @@ int rbmd_force_compute(rbmd_force_t h, const double* x, const double* y, const d
     return RBMD_OK;
 }
 
+int rbmd_force_time_pair(rbmd_force_t h, const double* x, const double* y, const double* z,
+                         void* fx, void* fy, void* fz, int32_t reps, double* mean_ms,
+                         void* stream)
+{
+    if (!h || !mean_ms) return RBMD_ERR_INVALID;
+    Engine& E = h->e;
+    if (reps < 1 || reps > 10000) return fail(E, RBMD_ERR_INVALID, "reps out of [1, 10000]");
+    *mean_ms = 0.0;
+    // two evaluations: the first may rebuild the rows, the second leaves no rebuild
+    // pending, so the launches below are the pair pass alone
+    for (int k = 0; k < 2; ++k) {
+        const int rc = rbmd_force_compute(h, x, y, z, fx, fy, fz, 0, stream);
+        if (rc != RBMD_OK) return rc;
+    }
+    if (E.n == 0) return RBMD_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const bool timing = E.timing;
+    E.timing = false;  // no per-launch events between the two below
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, s));
+    for (int r = 0; r < reps; ++r) launch_pair(E, fx, fy, fz, false, s);
+    HIP_TRY(hipEventRecord(b, s));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    HIP_TRY(hipEventDestroy(a));
+    HIP_TRY(hipEventDestroy(b));
+    E.timing = timing;
+    *mean_ms = (double)ms / (double)reps;
+    // the arrays end holding a whole evaluation again (pair plus k-space forces)
+    return rbmd_force_compute(h, x, y, z, fx, fy, fz, 0, stream);
+}
+
 int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* stream)
 {
     if (!h || !out) return RBMD_ERR_INVALID;
@@ -801,9 +838,9 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     int rc = maintain(E, s);
     if (rc != RBMD_OK) return rc;
     // bin the current positions for a rebuild in the first step (later steps'
-    // bins come from the tile pass of the step before)
-    HIP_TRY(hipMemsetAsync(E.flags + 4, 0, sizeof(uint32_t), s));
-    launch_md_bin(E, x, y, z, s);
+    // bins come from the tile pass of the step before); the binning kernel also
+    // clears the long-step flag and sets the graphs' step counter to step0
+    launch_md_bin(E, x, y, z, step0, s);
     if (E.thermo_type != kThermoNone) {
         if (!(thermo_dof(E) > 0))
             return fail(E, RBMD_ERR_STATE, "thermostat: no degrees of freedom (dof <= 0)");
@@ -866,18 +903,18 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
             HIP_TRY(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
             G.key = key;
         }
-        // device step counter = step0; the graph's rho_final kernel advances it
-        unsigned long long s0 = step0;
-        HIP_TRY(hipMemcpyAsync(E.d_step, &s0, 8, hipMemcpyHostToDevice, s));
+        // the device step counter holds step0 (set by the binning kernel); the
+        // graph's rho_final kernel advances it
         for (int64_t k = 0; k + kGraphSteps <= nsteps; k += kGraphSteps)
             HIP_TRY(hipGraphLaunch(E.graphs[0].exec, s));
         if (rem > 0) HIP_TRY(hipGraphLaunch(E.graphs[rem].exec, s));
     }
     if (E.thermo_type != kThermoNone) launch_thermo_end(E, rs, s);
-    HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost, s));
+    // the statistics and the flags after them: one copy, one wait
+    HIP_TRY(hipMemcpyAsync(E.h_stats, E.stats, 128 + 5 * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    uint32_t fl[5];
-    HIP_TRY(hipMemcpy(fl, E.flags, sizeof(fl), hipMemcpyDeviceToHost));
+    const uint32_t* fl = (const uint32_t*)((const char*)E.h_stats + 128);
     if (fl[1]) return fail(E, RBMD_ERR_INVALID, "non-finite positions during the run");
     if (fl[4])
         return fail(E, RBMD_ERR_STATE, kStepTooLong);
@@ -1362,6 +1399,14 @@ int rbmd_batch_create(int32_t nrep, const rbmd_force_params* params, const int64
     b->b.precision = params[0].precision;
     b->b.kspace = params[0].kspace;
     b->b.multi = params[0].ntypes > 1;
+    const size_t out_bytes = (size_t)nrep * kStatsOutWords * sizeof(unsigned long long);
+    if (hipMalloc((void**)&b->b.d_out, out_bytes) != hipSuccess ||
+        hipHostMalloc((void**)&b->b.h_out, out_bytes, hipHostMallocDefault) != hipSuccess) {
+        g_create_err = "batch readback allocation failed";
+        rbmd_batch_destroy(b);
+        return RBMD_ERR_HIP;
+    }
+    b->b.out_cap = (size_t)nrep;
     if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->order_event, hipEventDisableTiming) != hipSuccess) {
         g_create_err = "stream/event creation failed";
@@ -1382,6 +1427,8 @@ void rbmd_batch_destroy(rbmd_batch_t b)
     for (auto& ph : b->b.ph) ph.release();
     for (auto& ph : b->b.pair_more) ph.release();
     for (rbmd_force_t h : b->reps) rbmd_force_destroy(h);
+    if (b->b.d_out) hipFree(b->b.d_out);
+    if (b->b.h_out) hipHostFree(b->b.h_out);
     if (b->own_stream) hipStreamDestroy(b->own_stream);
     if (b->order_event) hipEventDestroy(b->order_event);
     delete b;
@@ -1448,16 +1495,16 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         o.mass = q.mass;
         Engine& E = *B.es[r];
         if (E.kspace == RBMD_KSPACE_RBE) B.sa[r] = sample_args(E, E.P, seeds[r], nullptr);
-        const unsigned long long s0 = step0;
-        BHIP_TRY(hipMemcpy(E.d_step, &s0, 8, hipMemcpyHostToDevice));
     }
+    // the binning kernel sets every replica's step counter to step0
+    B.step0 = step0;
     BHIP_TRY(nlist_batch_prepare(B));
     BHIP_TRY(pair_batch_prepare(B));
     BHIP_TRY(md_batch_prepare(B));
-    // bins of the current positions for a rebuild in the first step
-    for (Engine* E : B.es)
-        BHIP_TRY(hipMemsetAsync(E->md_count, 0, md_count_bytes(E->ncell), s));
-    for (Engine* E : B.es) BHIP_TRY(hipMemsetAsync(E->flags + 4, 0, sizeof(uint32_t), s));
+    // bins of the current positions for a rebuild in the first step: the counts
+    // cleared, then binned (which also clears the long-step flags and sets the step
+    // counters), one grid each over all replicas
+    nlist_batch_launch(B, BP_MDCLEAR, s);
     nlist_batch_launch(B, BP_MDBIN, s);
     nlist_batch_launch(B, BP_CHUNKSUM, s);
     if (B.thermo) md_batch_launch(B, BP_THERMO_BEGIN, s);
@@ -1502,12 +1549,17 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
         if (rem > 0) BHIP_TRY(hipGraphLaunch(b->graphs[rem].exec, s));
     }
     if (B.thermo) md_batch_launch(B, BP_THERMO_END, s);
+    // every replica's statistics and flags gathered into one block: one copy, one wait
+    nlist_batch_launch(B, BP_STATS_OUT, s);
+    BHIP_TRY(hipMemcpyAsync(B.h_out, B.d_out, R * kStatsOutWords * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, s));
     BHIP_TRY(hipStreamSynchronize(s));
     for (size_t r = 0; r < R; ++r) {
         Engine& E = *B.es[r];
+        const unsigned long long* o = B.h_out + r * kStatsOutWords;
+        memcpy(E.h_stats, o, 64);
         uint32_t fl[5];
-        BHIP_TRY(hipMemcpy(E.h_stats, E.stats, 64, hipMemcpyDeviceToHost));
-        BHIP_TRY(hipMemcpy(fl, E.flags, sizeof(fl), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 5; ++k) fl[k] = (uint32_t)o[8 + k];
         if (fl[1])
             return bfail(b, RBMD_ERR_INVALID,
                          "non-finite positions in replica " + std::to_string(r));

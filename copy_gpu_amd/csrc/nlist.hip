@@ -218,6 +218,8 @@ struct CellSort {
 // MD-step binning before the first step of a run: bins the current positions
 // into buffer (epoch & 1) of the MD bins (which the caller cleared), as the tile
 // pass of a previous step would have.
+// It also opens the run's device state: the long-step flag (flags[4]) cleared and
+// the graph's step counter set to the run's first step (no fill or copy launches).
 struct MdBin {
     static constexpr int kThreads = kBlock;
     struct Args {
@@ -227,15 +229,54 @@ struct MdBin {
         int3 dims;
         double3 cell;
         MdBins bins;
-        const uint32_t* flags;
+        uint32_t* flags;
+        unsigned long long* step;  // device step counter of graph replays
+        unsigned long long step0;
     };
     __device__ static void run(const Args& a, int blk, int)
     {
         const int i = blk * kBlock + threadIdx.x;
+        if (i == 0) {
+            a.flags[4] = 0u;
+            *a.step = a.step0;
+        }
         if (i >= a.n) return;
         const double p[3] = {a.x[i], a.y[i], a.z[i]};
         // any one-to-one slot numbering serves the cell sort: atom i bins in slot i
         md_bin_atom(a.bins, (int)(a.flags[3] & 1u), i, i, p, a.box, a.dims, a.cell);
+    }
+};
+
+// Zeroes n ints (a replica's MD bin counts before MdBin, one grid for the batch).
+struct ClearInts {
+    static constexpr int kThreads = kBlock;
+    struct Args {
+        int32_t* p;
+        int n;
+    };
+    __device__ static void run(const Args& a, int blk, int)
+    {
+        const int i = blk * kBlock + threadIdx.x;
+        if (i < a.n) a.p[i] = 0;
+    }
+};
+
+// One replica's build statistics and flags into its kStatsOutWords words of the
+// batch's readback block, so a batched run ends with one copy to the host.
+struct StatsOut {
+    static constexpr int kThreads = 64;
+    struct Args {
+        const unsigned long long* stats;
+        const uint32_t* flags;
+        unsigned long long* out;
+    };
+    __device__ static void run(const Args& a, int, int)
+    {
+        const int t = threadIdx.x;
+        if (t < 8)
+            a.out[t] = a.stats[t];
+        else if (t < 13)
+            a.out[t] = a.flags[t - 8];
     }
 };
 
@@ -380,10 +421,13 @@ Gather::Args gather_args(Engine& e, const double* x, const double* y, const doub
     return Gather::Args{x, y, z, e.xr, e.yr, e.zr, e.q, e.atom_rank, (int)e.n, e.box, e.xs};
 }
 
-MdBin::Args mdbin_args(Engine& e, const double* x, const double* y, const double* z)
+MdBin::Args mdbin_args(Engine& e, const double* x, const double* y, const double* z,
+                       unsigned long long step0)
 {
-    return MdBin::Args{x, y, z, (int)e.n, e.box, make_int3(e.dims[0], e.dims[1], e.dims[2]),
-                       make_double3(e.cell[0], e.cell[1], e.cell[2]), md_bins(e), e.flags};
+    return MdBin::Args{x,         y,           z,          (int)e.n, e.box,
+                       make_int3(e.dims[0], e.dims[1], e.dims[2]),
+                       make_double3(e.cell[0], e.cell[1], e.cell[2]),
+                       md_bins(e), e.flags,     e.d_step,   step0};
 }
 
 }  // namespace
@@ -421,10 +465,11 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
     // the Verlet rows are built inside the tile pass (pair.hip)
 }
 
-void launch_md_bin(Engine& e, const double* x, const double* y, const double* z, hipStream_t s)
+void launch_md_bin(Engine& e, const double* x, const double* y, const double* z,
+                   unsigned long long step0, hipStream_t s)
 {
     (void)hipMemsetAsync(e.md_count, 0, md_count_bytes(e.ncell), s);
-    k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z));
+    k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z, step0));
     if (md_nchunk(e.ncell) > 0)
         k_one<ChunkSum><<<md_nchunk(e.ncell), ChunkSum::kThreads, 0, s>>>(
             ChunkSum::Args{md_bins(e), e.flags});
@@ -461,6 +506,9 @@ hipError_t nlist_batch_prepare(Batch& b)
     std::vector<Scatter::Args> sp(R);
     std::vector<CellSort::Args> cs(R);
     std::vector<MdBin::Args> mb(R);
+    std::vector<ClearInts::Args> mc(R);
+    std::vector<StatsOut::Args> so(R);
+    std::vector<int> nmc(R);
     std::vector<ChunkSum::Args> cks(R);
     std::vector<Gather::Args> ga(R);
     std::vector<int> nat(R), none(R, 1), ncell(R), nch(R);
@@ -471,7 +519,11 @@ hipError_t nlist_batch_prepare(Batch& b)
         sc[r] = scan_args(e);
         sp[r] = scatter_args(e);
         cs[r] = cellsort_args(e);
-        mb[r] = mdbin_args(e, st.x, st.y, st.z);
+        mb[r] = mdbin_args(e, st.x, st.y, st.z, b.step0);
+        const int ncount = (int)(md_count_bytes(e.ncell) / sizeof(int32_t));
+        mc[r] = ClearInts::Args{e.md_count, ncount};
+        nmc[r] = (ncount + kBlock - 1) / kBlock;
+        so[r] = StatsOut::Args{e.stats, e.flags, b.d_out ? b.d_out + r * kStatsOutWords : nullptr};
         cks[r] = ChunkSum::Args{md_bins(e), e.flags};
         ga[r] = gather_args(e, st.x, st.y, st.z);
         nat[r] = nblk(e.n > 0 ? e.n : 1);
@@ -483,9 +535,12 @@ hipError_t nlist_batch_prepare(Batch& b)
     if ((err = phase_upload(b.ph[BP_SCAN], sc, none, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_SCATTER], sp, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_CELLSORT], cs, ncell, 0)) != hipSuccess) return err;
+    if ((err = phase_upload(b.ph[BP_MDCLEAR], mc, nmc, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_MDBIN], mb, nat, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_CHUNKSUM], cks, nch, 0)) != hipSuccess) return err;
     if ((err = phase_upload(b.ph[BP_GATHER], ga, nat, 0)) != hipSuccess) return err;
+    if (b.d_out && (err = phase_upload(b.ph[BP_STATS_OUT], so, none, 0)) != hipSuccess)
+        return err;
     return hipSuccess;
 }
 
@@ -496,7 +551,9 @@ void nlist_batch_launch(Batch& b, int phase, hipStream_t s)
     case BP_SCAN: phase_launch<Scan>(b.ph[phase], s); break;
     case BP_SCATTER: phase_launch<Scatter>(b.ph[phase], s); break;
     case BP_CELLSORT: phase_launch<CellSort>(b.ph[phase], s); break;
+    case BP_MDCLEAR: phase_launch<ClearInts>(b.ph[phase], s); break;
     case BP_MDBIN: phase_launch<MdBin>(b.ph[phase], s); break;
+    case BP_STATS_OUT: phase_launch<StatsOut>(b.ph[phase], s); break;
     case BP_CHUNKSUM: phase_launch<ChunkSum>(b.ph[phase], s); break;
     case BP_GATHER: phase_launch<Gather>(b.ph[phase], s); break;
     default: break;

@@ -192,6 +192,10 @@ struct RepState {
     const double* mass = nullptr;
 };
 
+// Words per replica in a batch's readback block: the 8 build statistics, then the
+// flags flags[0..4] one per word.
+constexpr int kStatsOutWords = 16;
+
 // Phases of the batched MD step, one launch each over all replicas.
 enum BatchPh {
     BP_KICKDRIFT = 0,  // segment opening kick/drift (md.hip)
@@ -199,6 +203,7 @@ enum BatchPh {
     BP_SCAN,
     BP_SCATTER,
     BP_CELLSORT,
+    BP_MDCLEAR,        // MD bin counts cleared before BP_MDBIN (nlist.hip)
     BP_MDBIN,
     BP_CHUNKSUM,       // chunk counts of the MD bins after BP_MDBIN (nlist.hip)
     BP_GATHER,
@@ -210,6 +215,7 @@ enum BatchPh {
     BP_THERMO_BEGIN,   // thermostatted replicas: state reset before the first step
     BP_THERMO_REDUCE,  // ... temperature and coefficient after each closing kick
     BP_THERMO_END,     // ... last velocity scale after the last step
+    BP_STATS_OUT,      // every replica's build statistics and flags into one block (nlist.hip)
     BP_NUM
 };
 
@@ -222,6 +228,12 @@ struct Batch {
     int kspace = 0;
     bool multi = false;
     bool thermo = false;  // some replica has a thermostat: one segment per step
+    unsigned long long step0 = 0;  // first step of the run (MdBin sets the counters)
+    // per replica kStatsOutWords words: the 8 build statistics, then flags[0..4];
+    // device block and its pinned host mirror, one copy per run
+    unsigned long long* d_out = nullptr;
+    unsigned long long* h_out = nullptr;
+    size_t out_cap = 0;  // replicas the blocks hold
     BatchPhase ph[BP_NUM];
     // the tile pass runs one launch per LDS class of replicas: ph[BP_PAIR] holds
     // the first class, pair_more the others (pair.hip pair_prepare_t)
@@ -276,7 +288,8 @@ void launch_check(Engine& e, const double* x, const double* y, const double* z, 
 void launch_force_rebuild(Engine& e, hipStream_t s);
 void launch_build(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
 void launch_gather(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
-void launch_md_bin(Engine& e, const double* x, const double* y, const double* z, hipStream_t s);
+void launch_md_bin(Engine& e, const double* x, const double* y, const double* z,
+                   unsigned long long step0, hipStream_t s);
 void launch_list_stats(Engine& e, unsigned long long* out, hipStream_t s);
 void launch_copy(void* dst, const void* src, int64_t nbytes, hipStream_t s);
 

@@ -248,6 +248,19 @@ class LJCutCoulKspaceForce:
                                                    C.byref(cnt)), "get_timing")
         return ms.value, cnt.value
 
+    def time_pair(self, x, y, z, fx, fy, fz, reps=20, stream=None):
+        """Mean device ms of the pair pass alone at (x, y, z): reps launches back to
+        back between one pair of HIP events; fx, fy, fz end holding the forces."""
+        for t in (x, y, z):
+            assert t.is_cuda and t.dtype == torch.float64 and t.numel() == self.n
+        for t in (fx, fy, fz):
+            assert t.is_cuda and t.dtype == self.fdtype and t.numel() == self.n
+        ms = C.c_double()
+        self._check(self.lib.rbmd_force_time_pair(self.h, _ptr(x), _ptr(y), _ptr(z), _ptr(fx),
+                                                  _ptr(fy), _ptr(fz), reps, C.byref(ms),
+                                                  _stream(stream)), "time_pair")
+        return ms.value
+
     # ------------------------------------------------------------ velocity Verlet
     _THERMO = {"NONE": L.THERMO_NONE, "BERENDSEN": L.THERMO_BERENDSEN,
                "TEMP_RESCALE": L.THERMO_RESCALE, "NOSE_HOOVER": L.THERMO_NOSE_HOOVER,

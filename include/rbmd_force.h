@@ -256,6 +256,17 @@ int rbmd_force_autotune(rbmd_force_t h, const double* x, const double* y, const 
 int rbmd_force_set_timing(rbmd_force_t h, int32_t enable);
 int rbmd_force_get_timing(rbmd_force_t h, const char* pass, double* mean_ms, int64_t* launches);
 
+/* Device time of the pair pass alone: two rbmd_force_compute calls at (x, y, z)
+ * (the first may rebuild the rows), then reps launches of the tile pass back to
+ * back between one pair of HIP events; *mean_ms = elapsed / reps, which a kernel
+ * trace of the same launches reproduces (the per-launch events of
+ * rbmd_force_get_timing add their own latency to every launch).  fx, fy, fz end
+ * holding the forces of one more rbmd_force_compute.  No counterpart in the
+ * reference: a measurement aid.  reps in [1, 10000]; synchronises the stream. */
+int rbmd_force_time_pair(rbmd_force_t h, const double* x, const double* y, const double* z,
+                         void* fx, void* fy, void* fz, int32_t reps, double* mean_ms,
+                         void* stream);
+
 /* ---- the k-space step without a force handle ----
  *
  * Structure factor rho(k) = sum_j q_j exp(i k.r_j) over a given k-list and the

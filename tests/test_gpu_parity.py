@@ -253,6 +253,26 @@ def test_half_shell_rows_equal_full_rows(orc, gpu, types):
     assert evg["e_real"] == evh["e_real"]
 
 
+def test_time_pair_leaves_a_whole_evaluation(gpu):
+    """rbmd_force_time_pair: a positive mean per pair-pass launch, and the force arrays
+    end holding the same bits as a plain compute at the same positions (pair plus
+    k-space forces)."""
+    _log("case: time_pair, 8000 ions, RBE P=100")
+    x, y, z, q, L = _ions(8000)
+    Lb = np.array([L] * 3)
+    f = _make(Lb, x.size, kspace="RBE")
+    f.Init(_dev(q))
+    f.sample_kvectors(100, SEED, 0)
+    F0 = _compute(f, x, y, z)
+    X, Y, Z = _dev(x), _dev(y), _dev(z)
+    F = [torch.zeros(x.size, device="cuda", dtype=f.fdtype) for _ in range(3)]
+    ms = f.time_pair(X, Y, Z, *F, reps=5)
+    torch.cuda.synchronize()
+    F1 = np.stack([t.double().cpu().numpy() for t in F], 1)
+    assert 0.0 < ms < 100.0, ms
+    assert np.array_equal(F1, F0)
+
+
 def test_pair_forces_two_types(orc, gpu):
     _log("case: pair forces FP64, two LJ types")
     x, y, z, q, L = _ions(3000, seed=3)
