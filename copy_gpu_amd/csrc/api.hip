@@ -825,16 +825,12 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
     if (E.kspace == RBMD_KSPACE_RBE && E.P < 1)
         return fail(E, RBMD_ERR_STATE, "no k-vectors: sample once to fix P");
     if (E.n == 0 || nsteps <= 0) return RBMD_OK;
-    // Eager steps run on the caller's stream.  Graph replay needs a capturable
-    // (non-null) stream: with a null caller stream the handle's own stream is
-    // used, ordered after the caller's earlier work through an event.
-    hipStream_t cs = (hipStream_t)stream;
-    hipStream_t s = cs;
-    if (use_graph && !cs) {
-        s = E.own_stream;
-        HIP_TRY(hipEventRecord(E.order_event, cs));
-        HIP_TRY(hipStreamWaitEvent(s, E.order_event, 0));
-    }
+    // Every launch and graph replay goes to the caller's stream (the null stream
+    // included).  Capture needs a non-null stream: with a null caller stream the
+    // graphs are captured on the handle's own stream (capture runs nothing, so no
+    // ordering is needed) and replayed on the null stream.
+    hipStream_t s = (hipStream_t)stream;
+    hipStream_t cap = s ? s : E.own_stream;
     int rc = maintain(E, s);
     if (rc != RBMD_OK) return rc;
     // bin the current positions for a rebuild in the first step (later steps'
@@ -892,9 +888,9 @@ int rbmd_md_run(rbmd_force_t h, double* x, double* y, double* z, double* vx, dou
             G.exec = nullptr;
             G.graph = nullptr;
             E.capturing = true;
-            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            md_segment(E, rs, dt, seed, nullptr, steps, s);
-            hipError_t st = hipStreamEndCapture(s, &G.graph);
+            HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            md_segment(E, rs, dt, seed, nullptr, steps, cap);
+            hipError_t st = hipStreamEndCapture(cap, &G.graph);
             E.capturing = false;
             if (st != hipSuccess) {
                 E.err = std::string("graph capture failed: ") + hipGetErrorString(st);
@@ -1466,13 +1462,10 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
                                                 ": thermostat without degrees of freedom");
     }
     if (nsteps <= 0) return RBMD_OK;
-    hipStream_t cs = (hipStream_t)stream;
-    hipStream_t s = cs;
-    if (use_graph && cs == nullptr) {
-        s = b->own_stream;
-        BHIP_TRY(hipEventRecord(b->order_event, cs));
-        BHIP_TRY(hipStreamWaitEvent(s, b->order_event, 0));
-    }
+    // launches and replays on the caller's stream; capture on a non-null stream
+    // (the batch's own when the caller's is null; capture runs nothing)
+    hipStream_t s = (hipStream_t)stream;
+    hipStream_t cap = s ? s : b->own_stream;
     for (size_t r = 0; r < R; ++r) {
         int rc = maintain(*B.es[r], s);
         if (rc != RBMD_OK) return bfail(b, rc, "replica " + std::to_string(r) + ": " + B.es[r]->err);
@@ -1535,9 +1528,9 @@ int rbmd_batch_md_run(rbmd_batch_t b, const rbmd_replica_state* st, double dt, i
             if (G.graph) hipGraphDestroy(G.graph);
             G.exec = nullptr;
             G.graph = nullptr;
-            BHIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            batch_segment(b, steps, s);
-            hipError_t e = hipStreamEndCapture(s, &G.graph);
+            BHIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            batch_segment(b, steps, cap);
+            hipError_t e = hipStreamEndCapture(cap, &G.graph);
             if (e != hipSuccess)
                 return bfail(b, RBMD_ERR_HIP, std::string("graph capture failed: ") +
                                                   hipGetErrorString(e));

@@ -468,7 +468,10 @@ void launch_build(Engine& e, const double* x, const double* y, const double* z, 
 void launch_md_bin(Engine& e, const double* x, const double* y, const double* z,
                    unsigned long long step0, hipStream_t s)
 {
-    (void)hipMemsetAsync(e.md_count, 0, md_count_bytes(e.ncell), s);
+    // the counts cleared by a kernel of the library (a runtime fill costs the host
+    // about twice the launch time)
+    const int nc = (int)(md_count_bytes(e.ncell) / sizeof(int32_t));
+    k_one<ClearInts><<<nblk(nc), kBlock, 0, s>>>(ClearInts::Args{e.md_count, nc});
     k_one<MdBin><<<nblk(e.n > 0 ? e.n : 1), kBlock, 0, s>>>(mdbin_args(e, x, y, z, step0));
     if (md_nchunk(e.ncell) > 0)
         k_one<ChunkSum><<<md_nchunk(e.ncell), ChunkSum::kThreads, 0, s>>>(
