@@ -11,7 +11,7 @@ is compared with the analytic sampler at P = 50/100/200/400: force RMS error
 against FP64 full Ewald (5 held-out configurations x 20 k-lists each) and
 atom-steps/s of each.  The headline value is the learned P = 50 run.  A step =
 kick, drift, list check (and rebuild when needed), k-sampling, structure factor,
-pair pass, k-space force, kick -- captured in HIP graphs (sixteen steps and the
+pair pass, k-space force, kick -- captured in HIP graphs (32 steps and the
 run's remainder) before the timed region and replayed.
 
 Prints ONE JSON line on stdout (logs go to stderr).  Also reported:
@@ -174,7 +174,7 @@ class State:
 # Steps per cached HIP graph in the library: a K-step run replays the GRAPH_STEPS-step
 # graph K // GRAPH_STEPS times and the graph of the K % GRAPH_STEPS remainder once,
 # each captured on first use.
-GRAPH_STEPS = 16
+GRAPH_STEPS = 32
 
 
 def graph_warmup(W, K):

@@ -44,7 +44,7 @@ struct SampleArgs {
 // (about 4 us more than a fused step), so segments are long: a run of K steps
 // replays the kGraphSteps-step graph K / kGraphSteps times and one graph of the
 // remaining K % kGraphSteps steps.
-constexpr int kGraphSteps = 16;
+constexpr int kGraphSteps = 32;
 
 struct Engine {
     // ---- parameters

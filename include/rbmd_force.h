@@ -152,8 +152,8 @@ int rbmd_force_get_energy_virial(rbmd_force_t h, rbmd_energy_virial* out, void* 
  * x,y,z,vx,vy,vz FP64 (n); f* as in compute and holding the forces of the
  * current positions on entry; img int32 (3n, xyz interleaved) or NULL;
  * mass FP64 (n) or NULL (=1).  With RBE, k-vectors are resampled every step
- * from (seed, step0 + s).  use_graph != 0 captures sixteen steps (and the
- * nsteps % 16 remainder) in HIP graphs, cached per handle, and replays them;
+ * from (seed, step0 + s).  use_graph != 0 captures 32 steps (and the
+ * nsteps % 32 remainder) in HIP graphs, cached per handle, and replays them;
  * graph and eager runs give the same bits.  A list rebuild takes its reference
  * positions from the binning one step earlier, so the lists are exact while no
  * atom moves more than skin/2 in one step; a run in which one did returns
@@ -319,7 +319,7 @@ int rbmd_util_copy(void* dst, const void* src, int64_t nbytes, void* stream);
  * the velocity-Verlet step (rho(k) partials + cell sort, rho(k) final + tile
  * pass, k-space force + kicks + drift + k-draw) is ONE launch over all
  * replicas, driven by a table of (replica, workgroup) work items with
- * per-replica arguments in device memory, and sixteen steps (and a run's
+ * per-replica arguments in device memory, and 32 steps (and a run's
  * remainder) are captured once in a HIP graph.  Each replica is an
  * ordinary force handle (rbmd_batch_replica) with the ensemble loop of
  * ExecutionNVT::Solve (rbmd/framework/run/ExecutionNVT.cpp:133-159) per replica;

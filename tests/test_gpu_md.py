@@ -269,14 +269,14 @@ def _oracle_thermo_run(orc, x, y, z, q, vx, vy, vz, Lb, alpha, steps, dt, kforce
 
 @pytest.mark.parametrize("kind", ["BERENDSEN", "TEMP_RESCALE", "NOSE_HOOVER", "LANGEVIN"])
 def test_md_thermostat_matches_oracle(orc, gpu, kind):
-    """FP64 Ewald NVT run of 30 steps (a 16-step graph and a 14-step remainder
+    """FP64 Ewald NVT run of 40 steps (a 32-step graph and an 8-step remainder
     graph) with per-atom masses and list rebuilds, against the oracle's loop in the
     reference's order: positions, velocities, image flags, the temperature of every
     step, and the Nose-Hoover friction."""
     from copy_gpu_amd import systems
 
-    _log(f"case: NVT {kind}, FP64 Ewald, 1500 ions, 30 steps")
-    n, dt, steps, alpha, tol = 1500, 0.002, 30, 0.5, 1e-12
+    _log(f"case: NVT {kind}, FP64 Ewald, 1500 ions, 40 steps")
+    n, dt, steps, alpha, tol = 1500, 0.002, 40, 0.5, 1e-12
     T0, tdamp, gamma, kbt, nseed, ns0 = 1.1, 0.05, 100.0, 1.0, 77, 5
     x, y, z, q, L = systems.random_ions(n, 0.3, 0.8, seed=61)
     vx, vy, vz = systems.velocities(n, 1.0, seed=61)

@@ -505,7 +505,7 @@ def _state(n, seed=SEED):
 
 
 def test_md_graph_equals_eager(gpu):
-    _log("case: 30 MD steps, graph replay vs eager launches")
+    _log("case: 40 MD steps (a 32-step graph and an 8-step one), graph replay vs eager")
     x, y, z, q, L, vx, vy, vz = _state(4000)
     Lb = np.array([L] * 3)
     outs = []
@@ -517,7 +517,7 @@ def test_md_graph_equals_eager(gpu):
         F = [torch.zeros(x.size, device="cuda", dtype=torch.float32) for _ in range(3)]
         f.Execute(X, Y, Z, *F)
         img = torch.zeros(3 * x.size, device="cuda", dtype=torch.int32)
-        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=0.002, nsteps=30, seed=SEED, step0=1, img=img,
+        f.md_run(X, Y, Z, VX, VY, VZ, *F, dt=0.002, nsteps=40, seed=SEED, step0=1, img=img,
                  use_graph=use_graph)
         torch.cuda.synchronize()
         outs.append([t.cpu().numpy() for t in (X, Y, Z, VX, VY, VZ, *F, img)])
