@@ -230,3 +230,24 @@ def test_batch_autotune_then_run(gpu):
         Fc = torch.stack(Fc, 1).double()
         assert float(torch.linalg.norm(F - Fc) / torch.linalg.norm(Fc)) < 1e-5
     b.close()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_step_longer_than_half_skin_names_the_replica(gpu, use_graph):
+    """A batched run ends with one gathered copy of every replica's statistics and
+    flags; the long-step check reaches the caller through it: one atom of replica 1
+    moving 0.2 in a 0.002 step (half the 0.3 skin is 0.15) makes the run report
+    replica 1, and a batch of the same replicas without it runs clean."""
+    from copy_gpu_amd import RBMDError
+
+    _log(f"case: batch long-step flag, 2 replicas, graph={use_graph}")
+    syss = _systems([3000, 4000], seed0=300)
+    outs, _ = _run_batch(syss, [5, 6], 6, "RBE", 100, use_graph)
+    assert all(np.isfinite(o[0]).all() for o in outs)
+    x, y, z, q, L, vx, vy, vz = syss[1]
+    vx = vx.copy()
+    vx[7] = 100.0
+    syss[1] = (x, y, z, q, L, vx, vy, vz)
+    with pytest.raises(RBMDError, match=r"replica 1\)"):
+        _run_batch(syss, [5, 6], 6, "RBE", 100, use_graph)
+    torch.cuda.synchronize()
