@@ -608,7 +608,10 @@ def roofline_of(f, st, wl, wname, K, step, measured_limiter):
 
 
 PAIR_LIMITER = ("VALU issue, LDS gathers and latency together (the bytes moved equal the "
-                "algorithmic bytes, so the pass does not reach the HBM roofline)")
+                "algorithmic bytes, so the pass does not reach the HBM roofline; at 100,000 "
+                "ions its waves spend 49% of their time waiting on LDS or memory counters, "
+                "23% ready but not issued and 28% issuing, with the VALU busy about 75% of "
+                "the launch: profiles/pair_c4_pmc.md)")
 
 
 def plain_leg(wname, args, W, K, use_graph, with_nvt=False):
