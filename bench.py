@@ -292,7 +292,7 @@ def pair_pass_at_scale(wname, reps=20):
     return out
 
 
-def ewald_refs(wl, L, Q, configs):
+def ewald_refs(wl, L, Q, snaps):
     """FP64 full-Ewald total forces and the Ewald k-space part for each configuration."""
     import torch
 
@@ -302,7 +302,7 @@ def ewald_refs(wl, L, Q, configs):
     nk = make_force(wl, L, 0, kspace="NONE", precision=64)
     nk.Init(Q)
     refs = []
-    for (X, Y, Z) in configs:
+    for (X, Y, Z) in snaps:
         Fe = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
         Fr = [torch.zeros(n, device="cuda", dtype=torch.float64) for _ in range(3)]
         ew.Execute(X, Y, Z, *Fe)
@@ -315,16 +315,16 @@ def ewald_refs(wl, L, Q, configs):
     return refs, nkv
 
 
-def force_error(f, P, configs, refs, lists):
+def force_error(f, P, snaps, refs, lists):
     """eps_F over configurations x k-lists: RMS of F - F_Ewald relative to the
     total Ewald force and to its k-space part (same real-space part, so the
     difference is the k-space estimator's error)."""
     import torch
 
     num = den = denk = 0.0
-    n = configs[0][0].numel()
+    n = snaps[0][0].numel()
     F = [torch.zeros(n, device="cuda", dtype=torch.float32) for _ in range(3)]
-    for c, ((X, Y, Z), (Fe, Fek)) in enumerate(zip(configs, refs)):
+    for c, ((X, Y, Z), (Fe, Fek)) in enumerate(zip(snaps, refs)):
         for s in range(lists):
             f.sample_kvectors(P, SEED + 1, 1000 * c + s)
             f.Execute(X, Y, Z, *F)
@@ -380,15 +380,15 @@ def learned_eval(wl, L, Q, st, step, args):
         f"optimal k-only proposal {g_opt:.4f}")
 
     # held-out configurations: 5 more snapshots, 20 steps apart
-    configs = []
+    snaps = []
     for i in range(5):
         st.run(f, wl, 20, step)
         step += 20
-        configs.append((st.X.clone(), st.Y.clone(), st.Z.clone()))
+        snaps.append((st.X.clone(), st.Y.clone(), st.Z.clone()))
     f.close()
-    refs, nkv = ([], 0) if args.skip_ewald else ewald_refs(wl, L, Q, configs)
+    refs, nkv = ([], 0) if args.skip_ewald else ewald_refs(wl, L, Q, snaps)
     if refs:
-        log(f"FP64 Ewald references: {len(configs)} configurations, {nkv} half-space k-vectors")
+        log(f"FP64 Ewald references: {len(snaps)} configurations, {nkv} half-space k-vectors")
 
     runs = []
     K = args.steps if args.steps is not None else wl["steps"]
@@ -401,7 +401,7 @@ def learned_eval(wl, L, Q, st, step, args):
             prop.attach(g)
         eps = epsk = None
         if refs:
-            eps, epsk = force_error(g, P, configs, refs, 20)
+            eps, epsk = force_error(g, P, snaps, refs, 20)
         s2 = base.clone()
         g.sample_kvectors(P, SEED, step)
         g.Execute(s2.X, s2.Y, s2.Z, *s2.F)
@@ -701,8 +701,8 @@ def main():
     global HALF_SHELL
     HALF_SHELL = args.half_shell
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU "
-              "decomposition is not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the passes "
+              "are not decomposed over several GPUs", file=sys.stderr)
         sys.exit(2)
 
     import torch

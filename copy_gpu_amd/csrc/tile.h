@@ -418,7 +418,7 @@ __device__ inline void list_wave(const TileHdr& h, const RefStage& rs,
         uint16_t* __restrict__ row = nlist + (size_t)s * cap;
         int n_in = n;
         if (RBL && n < cap) {
-            // random batch list: the near band, and the shell pairs this build
+            // random batch list: the near band, and the shell pairs this list build
             // selects (tagged kRblSel), in sweep order; the pair pass evaluates LJ on
             // every entry
             __syncwarp();

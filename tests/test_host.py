@@ -121,3 +121,14 @@ def test_langevin_noise_stream():
     assert np.abs(C - np.eye(3)).max() < 0.04
     # the stream depends on the seed
     assert not np.array_equal(systems.langevin_noise(11, 3), systems.langevin_noise(12, 3))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 ends with a message and no JSON line."""
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""
