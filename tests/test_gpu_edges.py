@@ -108,3 +108,37 @@ def test_two_ions_closed_form(gpu, precision, k):
     assert abs(ev["e_real"] - e_re) <= rel * abs(e_re) + erfc_abs / r
     assert np.abs(ev["virial"] - W).max() <= rel * np.abs(W).max() + erfc_abs / r, (
         ev["virial"], W)
+
+
+def test_more_atoms_than_capacity_is_an_error(gpu):
+    from copy_gpu_amd import RBMDError
+
+    _log("case: set_atoms beyond n_max (error expected)")
+    f = _make(4, 64)
+    with pytest.raises(RBMDError, match="n_max"):
+        f.Init(_dev(np.ones(5)))
+    f.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_lone_ion_feels_no_force(gpu, precision):
+    """A single charge in the periodic box: no pair partner, and its k-space force
+    (the sum over k of the ion's own structure factor against its own phase) vanishes,
+    so no self-force leaks through rho(k)."""
+    from copy_gpu_amd import ForceConfig, LJCutCoulKspaceForce
+
+    _log(f"case: one ion, RBE P = 200, precision {precision}")
+    cfg = ForceConfig(box_len=(L, L, L), cut_off=RC_LJ, coulomb_cut=RC_COUL, skin=0.3,
+                      alpha=ALPHA, kspace="RBE", coulomb_sample_num=200, precision=precision,
+                      n_max=1)
+    f = LJCutCoulKspaceForce(cfg)
+    f.Init(_dev([2.0]))
+    f.sample_kvectors(200, 7, 0)
+    F = [torch.zeros(1, device="cuda", dtype=f.fdtype) for _ in range(3)]
+    f.Execute(_dev([3.3]), _dev([11.7]), _dev([17.2]), *F)
+    torch.cuda.synchronize()
+    got = np.array([float(t.cpu()[0]) for t in F])
+    f.close()
+    # one k-vector's term has magnitude of order q^2 S/(P V) |k| <= 4 / 8000 * 2.4; FP32
+    # phases leave a cancellation residue of a few ulp of that per term
+    assert np.abs(got).max() <= (1e-12 if precision == 64 else 1e-6), got
